@@ -1,0 +1,177 @@
+"""Benchmark: full Barnes-Hut step throughput (body-steps/s) on one MI355X.
+
+Workload (the reference's maximum scale, 800k bodies): 2D exponential disk,
+radial scale length 0.2, circular velocities from the enclosed mass plus
+N(0, 0.05 v_c) dispersion, lognormal(sigma=0.5) masses normalised to 1,
+G=1, constant softening eps=1e-3, theta=0.7, dt=5e-5, 30-bit Morton keys
+over the per-step bounding box, seed 1.  A step is the whole pipeline of
+src/simulation.ts:393-421: bbox + Morton encode, radix sort, LBVH build,
+mass/COM fill, tree walk with kick, drift.
+
+Timed region: K steps replayed from a captured HIP graph with the bodies
+already resident in HBM, bracketed by stream synchronisation.  Afterwards,
+untimed: per-phase device times from HIP events on the engine's stream
+(eager launches), walk counters for the flop model, and the CPU baseline
+(the oracle's OpenMP restatement of the same step on a bounded sample).
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "webgpu-fluid-sim_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "body-steps/s, full BH step, N=800k θ=0.7; ensemble & adjoint vs plain"
+N_BODIES = 800_000
+THETA = 0.7
+DT = 5e-5
+EPS = 1e-3
+SEED = 1
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+FLOPS_ACCEPT = 18          # per accepted interaction (constant softening, rsqrt = 1)
+FLOPS_OPEN = 9             # per opened node
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload():
+    import nbody_sim
+
+    mass, pos, vel = nbody_sim.generate("exp-disk", N_BODIES, seed=SEED, scale=0.2,
+                                        dispersion=0.05, mass_sigma=0.5)
+    cfg = nbody_sim.Config(gravConstant=1.0, bhTheta=THETA, deltaTime=DT, epsilon=EPS,
+                           softeningMode="constant", mortonMode="bbox30", mathMode="fast")
+    return cfg, mass, pos, vel
+
+
+def cpu_baseline(mass, pos, vel, budget_s=15.0):
+    """The oracle's restatement of one full step, OpenMP over the host cores,
+    repeated until ~budget_s of work (at least one step)."""
+    from oracle import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    P = O.Params(grav_constant=1.0, delta_time=DT, bh_theta=THETA, epsilon=EPS,
+                 softening_mode=O.SOFTEN_CONSTANT, morton_mode=O.MORTON_BBOX30)
+    steps = 0
+    t0 = time.perf_counter()
+    p, v = pos, vel
+    while True:
+        p, v = O.step(mass, p, v, P, nsteps=1, threads=threads)
+        steps += 1
+        el = time.perf_counter() - t0
+        log(f"cpu baseline step {steps}: {el:.1f}s")
+        if el >= budget_s:
+            break
+    return {"value": N_BODIES * steps / el, "unit": "body-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} full step(s) of the same 800k-body workload from its initial "
+                      f"state, oracle C restatement, OpenMP {threads} threads, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path runs on a single MI355X).", file=sys.stderr)
+        sys.exit(2)
+
+    import nbody_sim
+
+    cfg, mass, pos, vel = workload()
+    log(f"workload generated: n={N_BODIES}")
+    sim = nbody_sim.Simulation(cfg, device=0)
+    sim.setBodies(mass, pos, vel)
+    sim.useGraph(True)
+    sim.step(args.warmup)
+    sim.synchronize()
+    log(f"warmup {args.warmup} steps done")
+
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = 1e3 * el / args.steps
+    value = N_BODIES * args.steps / el
+    log(f"timed {args.steps} steps: {el:.3f}s, {ms_per_step:.4f} ms/step")
+
+    # Untimed: per-phase event timing and walk counters over the same steps.
+    _, p_state, v_state = sim.getBodies()
+    sim.useGraph(False)
+    sim.setProfiling(True)
+    sim.phaseTimes(reset=True)
+    sim.step(args.profile_steps)
+    sim.synchronize()
+    phases, psteps = sim.phaseTimes(reset=True)
+    sim.setProfiling(False)
+    sim.setBodies(mass, p_state, v_state)
+    sim.setDebug(stats=True)
+    sim.walkStats(reset=True)
+    sim.step(args.profile_steps)
+    sim.synchronize()
+    st = sim.walkStats(reset=True)
+    sim.setDebug()
+    sim.close()
+    phase_avg = {k: v / psteps for k, v in phases.items()}
+    walk_ms = phase_avg["walk"]
+    flops = (FLOPS_ACCEPT * st["accepted"] + FLOPS_OPEN * st["opened"]) / args.profile_steps
+    achieved = flops / (walk_ms * 1e-3) / 1e12
+    log(f"phases ms/step: {phase_avg}; walk flops/launch {flops:.3e}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain BH step, 2D exponential disk N=800000, theta=0.7, "
+                               "eps=1e-3, dt=5e-5, bbox30 keys, seed 1",
+                   "n_bodies": N_BODIES, "theta": THETA, "dt": DT, "eps": EPS,
+                   "math_mode": "fast", "graph": True},
+        "phases_ms": {k: round(v, 5) for k, v in phase_avg.items()},
+        "roofline": {
+            "kernel": "walk_kernel (tree walk + kick + drift)",
+            "bound": "valu-fp32",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "flops_per_launch": flops,
+            "walk_ms": walk_ms,
+            "interactions_per_step": st["accepted"] / args.profile_steps,
+            "opened_per_step": st["opened"] / args.profile_steps,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mass, pos, vel, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

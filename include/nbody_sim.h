@@ -1,0 +1,183 @@
+/*
+ * nbody_sim.h -- C ABI of the MI355X Barnes-Hut step (libnbody_sim.so).
+ *
+ * Drop-in boundary for the compute side of the reference's `Simulation`
+ * class (src/simulation.ts:39-433) and its `Config` (src/config.ts:1-22).
+ * Each entry point names the reference member it replaces.  Conventions:
+ *   - every function returns NBODY_OK (0) or a negative NBODY_ERR_* code and
+ *     never throws; nbody_last_error() gives the message;
+ *   - host arrays are caller-owned; device memory is owned by the handle
+ *     (as the reference's Simulation owns its GPUBuffers, simulation.ts:68-104);
+ *   - pos/vel use the reference's host layout: float32 [x0,y0,x1,y1,...]
+ *     (the vec2f buffers of simulation.ts:237-238, 376-380);
+ *   - one handle is used by one host thread at a time; work is ordered on
+ *     the handle's HIP stream (the reference orders it by queue submission,
+ *     main.ts:38-41).
+ */
+#ifndef NBODY_SIM_H
+#define NBODY_SIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBODY_OK 0
+#define NBODY_ERR_INVALID_ARG (-1)
+#define NBODY_ERR_HIP (-2)
+#define NBODY_ERR_CAPACITY (-3)
+#define NBODY_ERR_STACK_OVERFLOW (-4)
+#define NBODY_ERR_STATE (-5)
+#define NBODY_ERR_NO_DEVICE (-6)
+
+/* Softening: NBODY_SOFTEN_REFERENCE is barnes_hut.wgsl:90-91,
+ * eps = epsilon_multiplier * sqrt(G * max(m1, m2) * dt); NBODY_SOFTEN_CONSTANT
+ * uses `epsilon` for every pair. */
+#define NBODY_SOFTEN_REFERENCE 0
+#define NBODY_SOFTEN_CONSTANT 1
+
+/* Morton keys: NBODY_MORTON_REFERENCE is lbvh.wgsl:42-69 (fixed
+ * [-1000,1000]^2 box, 16 bits per axis); NBODY_MORTON_BBOX30 quantises
+ * 15 bits per axis over each step's square bounding box. */
+#define NBODY_MORTON_REFERENCE 0
+#define NBODY_MORTON_BBOX30 1
+
+/* Walk arithmetic: FAST uses the hardware reciprocal square root and fused
+ * multiply-adds; EXACT evaluates 1/sqrt and every product and sum as single
+ * correctly rounded binary32 operations in the reference's order. */
+#define NBODY_MATH_FAST 0
+#define NBODY_MATH_EXACT 1
+
+/* Replaces `Config` (config.ts:1-7) plus the `Metadata` uniform that
+ * updateMetadataBuffer() packs (simulation.ts:212-225, lbvh.wgsl:3-10). */
+typedef struct nbody_config {
+    float grav_constant;      /* Config.gravConstant      (config.ts:2)  */
+    float bh_theta;           /* Config.bhTheta           (config.ts:3)  */
+    int32_t substeps;         /* Config.substeps          (config.ts:4)  */
+    float delta_time;         /* Config.deltaTime         (config.ts:5)  */
+    float epsilon_multiplier; /* Config.epsilonMultiplier (config.ts:6)  */
+    float epsilon;            /* softening length for NBODY_SOFTEN_CONSTANT */
+    int32_t softening_mode;   /* NBODY_SOFTEN_*  */
+    int32_t morton_mode;      /* NBODY_MORTON_*  */
+    int32_t math_mode;        /* NBODY_MATH_*    */
+    uint32_t offset;          /* Metadata.offset (simulation.ts:222, lbvh.wgsl:171); the
+                                 race-free fill gives the same tree for every value */
+} nbody_config;
+
+typedef struct nbody_sim nbody_sim;
+
+/* getDefaultConfig() (config.ts:9-22): G=1, theta=0.6, substeps=1,
+ * dt=0.1/60, epsilonMultiplier=1, reference softening and keys. */
+int nbody_default_config(nbody_config *out);
+
+/* new Simulation(config, device) (simulation.ts:55-66).  Allocates device
+ * storage for `capacity` bodies on HIP device `device` (grown on demand by
+ * nbody_set_bodies).  *err (may be NULL) receives the status. */
+nbody_sim *nbody_create(const nbody_config *cfg, int64_t capacity, int32_t device, int *err);
+
+/* Ensemble handle: `num_sims` independent simulations with n_per_sim[s]
+ * bodies each, stepped by one sort, one tree build and one walk per step.
+ * Bodies are passed concatenated in sim order.  Keys are bbox30 codes per
+ * sim prefixed by the sim id (num_sims <= 256).  No reference counterpart:
+ * the reference steps one simulation (simulation.ts:39). */
+nbody_sim *nbody_ensemble_create(const nbody_config *cfg, int32_t num_sims,
+                                 const int64_t *n_per_sim, int32_t device, int *err);
+
+int nbody_destroy(nbody_sim *sim);
+
+/* updateMetadataBuffer() (simulation.ts:212-225): replace the parameters. */
+int nbody_set_config(nbody_sim *sim, const nbody_config *cfg);
+int nbody_get_config(const nbody_sim *sim, nbody_config *out);
+
+/* setNumBodies(n) + updatePhysicsBuffers() (simulation.ts:227-231, 376-380):
+ * upload n bodies.  For an ensemble handle n must equal the total body count. */
+int nbody_set_bodies(nbody_sim *sim, int64_t n, const float *mass, const float *pos_xy,
+                     const float *vel_xy);
+
+/* getBuffers() (simulation.ts:427-429), read back to host.  Any pointer may
+ * be NULL.  Synchronises the handle's stream. */
+int nbody_get_bodies(nbody_sim *sim, float *mass, float *pos_xy, float *vel_xy);
+
+/* getNumBodies() (simulation.ts:430-432). */
+int64_t nbody_get_num_bodies(const nbody_sim *sim);
+
+/* getCommands() + device.queue.submit() (simulation.ts:386-425, main.ts:38-41):
+ * enqueue `nsteps` full steps (Morton encode, sort, tree build, fill, walk
+ * with kick, drift) on the handle's stream.  Asynchronous. */
+int nbody_step(nbody_sim *sim, int32_t nsteps);
+
+/* Wait for queued work; reports a walk stack overflow seen since the last call. */
+int nbody_synchronize(nbody_sim *sim);
+
+/* Run nbody_step's work on `hip_stream` (a hipStream_t; NULL = the handle's own). */
+int nbody_set_stream(nbody_sim *sim, void *hip_stream);
+
+/* Replay steps from a captured HIP graph (1) or launch kernels one by one (0). */
+int nbody_use_graph(nbody_sim *sim, int32_t enable);
+
+/* Debug flags for later steps: NBODY_DEBUG_ACCEL records each body's
+ * acceleration; NBODY_DEBUG_STATS counts accepted and opened nodes. */
+#define NBODY_DEBUG_ACCEL 1
+#define NBODY_DEBUG_STATS 2
+int nbody_set_debug(nbody_sim *sim, int32_t flags);
+
+/* Intermediate buffers of the last step (reference buffers in comments). */
+#define NBODY_BUF_KEYS 1         /* sorted Morton keys: u32[n] (u64[n] for ensembles); mortonCodes */
+#define NBODY_BUF_BODY_INDICES 2 /* u32[n] sorted position -> body; bodyIndices */
+#define NBODY_BUF_CHILDREN 3     /* u32[2*(n-1)] {left,right} per internal node; NodeData.left/right */
+#define NBODY_BUF_PARENT 4       /* u32[2n-1]; NodeData.parent */
+#define NBODY_BUF_NODE_CM 5      /* f32[4*(2n-1)] {com.x, com.y, total_mass, length} */
+#define NBODY_BUF_NODE_AABB 6    /* f32[4*(2n-1)] {min.x, min.y, max.x, max.y} */
+#define NBODY_BUF_ACCEL 7        /* f32[2n] per body (needs NBODY_DEBUG_ACCEL) */
+#define NBODY_BUF_BBOX 8         /* f32[4*num_sims] bounding box of the last step */
+#define NBODY_BUF_SIM_ROOTS 9    /* u32[num_sims] sub-tree root of each sim */
+int nbody_debug_get(nbody_sim *sim, int32_t which, void *dst, int64_t bytes);
+
+/* Walk counters since the last reset: {accepted, opened, overflowed}. */
+int nbody_walk_stats(nbody_sim *sim, uint64_t *out3, int32_t reset);
+
+/* Per-phase device time (HIP events on the handle's stream) while profiling
+ * is enabled; phases: 0 bbox, 1 encode, 2 sort, 3 build, 4 fill, 5 walk+kick+drift.
+ * ms_out[6] receives the sums since the last reset, *steps the step count. */
+int nbody_set_profiling(nbody_sim *sim, int32_t enable);
+int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t reset);
+
+/* Tunables of the launch (0 keeps the current value): walk workgroup size
+ * (64..1024, multiple of 64) and walk stack entries per wave (8..256). */
+int nbody_set_tuning(nbody_sim *sim, int32_t walk_block, int32_t walk_stack);
+
+/* Message of the last failure on `sim` (NULL: the last failure of a call
+ * without a usable handle).  Never NULL. */
+const char *nbody_last_error(const nbody_sim *sim);
+
+/* Initial conditions.  Scenarios 0-2 are setScenario()'s generators
+ * (simulation.ts:243-362) with Math.random replaced by Philox-4x32-10;
+ * 3-6 are the seeded workloads of the benchmark configurations. */
+#define NBODY_SCENARIO_DEFAULT 0        /* simulation.ts:243-266 */
+#define NBODY_SCENARIO_TWO_CLUSTERS 1   /* simulation.ts:267-306 */
+#define NBODY_SCENARIO_PLANETARY_DISK 2 /* simulation.ts:308-361 */
+#define NBODY_SCENARIO_UNIFORM_DISK 3   /* unit disk, v=0, m=1/n */
+#define NBODY_SCENARIO_EXP_DISK 4       /* exponential disk, circular v + dispersion, lognormal m */
+#define NBODY_SCENARIO_COLLISION 5      /* two exponential disks on a collision course */
+#define NBODY_SCENARIO_PLUMMER 6        /* projected 2D Plummer, v=0, m=1/n */
+typedef struct nbody_ic_params {
+    float grav_constant; /* used by the circular-velocity scenarios */
+    float scale;         /* disk scale length / Plummer radius (0 = scenario default) */
+    float dispersion;    /* velocity dispersion as a fraction of v_circ (EXP_DISK) */
+    float mass_sigma;    /* lognormal sigma of masses (EXP_DISK; 0 = equal masses) */
+} nbody_ic_params;
+int nbody_generate(int32_t scenario, int64_t n, uint64_t seed, const nbody_ic_params *params,
+                   float *mass, float *pos_xy, float *vel_xy);
+
+/* Philox-4x32-10 (Salmon et al., SC'11) used by nbody_generate. */
+void nbody_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Library version string. */
+const char *nbody_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBODY_SIM_H */

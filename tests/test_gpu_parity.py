@@ -1,0 +1,151 @@
+"""Parity of the HIP step (through the C ABI) with the CPU oracle.
+
+Bit-exact: sorted Morton keys, the sorted permutation (ties by body index),
+the LBVH child/parent arrays, node mass/COM/AABB/length, and -- in the exact
+arithmetic mode -- accelerations and the kicked/drifted state.  The fast mode
+(hardware rsqrt + FMA) is held to 1e-5 relative per body, with the bodies
+whose opening decision flips at the theta boundary counted separately.
+"""
+import numpy as np
+import pytest
+
+import nbody_sim
+from helpers import gpu_step, oracle_step, params, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_tree(g, r):
+    n = g["keys"].shape[0]
+    assert np.array_equal(g["keys"], r["keys"]), "sorted keys"
+    assert np.array_equal(g["body_indices"], r["idx"]), "permutation"
+    assert np.array_equal(g["children"][:, 0], r["left"]), "left children"
+    assert np.array_equal(g["children"][:, 1], r["right"]), "right children"
+    assert np.array_equal(g["parent"][1:], r["parent"][1:]), "parents"
+    if n > 1:
+        assert g["parent"][0] == 0xFFFFFFFF
+    assert np.array_equal(g["node_cm"], r["node_cm"]), "node com/mass/length"
+    assert np.array_equal(g["node_aabb"], r["node_aabb"]), "node aabb"
+
+
+@pytest.mark.parametrize("scenario,n,seed", [
+    ("uniform-disk", 16384, 1),   # the small plain config's inputs
+    ("exp-disk", 20000, 2),
+    ("plummer", 5000, 3),
+    ("uniform-disk", 1000, 4),
+])
+def test_step_bit_exact_bbox30(scenario, n, seed):
+    mass, pos, vel = nbody_sim.generate(scenario, n, seed=seed, dispersion=0.05, mass_sigma=0.5)
+    cfg, P = params(theta=0.5, dt=1e-4, eps=1e-3)
+    g = gpu_step(cfg, mass, pos, vel)
+    r = oracle_step(P, mass, pos, vel)
+    _compare_tree(g, r)
+    assert np.array_equal(g["bbox"][0], r["bbox"])
+    assert np.array_equal(g["accel"], r["accel"]), "accelerations (exact mode)"
+    assert np.array_equal(g["vel"], r["vel"]) and np.array_equal(g["pos"], r["pos"])
+
+
+@pytest.mark.parametrize("scenario", ["default", "two-clusters", "planetary-disk"])
+def test_reference_mode_scenarios(scenario):
+    """Reference keys (fixed box) and reference softening on the reference's scenarios."""
+    n = 8192
+    mass, pos, vel = nbody_sim.generate(scenario, n, seed=7)
+    cfg, P = params(theta=0.6, dt=0.1 / 60.0, soft="reference", morton="reference")
+    g = gpu_step(cfg, mass, pos, vel)
+    r = oracle_step(P, mass, pos, vel)
+    _compare_tree(g, r)
+    assert np.array_equal(g["accel"], r["accel"])
+    assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
+
+
+def test_reference_keys_heavy_duplicates():
+    """Unit-scale bodies under the fixed [-1000,1000] box collapse onto few
+    cells: the sort's stability and the Karras index tie-break decide the tree."""
+    mass, pos, vel = nbody_sim.generate("uniform-disk", 5000, seed=11)
+    cfg, P = params(theta=0.5, soft="constant", morton="reference")
+    g = gpu_step(cfg, mass, pos, vel)
+    r = oracle_step(P, mass, pos, vel)
+    assert len(np.unique(r["keys"])) < 200
+    _compare_tree(g, r)
+    assert np.array_equal(g["accel"], r["accel"])
+
+
+def test_clamped_and_out_of_box_keys():
+    rng = np.random.default_rng(5)
+    n = 3000
+    pos = rng.uniform(-1500, 1500, (n, 2)).astype(np.float32)
+    mass = np.full(n, 1.0 / n, np.float32)
+    vel = np.zeros_like(pos)
+    cfg, P = params(morton="reference", theta=0.7)
+    g = gpu_step(cfg, mass, pos, vel)
+    r = oracle_step(P, mass, pos, vel)
+    _compare_tree(g, r)
+
+
+def test_multistep_trajectory_bit_exact():
+    mass, pos, vel = nbody_sim.generate("exp-disk", 4096, seed=3, dispersion=0.05, mass_sigma=0.5)
+    cfg, P = params(theta=0.7, dt=5e-5, eps=1e-3)
+    g = gpu_step(cfg, mass, pos, vel, nsteps=10)
+    r = oracle_step(P, mass, pos, vel, nsteps=10)
+    _compare_tree(g, r)
+    assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
+
+
+def test_graph_replay_matches_eager():
+    mass, pos, vel = nbody_sim.generate("uniform-disk", 10000, seed=9)
+    cfg, _ = params(theta=0.5)
+    a = gpu_step(cfg, mass, pos, vel, nsteps=5, math="fast", graph=False)
+    b = gpu_step(cfg, mass, pos, vel, nsteps=5, math="fast", graph=True)
+    for k in ("pos", "vel", "keys", "body_indices", "node_cm"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_fast_mode_within_tolerance():
+    mass, pos, vel = nbody_sim.generate("exp-disk", 20000, seed=4, dispersion=0.05, mass_sigma=0.5)
+    cfg, P = params(theta=0.7, dt=5e-5)
+    g = gpu_step(cfg, mass, pos, vel, math="fast")
+    r = oracle_step(P, mass, pos, vel)
+    _compare_tree(g, r)
+    e = rel_err(g["accel"], r["accel"])
+    # Bodies whose opening test lands within rounding of theta take the other
+    # branch; every other body is within 1e-5 relative.
+    frac_ok = np.mean(e <= 1e-5)
+    print(f"fast mode: median rel {np.median(e):.2e}, frac<=1e-5 {frac_ok:.6f}, max {e.max():.2e}")
+    assert frac_ok >= 0.999
+    assert np.median(e) < 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 257])
+def test_small_sizes(n):
+    rng = np.random.default_rng(n)
+    pos = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    mass = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    vel = rng.normal(0, 0.1, (n, 2)).astype(np.float32)
+    cfg, P = params(theta=0.5, dt=1e-3, eps=1e-2)
+    g = gpu_step(cfg, mass, pos, vel, nsteps=2)
+    r = oracle_step(P, mass, pos, vel, nsteps=2)
+    assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
+    assert np.array_equal(g["accel"], r["accel"])
+
+
+def test_coincident_bodies():
+    n = 500
+    pos = np.zeros((n, 2), np.float32)
+    pos[:250] = 0.25
+    mass = np.full(n, 1.0 / n, np.float32)
+    vel = np.zeros_like(pos)
+    cfg, P = params(theta=0.5, eps=1e-2)
+    g = gpu_step(cfg, mass, pos, vel)
+    r = oracle_step(P, mass, pos, vel)
+    _compare_tree(g, r)
+    assert np.array_equal(g["accel"], r["accel"])
+
+
+def test_stats_match_oracle():
+    mass, pos, vel = nbody_sim.generate("uniform-disk", 8192, seed=2)
+    cfg, P = params(theta=0.5)
+    g = gpu_step(cfg, mass, pos, vel, stats=True)
+    r = oracle_step(P, mass, pos, vel)
+    assert g["stats"]["accepted"] == int(r["stats"][:, 0].sum())
+    assert g["stats"]["opened"] == int(r["stats"][:, 1].sum())
+    assert g["stats"]["overflowed"] == 0

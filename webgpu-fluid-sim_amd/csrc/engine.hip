@@ -1,0 +1,665 @@
+// engine.hip -- the step engine behind the C ABI (include/nbody_sim.h).
+//
+// Owns every device buffer of a simulation (the reference's SimBuffers,
+// src/simulation.ts:8-18, 68-104) in one arena allocation, orders the six
+// phases of a step on one HIP stream (the reference's getCommands(),
+// simulation.ts:386-425), and can replay a step from a captured HIP graph.
+//
+// HBM layout (capacity C bodies, all 256-byte aligned):
+//   mass f32[C], pos float2[C], vel float2[C]      -- by body index
+//   keys[2] u32|u64[C], vals[2] u32[C]             -- sort ping-pong
+//   cm float4[2C-1] {com.x, com.y, mass, length}   -- what the walk reads
+//   aabb float4[2C-1]                              -- what the fill merges
+//   child uint2[C-1], parent u32[2C-1], status u32[C-1]
+//   sort scratch, bbox partials, debug accel float2[C], counters
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nbody_sim.h"
+#include "nbody_internal.h"
+
+using namespace nbs;
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+constexpr int kNumPhases = 6;
+
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    template <typename T>
+    T *take(int64_t count) {
+        size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+        bytes = (bytes + 255) & ~(size_t)255;
+        T *p = reinterpret_cast<T *>(base + used);
+        used += bytes;
+        return p;
+    }
+};
+
+}  // namespace
+
+struct nbody_sim {
+    nbody_config cfg{};
+    int device = 0;
+    bool ensemble = false;
+    int num_sims = 1;
+    std::vector<int64_t> sim_off_host;  // [num_sims + 1]
+    int64_t cap = 0, n = 0;
+    int passes = 4, key_bits = 32;
+
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    Arena arena;
+
+    float *mass = nullptr;
+    float2 *pos = nullptr, *vel = nullptr;
+    void *keys[2] = {nullptr, nullptr};
+    uint32_t *vals[2] = {nullptr, nullptr};
+    float4 *cm = nullptr, *aabb = nullptr;
+    uint2 *child = nullptr;
+    uint32_t *parent = nullptr, *status = nullptr;
+    uint32_t *sort_words = nullptr;
+    SortScratch sort{};
+    float4 *bbox_partial = nullptr, *bbox = nullptr;
+    int max_chunks = 1;
+    int64_t *sim_off = nullptr;
+    uint8_t *body_sim = nullptr;
+    uint32_t *sim_root = nullptr;
+    float2 *accel = nullptr;
+    unsigned long long *stats = nullptr;
+    uint32_t *overflow = nullptr;
+    int result_buf = 0;  // which sort buffer holds the last sorted keys/values
+
+    int32_t debug = 0;
+    int walk_block = 256, walk_stack = 128;
+
+    bool use_graph = false;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool graph_dirty = true;
+
+    bool profiling = false;
+    hipEvent_t ev[kNumPhases + 1] = {};
+    float phase_ms[kNumPhases] = {};
+    int64_t profiled_steps = 0;
+
+    std::string last_error;
+};
+
+namespace {
+
+int fail(nbody_sim *s, int code, const std::string &msg) {
+    if (s) s->last_error = msg;
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(nbody_sim *s, hipError_t e, const char *what) {
+    return fail(s, NBODY_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(s, call)                                       \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return hip_fail((s), e_, #call); \
+    } while (0)
+
+StepParams step_params(const nbody_config &c) {
+    StepParams p;
+    p.grav = c.grav_constant;
+    p.dt = c.delta_time;
+    p.eps_mult = c.epsilon_multiplier;
+    p.theta = c.bh_theta;
+    p.eps = c.epsilon;
+    p.soft_mode = c.softening_mode;
+    p.morton_mode = c.morton_mode;
+    p.math_mode = c.math_mode;
+    return p;
+}
+
+int validate_config(const nbody_config *c) {
+    if (!c) return fail(nullptr, NBODY_ERR_INVALID_ARG, "config is NULL");
+    if (c->softening_mode != NBODY_SOFTEN_REFERENCE && c->softening_mode != NBODY_SOFTEN_CONSTANT)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "unknown softening_mode");
+    if (c->morton_mode != NBODY_MORTON_REFERENCE && c->morton_mode != NBODY_MORTON_BBOX30)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "unknown morton_mode");
+    if (c->math_mode != NBODY_MATH_FAST && c->math_mode != NBODY_MATH_EXACT)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "unknown math_mode");
+    if (c->substeps < 1) return fail(nullptr, NBODY_ERR_INVALID_ARG, "substeps must be >= 1");
+    return NBODY_OK;
+}
+
+void release_graph(nbody_sim *s) {
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+    s->graph_dirty = true;
+}
+
+// (Re)allocate every buffer for `cap` bodies.  Contents are not preserved.
+int allocate(nbody_sim *s, int64_t cap) {
+    release_graph(s);
+    if (s->arena.base) {
+        HIP_TRY(s, hipFree(s->arena.base));
+        s->arena = Arena{};
+    }
+    cap = std::max<int64_t>(cap, 1);
+    const size_t key_size = s->ensemble ? 8 : 4;
+    const int64_t nodes = 2 * cap - 1;
+    s->max_chunks = (int)std::max<int64_t>(1, (cap + kBBoxChunk - 1) / kBBoxChunk);
+    const int64_t sort_words = sort_scratch_words(cap, s->passes);
+    // Size pass: lay out with a null base, then allocate the total.
+    for (int pass = 0; pass < 2; ++pass) {
+        Arena &a = s->arena;
+        a.used = 0;
+        s->mass = a.take<float>(cap);
+        s->pos = a.take<float2>(cap);
+        s->vel = a.take<float2>(cap);
+        for (int b = 0; b < 2; ++b) {
+            s->keys[b] = key_size == 8 ? (void *)a.take<uint64_t>(cap) : (void *)a.take<uint32_t>(cap);
+            s->vals[b] = a.take<uint32_t>(cap);
+        }
+        s->cm = a.take<float4>(nodes);
+        s->aabb = a.take<float4>(nodes);
+        s->child = a.take<uint2>(cap);
+        s->parent = a.take<uint32_t>(nodes);
+        s->status = a.take<uint32_t>(cap);
+        s->sort_words = a.take<uint32_t>(sort_words);
+        s->bbox_partial = a.take<float4>((int64_t)s->max_chunks * s->num_sims);
+        s->bbox = a.take<float4>(s->num_sims);
+        s->sim_off = a.take<int64_t>(s->num_sims + 1);
+        s->body_sim = s->ensemble ? a.take<uint8_t>(cap) : nullptr;
+        s->sim_root = a.take<uint32_t>(s->num_sims);
+        s->accel = a.take<float2>(cap);
+        s->stats = a.take<unsigned long long>(4);
+        s->overflow = a.take<uint32_t>(1);
+        if (pass == 0) {
+            a.size = a.used;
+            void *p = nullptr;
+            hipError_t e = hipMalloc(&p, a.size);
+            if (e != hipSuccess) {
+                s->cap = 0;
+                return fail(s, NBODY_ERR_CAPACITY,
+                            "hipMalloc of " + std::to_string(a.size) + " bytes failed: " +
+                                hipGetErrorString(e));
+            }
+            a.base = static_cast<char *>(p);
+        }
+    }
+    sort_bind_scratch(s->sort, s->sort_words, cap, s->passes);
+    s->cap = cap;
+    HIP_TRY(s, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->overflow, 0, sizeof(uint32_t), s->stream));
+    return NBODY_OK;
+}
+
+// Enqueue one step: simulation.ts:393-421.
+int enqueue_step(nbody_sim *s) {
+    const int64_t n = s->n;
+    if (n <= 0) return NBODY_OK;
+    hipStream_t st = s->stream;
+    const StepParams p = step_params(s->cfg);
+    const bool prof = s->profiling && !s->use_graph;
+    auto mark = [&](int i) {
+        if (prof) (void)hipEventRecord(s->ev[i], st);
+    };
+
+    mark(0);
+    // phase 0: bounding box (bbox30 keys only)
+    if (s->ensemble || s->cfg.morton_mode == NBODY_MORTON_BBOX30)
+        launch_bbox(s->pos, s->sim_off, s->num_sims, s->max_chunks, s->bbox_partial, s->bbox, st);
+    mark(1);
+    // phase 1: Morton encode (lbvh.wgsl:42-69)
+    launch_encode(s->pos, n, s->cfg.morton_mode, s->bbox, s->body_sim,
+                  s->ensemble ? nullptr : (uint32_t *)s->keys[0],
+                  s->ensemble ? (uint64_t *)s->keys[0] : nullptr, s->vals[0], st);
+    mark(2);
+    // phase 2: stable radix sort (simulation.ts:400)
+    if (s->ensemble)
+        s->result_buf = launch_sort_u64((uint64_t *)s->keys[0], s->vals[0], (uint64_t *)s->keys[1],
+                                        s->vals[1], n, s->key_bits, s->sort, st);
+    else
+        s->result_buf = launch_sort_u32((uint32_t *)s->keys[0], s->vals[0], (uint32_t *)s->keys[1],
+                                        s->vals[1], n, s->key_bits, s->sort, st);
+    const uint32_t *idx = s->vals[s->result_buf];
+    mark(3);
+    // phase 3: Karras build (lbvh.wgsl:85-165)
+    if (s->ensemble)
+        launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->child,
+                         s->parent, s->status, s->sim_root, st);
+    else
+        launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->child, s->parent,
+                         s->status, st);
+    mark(4);
+    // phase 4: bottom-up fill (lbvh.wgsl:168-233)
+    launch_fill(idx, s->mass, s->pos, n, s->child, s->parent, s->status, s->cm, s->aabb, st);
+    mark(5);
+    // phase 5: walk + kick + drift (barnes_hut.wgsl:48-121)
+    WalkArgs w{};
+    w.cm = s->cm;
+    w.child = s->child;
+    w.idx = idx;
+    w.keys64 = s->ensemble ? (const uint64_t *)s->keys[s->result_buf] : nullptr;
+    w.sim_root = s->sim_root;
+    w.vel = s->vel;
+    w.pos = s->pos;
+    w.accel = (s->debug & NBODY_DEBUG_ACCEL) ? s->accel : nullptr;
+    w.stats = s->stats;
+    w.overflow = s->overflow;
+    w.n = (uint32_t)n;
+    w.p = p;
+    launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
+                st);
+    mark(6);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(s, e, "kernel launch");
+    return NBODY_OK;
+}
+
+int accumulate_profile(nbody_sim *s) {
+    HIP_TRY(s, hipEventSynchronize(s->ev[kNumPhases]));
+    for (int i = 0; i < kNumPhases; ++i) {
+        float ms = 0.0f;
+        HIP_TRY(s, hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+        s->phase_ms[i] += ms;
+    }
+    s->profiled_steps += 1;
+    return NBODY_OK;
+}
+
+int capture_graph(nbody_sim *s) {
+    release_graph(s);
+    HIP_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc != NBODY_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(s, e, "hipStreamEndCapture");
+    s->graph = g;
+    e = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        release_graph(s);
+        return hip_fail(s, e, "hipGraphInstantiate");
+    }
+    s->graph_dirty = false;
+    return NBODY_OK;
+}
+
+nbody_sim *create_common(const nbody_config *cfg, int32_t device, int *err, bool ensemble,
+                         int num_sims, const int64_t *n_per_sim, int64_t capacity) {
+    auto set_err = [&](int code) {
+        if (err) *err = code;
+    };
+    int rc = validate_config(cfg);
+    if (rc != NBODY_OK) {
+        set_err(rc);
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err(fail(nullptr, NBODY_ERR_NO_DEVICE, "no HIP device available"));
+        return nullptr;
+    }
+    if (device < 0 || device >= ndev) {
+        set_err(fail(nullptr, NBODY_ERR_INVALID_ARG, "device index out of range"));
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        set_err(fail(nullptr, NBODY_ERR_HIP, "hipSetDevice failed"));
+        return nullptr;
+    }
+    nbody_sim *s = new nbody_sim();
+    s->cfg = *cfg;
+    s->device = device;
+    s->ensemble = ensemble;
+    s->num_sims = num_sims;
+    s->sim_off_host.assign(num_sims + 1, 0);
+    if (ensemble) {
+        for (int i = 0; i < num_sims; ++i) s->sim_off_host[i + 1] = s->sim_off_host[i] + n_per_sim[i];
+        capacity = s->sim_off_host[num_sims];
+        int sim_bits = 0;
+        while ((1 << sim_bits) < num_sims) ++sim_bits;
+        s->key_bits = kSimShift + sim_bits;
+    } else {
+        s->key_bits = cfg->morton_mode == NBODY_MORTON_BBOX30 ? 30 : 32;
+    }
+    s->passes = (s->key_bits + 7) / 8;
+    hipError_t e = hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_err(hip_fail(nullptr, e, "hipStreamCreate"));
+        delete s;
+        return nullptr;
+    }
+    s->stream = s->own_stream;
+    for (auto &ev : s->ev) (void)hipEventCreate(&ev);
+    rc = allocate(s, capacity);
+    if (rc != NBODY_OK) {
+        g_last_error = s->last_error;
+        set_err(rc);
+        nbody_destroy(s);
+        return nullptr;
+    }
+    set_err(NBODY_OK);
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbody_default_config(nbody_config *out) {
+    if (!out) return fail(nullptr, NBODY_ERR_INVALID_ARG, "out is NULL");
+    // config.ts:9-22
+    std::memset(out, 0, sizeof(*out));
+    out->grav_constant = 1.0f;
+    out->bh_theta = 0.6f;
+    out->substeps = 1;
+    out->delta_time = (float)(0.1 * 1.0 / (60.0 * 1));
+    out->epsilon_multiplier = 1.0f;
+    out->epsilon = 1e-3f;
+    out->softening_mode = NBODY_SOFTEN_REFERENCE;
+    out->morton_mode = NBODY_MORTON_REFERENCE;
+    out->math_mode = NBODY_MATH_FAST;
+    out->offset = 0;
+    return NBODY_OK;
+}
+
+nbody_sim *nbody_create(const nbody_config *cfg, int64_t capacity, int32_t device, int *err) {
+    if (capacity < 0 || capacity >= (int64_t(1) << 30)) {
+        int rc = fail(nullptr, NBODY_ERR_INVALID_ARG, "capacity must be in [0, 2^30)");
+        if (err) *err = rc;
+        return nullptr;
+    }
+    return create_common(cfg, device, err, false, 1, nullptr, capacity);
+}
+
+nbody_sim *nbody_ensemble_create(const nbody_config *cfg, int32_t num_sims,
+                                 const int64_t *n_per_sim, int32_t device, int *err) {
+    auto bad = [&](const char *msg) -> nbody_sim * {
+        int rc = fail(nullptr, NBODY_ERR_INVALID_ARG, msg);
+        if (err) *err = rc;
+        return nullptr;
+    };
+    if (num_sims < 1 || num_sims > kMaxSims) return bad("num_sims must be in [1, 256]");
+    if (!n_per_sim) return bad("n_per_sim is NULL");
+    int64_t total = 0;
+    for (int i = 0; i < num_sims; ++i) {
+        if (n_per_sim[i] < 1) return bad("every sim needs at least one body");
+        total += n_per_sim[i];
+    }
+    if (total >= (int64_t(1) << 30)) return bad("total bodies must be < 2^30");
+    return create_common(cfg, device, err, true, num_sims, n_per_sim, total);
+}
+
+int nbody_destroy(nbody_sim *s) {
+    if (!s) return NBODY_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    release_graph(s);
+    if (s->arena.base) (void)hipFree(s->arena.base);
+    for (auto &ev : s->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    delete s;
+    return NBODY_OK;
+}
+
+int nbody_set_config(nbody_sim *s, const nbody_config *cfg) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    int rc = validate_config(cfg);
+    if (rc != NBODY_OK) return fail(s, rc, g_last_error);
+    if (!s->ensemble && cfg->morton_mode != s->cfg.morton_mode) {
+        s->key_bits = cfg->morton_mode == NBODY_MORTON_BBOX30 ? 30 : 32;
+    }
+    if (s->ensemble && cfg->morton_mode != NBODY_MORTON_BBOX30 &&
+        cfg->morton_mode != NBODY_MORTON_REFERENCE)
+        return fail(s, NBODY_ERR_INVALID_ARG, "unknown morton_mode");
+    s->cfg = *cfg;
+    s->graph_dirty = true;
+    return NBODY_OK;
+}
+
+int nbody_get_config(const nbody_sim *s, nbody_config *out) {
+    if (!s || !out) return fail(nullptr, NBODY_ERR_INVALID_ARG, "NULL argument");
+    *out = s->cfg;
+    return NBODY_OK;
+}
+
+int nbody_set_bodies(nbody_sim *s, int64_t n, const float *mass, const float *pos_xy,
+                     const float *vel_xy) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (n < 0 || n >= (int64_t(1) << 30)) return fail(s, NBODY_ERR_INVALID_ARG, "n out of range");
+    if (n > 0 && (!mass || !pos_xy || !vel_xy))
+        return fail(s, NBODY_ERR_INVALID_ARG, "mass/pos/vel must be non-NULL");
+    if (s->ensemble && n != s->sim_off_host[s->num_sims])
+        return fail(s, NBODY_ERR_INVALID_ARG, "ensemble body count must equal the sum of n_per_sim");
+    HIP_TRY(s, hipSetDevice(s->device));
+    if (n > s->cap) {
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        int rc = allocate(s, n);
+        if (rc != NBODY_OK) return rc;
+    }
+    s->n = n;
+    if (!s->ensemble) {
+        s->sim_off_host[0] = 0;
+        s->sim_off_host[1] = n;
+    }
+    if (n > 0) {
+        HIP_TRY(s, hipMemcpyAsync(s->mass, mass, 4 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(s, hipMemcpyAsync(s->pos, pos_xy, 8 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(s, hipMemcpyAsync(s->vel, vel_xy, 8 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+    }
+    HIP_TRY(s, hipMemcpyAsync(s->sim_off, s->sim_off_host.data(), 8 * (size_t)(s->num_sims + 1),
+                              hipMemcpyHostToDevice, s->stream));
+    if (s->ensemble) {
+        std::vector<uint8_t> bs((size_t)n);
+        for (int i = 0; i < s->num_sims; ++i)
+            std::fill(bs.begin() + s->sim_off_host[i], bs.begin() + s->sim_off_host[i + 1],
+                      (uint8_t)i);
+        HIP_TRY(s, hipMemcpyAsync(s->body_sim, bs.data(), (size_t)n, hipMemcpyHostToDevice,
+                                  s->stream));
+        HIP_TRY(s, hipStreamSynchronize(s->stream));  // bs goes out of scope
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->graph_dirty = true;
+    return NBODY_OK;
+}
+
+int nbody_get_bodies(nbody_sim *s, float *mass, float *pos_xy, float *vel_xy) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const size_t n = (size_t)s->n;
+    if (n > 0) {
+        if (mass) HIP_TRY(s, hipMemcpyAsync(mass, s->mass, 4 * n, hipMemcpyDeviceToHost, s->stream));
+        if (pos_xy) HIP_TRY(s, hipMemcpyAsync(pos_xy, s->pos, 8 * n, hipMemcpyDeviceToHost, s->stream));
+        if (vel_xy) HIP_TRY(s, hipMemcpyAsync(vel_xy, s->vel, 8 * n, hipMemcpyDeviceToHost, s->stream));
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NBODY_OK;
+}
+
+int64_t nbody_get_num_bodies(const nbody_sim *s) { return s ? s->n : -1; }
+
+int nbody_step(nbody_sim *s, int32_t nsteps) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (nsteps < 0) return fail(s, NBODY_ERR_INVALID_ARG, "nsteps must be >= 0");
+    HIP_TRY(s, hipSetDevice(s->device));
+    if (s->n <= 0 || nsteps == 0) return NBODY_OK;
+    if (s->use_graph) {
+        if (s->graph_dirty || !s->graph_exec) {
+            int rc = capture_graph(s);
+            if (rc != NBODY_OK) return rc;
+        }
+        for (int i = 0; i < nsteps; ++i) HIP_TRY(s, hipGraphLaunch(s->graph_exec, s->stream));
+        return NBODY_OK;
+    }
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = enqueue_step(s);
+        if (rc != NBODY_OK) return rc;
+        if (s->profiling) {
+            rc = accumulate_profile(s);
+            if (rc != NBODY_OK) return rc;
+        }
+    }
+    return NBODY_OK;
+}
+
+int nbody_synchronize(nbody_sim *s) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    HIP_TRY(s, hipSetDevice(s->device));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    uint32_t of = 0;
+    HIP_TRY(s, hipMemcpy(&of, s->overflow, sizeof(of), hipMemcpyDeviceToHost));
+    if (of) {
+        HIP_TRY(s, hipMemset(s->overflow, 0, sizeof(uint32_t)));
+        return fail(s, NBODY_ERR_STACK_OVERFLOW,
+                    "tree walk stack overflow: nodes were dropped (raise walk_stack)");
+    }
+    return NBODY_OK;
+}
+
+int nbody_set_stream(nbody_sim *s, void *hip_stream) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    HIP_TRY(s, hipSetDevice(s->device));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : s->own_stream;
+    s->graph_dirty = true;
+    return NBODY_OK;
+}
+
+int nbody_use_graph(nbody_sim *s, int32_t enable) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    s->use_graph = enable != 0;
+    s->graph_dirty = true;
+    return NBODY_OK;
+}
+
+int nbody_set_debug(nbody_sim *s, int32_t flags) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (flags & ~(NBODY_DEBUG_ACCEL | NBODY_DEBUG_STATS))
+        return fail(s, NBODY_ERR_INVALID_ARG, "unknown debug flag");
+    s->debug = flags;
+    s->graph_dirty = true;
+    return NBODY_OK;
+}
+
+int nbody_debug_get(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
+    if (!s || !dst) return fail(s, NBODY_ERR_INVALID_ARG, "NULL argument");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const int64_t n = s->n;
+    const void *src = nullptr;
+    int64_t need = 0;
+    switch (which) {
+        case NBODY_BUF_KEYS:
+            src = s->keys[s->result_buf];
+            need = n * (s->ensemble ? 8 : 4);
+            break;
+        case NBODY_BUF_BODY_INDICES:
+            src = s->vals[s->result_buf];
+            need = n * 4;
+            break;
+        case NBODY_BUF_CHILDREN:
+            src = s->child;
+            need = std::max<int64_t>(n - 1, 0) * 8;
+            break;
+        case NBODY_BUF_PARENT:
+            src = s->parent;
+            need = std::max<int64_t>(2 * n - 1, 0) * 4;
+            break;
+        case NBODY_BUF_NODE_CM:
+            src = s->cm;
+            need = std::max<int64_t>(2 * n - 1, 0) * 16;
+            break;
+        case NBODY_BUF_NODE_AABB:
+            src = s->aabb;
+            need = std::max<int64_t>(2 * n - 1, 0) * 16;
+            break;
+        case NBODY_BUF_ACCEL:
+            src = s->accel;
+            need = n * 8;
+            break;
+        case NBODY_BUF_BBOX:
+            src = s->bbox;
+            need = (int64_t)s->num_sims * 16;
+            break;
+        case NBODY_BUF_SIM_ROOTS:
+            src = s->sim_root;
+            need = (int64_t)s->num_sims * 4;
+            break;
+        default:
+            return fail(s, NBODY_ERR_INVALID_ARG, "unknown buffer id");
+    }
+    if (bytes < need)
+        return fail(s, NBODY_ERR_INVALID_ARG,
+                    "destination too small: need " + std::to_string(need) + " bytes");
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (need > 0) HIP_TRY(s, hipMemcpy(dst, src, (size_t)need, hipMemcpyDeviceToHost));
+    return NBODY_OK;
+}
+
+int nbody_walk_stats(nbody_sim *s, uint64_t *out3, int32_t reset) {
+    if (!s || !out3) return fail(s, NBODY_ERR_INVALID_ARG, "NULL argument");
+    HIP_TRY(s, hipSetDevice(s->device));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    HIP_TRY(s, hipMemcpy(h, s->stats, sizeof(h), hipMemcpyDeviceToHost));
+    out3[0] = h[0];
+    out3[1] = h[1];
+    out3[2] = h[2];
+    if (reset) HIP_TRY(s, hipMemset(s->stats, 0, sizeof(h)));
+    return NBODY_OK;
+}
+
+int nbody_set_profiling(nbody_sim *s, int32_t enable) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    s->profiling = enable != 0;
+    return NBODY_OK;
+}
+
+int nbody_phase_times(nbody_sim *s, float *ms_out, int64_t *steps, int32_t reset) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (ms_out)
+        for (int i = 0; i < kNumPhases; ++i) ms_out[i] = s->phase_ms[i];
+    if (steps) *steps = s->profiled_steps;
+    if (reset) {
+        for (float &v : s->phase_ms) v = 0.0f;
+        s->profiled_steps = 0;
+    }
+    return NBODY_OK;
+}
+
+int nbody_set_tuning(nbody_sim *s, int32_t walk_block, int32_t walk_stack) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (walk_block) {
+        if (walk_block < 64 || walk_block > 1024 || walk_block % 64)
+            return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");
+        s->walk_block = walk_block;
+    }
+    if (walk_stack) {
+        if (walk_stack < 8 || walk_stack > 256)
+            return fail(s, NBODY_ERR_INVALID_ARG, "walk_stack must be in [8, 256]");
+        s->walk_stack = walk_stack;
+    }
+    s->graph_dirty = true;
+    return NBODY_OK;
+}
+
+const char *nbody_last_error(const nbody_sim *s) {
+    return s ? s->last_error.c_str() : g_last_error.c_str();
+}
+
+const char *nbody_version(void) { return "nbody_sim 0.1.0 (gfx950)"; }
+
+}  // extern "C"

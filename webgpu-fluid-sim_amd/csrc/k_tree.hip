@@ -1,0 +1,201 @@
+// k_tree.hip -- Karras LBVH construction and the bottom-up mass/COM fill.
+//
+// build: build_lbvh_main (src/shaders/compute/lbvh.wgsl:85-165), Karras 2012
+//   Fig. 4, one thread per internal node.  Internal nodes are [0, n-2],
+//   leaves [n-1, 2n-2] (lbvh.wgsl:94).  Children go to a packed uint2 array
+//   (8 B per internal node, what the walk loads), parents to their own array.
+// fill: fill_lbvh_main (lbvh.wgsl:168-233).  One thread per leaf in sorted
+//   order climbs toward the root; the first thread to reach a parent stops,
+//   the second merges both children.  The reference's arrival counter is a
+//   relaxed atomic with workgroup-scope barriers, so the second arriver may
+//   read stale sibling data (README.md:18).  Here the counter is an
+//   agent-scope acquire-release fetch-add: the first arriver's node writes
+//   happen-before the second arriver's reads on any XCD, and the merge
+//   formula does not depend on which child arrives last, so the result is
+//   deterministic.  The reference's random leaf rotation (`offset`,
+//   lbvh.wgsl:171) is therefore unnecessary and not applied.
+#include "nbody_internal.h"
+
+namespace nbs {
+
+namespace {
+
+struct DeltaU32 {
+    const uint32_t *__restrict__ k;
+    int64_t n;
+    // lbvh.wgsl:72-83
+    __device__ __forceinline__ int operator()(int64_t i, int64_t j) const {
+        if (j < 0 || j >= n) return -1;
+        const uint32_t a = k[i], b = k[j];
+        if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+        return __clz((int)(a ^ b));
+    }
+};
+
+// Ensemble keys carry the sim id above bit 30.  Equal keys share a sim and
+// are tie-broken on sim-local sorted positions, so every sim's sub-tree has
+// the shape of the tree a plain run of that sim builds.
+struct DeltaU64 {
+    const uint64_t *__restrict__ k;
+    const int64_t *__restrict__ sim_off;
+    int64_t n;
+    __device__ __forceinline__ int operator()(int64_t i, int64_t j) const {
+        if (j < 0 || j >= n) return -1;
+        const uint64_t a = k[i], b = k[j];
+        if (a == b) {
+            const int64_t off = sim_off[a >> kSimShift];
+            return 64 + __clz((int)((uint32_t)(i - off) ^ (uint32_t)(j - off)));
+        }
+        return __clzll((long long)(a ^ b));
+    }
+};
+
+struct Split {
+    int64_t lo, hi, gamma;
+    uint32_t lc, rc;
+};
+
+template <typename Delta>
+__device__ __forceinline__ Split karras_node(const Delta &delta, int64_t i, int64_t n) {
+    // lbvh.wgsl:101-152
+    const int d = delta(i, i + 1) > delta(i, i - 1) ? 1 : -1;
+    const int delta_min = delta(i, i - d);
+    int64_t l_max = 2;
+    while (delta(i, i + d * l_max) > delta_min) l_max *= 2;
+    int64_t l = 0;
+    for (int64_t t = l_max / 2; t >= 1; t /= 2)
+        if (delta(i, i + d * (l + t)) > delta_min) l += t;
+    const int64_t j = i + d * l;
+    const int delta_node = delta(i, j);
+    int64_t s = 0;
+    int64_t t = (l + 1) / 2;
+    for (;;) {
+        if (delta(i, i + d * (s + t)) > delta_node) s += t;
+        if (t == 1) break;
+        t = (t + 1) / 2;
+    }
+    Split r;
+    r.gamma = i + d * s + (d < 0 ? d : 0);
+    r.lo = i < j ? i : j;
+    r.hi = i < j ? j : i;
+    r.lc = (r.lo == r.gamma) ? (uint32_t)(n - 1 + r.gamma) : (uint32_t)r.gamma;
+    r.rc = (r.hi == r.gamma + 1) ? (uint32_t)(n - 1 + r.gamma + 1) : (uint32_t)(r.gamma + 1);
+    return r;
+}
+
+__global__ __launch_bounds__(256) void build_u32_kernel(const uint32_t *__restrict__ keys,
+                                                        int64_t n, uint2 *__restrict__ child,
+                                                        uint32_t *__restrict__ parent,
+                                                        uint32_t *__restrict__ status) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1) return;
+    DeltaU32 delta{keys, n};
+    const Split sp = karras_node(delta, i, n);
+    child[i] = make_uint2(sp.lc, sp.rc);
+    parent[sp.lc] = (uint32_t)i;  // lbvh.wgsl:155-161
+    parent[sp.rc] = (uint32_t)i;
+    if (i == 0) parent[0] = kNoNode;
+    status[i] = 0u;  // lbvh.wgsl:164
+}
+
+__global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restrict__ keys,
+                                                        int64_t n,
+                                                        const int64_t *__restrict__ sim_off,
+                                                        uint2 *__restrict__ child,
+                                                        uint32_t *__restrict__ parent,
+                                                        uint32_t *__restrict__ status,
+                                                        uint32_t *__restrict__ sim_root) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1) return;
+    DeltaU64 delta{keys, sim_off, n};
+    const Split sp = karras_node(delta, i, n);
+    child[i] = make_uint2(sp.lc, sp.rc);
+    parent[sp.lc] = (uint32_t)i;
+    parent[sp.rc] = (uint32_t)i;
+    status[i] = 0u;
+    // A node spanning two sims sits above the sim level; a child whose
+    // range lies inside one sim is that sim's root.
+    const uint64_t s_lo = keys[sp.lo] >> kSimShift, s_hi = keys[sp.hi] >> kSimShift;
+    if (s_lo != s_hi) {
+        if ((keys[sp.gamma] >> kSimShift) == s_lo) sim_root[s_lo] = sp.lc;
+        if ((keys[sp.gamma + 1] >> kSimShift) == s_hi) sim_root[s_hi] = sp.rc;
+    }
+    if (i == 0) {
+        parent[0] = kNoNode;
+        const uint64_t s0 = keys[0] >> kSimShift;
+        if (s0 == (keys[n - 1] >> kSimShift)) sim_root[s0] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(const uint32_t *__restrict__ idx,
+                                                   const float *__restrict__ mass,
+                                                   const float2 *__restrict__ pos, int64_t n,
+                                                   const uint2 *__restrict__ child,
+                                                   const uint32_t *__restrict__ parent,
+                                                   uint32_t *status, float4 *cm, float4 *aabb) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    uint32_t cur = (uint32_t)(n - 1 + t);
+    const uint32_t b = idx[t];
+    const float2 p = pos[b];
+    const float m = mass[b];
+    // lbvh.wgsl:179-187
+    cm[cur] = make_float4(p.x, p.y, m, 0.0f);
+    aabb[cur] = make_float4(p.x, p.y, p.x, p.y);
+    // lbvh.wgsl:191-232
+    // A valid tree reaches the root in <= 96 levels; the level bound and the
+    // index check only guard against a corrupt parent array.
+    for (int level = 0; cur != 0u && level < 128; ++level) {
+        const uint32_t par = parent[cur];
+        if (par >= (uint32_t)(n - 1)) break;
+        const uint32_t prev =
+            __hip_atomic_fetch_add(&status[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == 0u) break;
+        const uint2 ch = child[par];
+        const float4 L = cm[ch.x], R = cm[ch.y];
+        const float4 LB = aabb[ch.x], RB = aabb[ch.y];
+        float4 B;
+        B.x = fminf(LB.x, RB.x);
+        B.y = fminf(LB.y, RB.y);
+        B.z = fmaxf(LB.z, RB.z);
+        B.w = fmaxf(LB.w, RB.w);
+        float4 C;
+        C.z = L.z + R.z;
+        if (C.z > 0.0f) {
+            C.x = (L.x * L.z + R.x * R.z) / C.z;
+            C.y = (L.y * L.z + R.y * R.z) / C.z;
+        } else {
+            C.x = 0.0f;
+            C.y = 0.0f;
+        }
+        C.w = fmaxf(B.z - B.x, B.w - B.y);
+        aabb[par] = B;
+        cm[par] = C;
+        cur = par;
+    }
+}
+
+}  // namespace
+
+void launch_build_u32(const uint32_t *keys, int64_t n, uint2 *child, uint32_t *parent,
+                      uint32_t *status, hipStream_t s) {
+    if (n < 2) return;
+    build_u32_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, child, parent, status);
+}
+
+void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, uint2 *child,
+                      uint32_t *parent, uint32_t *status, uint32_t *sim_root, hipStream_t s) {
+    if (n < 2) return;
+    build_u64_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, sim_off, child,
+                                                                     parent, status, sim_root);
+}
+
+void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
+                 const uint2 *child, const uint32_t *parent, uint32_t *status, float4 *cm,
+                 float4 *aabb, hipStream_t s) {
+    if (n < 1) return;
+    fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(idx, mass, pos, n, child, parent,
+                                                            status, cm, aabb);
+}
+
+}  // namespace nbs

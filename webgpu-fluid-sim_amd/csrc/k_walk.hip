@@ -1,0 +1,277 @@
+// k_walk.hip -- Barnes-Hut tree walk with fused kick and drift.
+//
+// Reference: bh_vel_step_main (src/shaders/compute/barnes_hut.wgsl:48-111)
+// walks the tree once per body with a private 256-entry stack, accepting a
+// node when it is a leaf or length * rsqrt(d^2 + eps^2) < theta, otherwise
+// pushing left then right (so right is visited first), and skipping the
+// body's own leaf; then v += a*dt.  bh_pos_step_main (barnes_hut.wgsl:113-121)
+// is a second dispatch doing x += v*dt.
+//
+// MI355X design: the 64 lanes of a wavefront hold 64 consecutive bodies in
+// Morton order and share ONE depth-first stack in LDS whose entries are
+// (node, lane mask).  A node is visited once per wavefront: its 16-byte
+// {com, mass, length} record and 8-byte child pair are wave-uniform scalar
+// loads, and each lane named in the mask applies the reference's own test.
+// Lanes that open the node form the mask of its children.  Because every
+// lane's accepted nodes, and the order it meets them, are exactly those of
+// its private reference walk, each lane's sum is the reference's sum term
+// for term.  The walk reads the tree only (the body's own position comes
+// from its leaf record), so the drift is fused into the epilogue without
+// changing any force: x_new = x + (v + a*dt)*dt, written by body index.
+//
+// Stack depth: keys are unique once tie-broken by position, the common
+// prefix length grows by at least one per level, so depth <= 64 (96 for the
+// ensemble's 38-bit keys) and the stack never holds more than depth + 1
+// entries.  A full stack drops both children like the reference
+// (barnes_hut.wgsl:100-105) and raises an overflow counter.
+#include "nbody_internal.h"
+
+namespace nbs {
+
+namespace {
+
+// Tree records are read-only during the walk and indexed by a wave-uniform
+// node id; reading them through the constant address space makes them
+// scalar loads (s_load) issued once per wavefront.
+template <typename T>
+using const_as = const __attribute__((address_space(4))) T;
+
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = uniform_u32((uint32_t)v), hi = uniform_u32((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One interaction test of body (px, py, m1) against node record c.
+// Returns true when the lane must open the node; otherwise adds the
+// node's pull to (ax, ay) (barnes_hut.wgsl:78-97).
+template <bool kExact, bool kRefSoft>
+__device__ __forceinline__ bool interact(const float4 c, bool leaf, float px, float py, float m1,
+                                         const StepParams &p, float eps2c, float &ax, float &ay) {
+    const float rx = c.x - px;
+    const float ry = c.y - py;
+    float eps2;
+    if constexpr (kRefSoft) {
+        const float m_eff = fmaxf(m1, c.z);
+        const float eps = p.eps_mult * sqrtf(p.grav * m_eff * p.dt);
+        eps2 = eps * eps;
+    } else {
+        eps2 = eps2c;
+    }
+    if constexpr (kExact) {
+        // dot(r, r), + eps^2, 1/sqrt: each one correctly rounded binary32 op.
+        const float d2 = rx * rx + ry * ry;
+        const float inv = 1.0f / sqrtf(d2 + eps2);
+        if (leaf || c.w * inv < p.theta) {
+            const float inv3 = inv * inv * inv;
+            const float gm = p.grav * c.z;
+            ax = ax + gm * rx * inv3;
+            ay = ay + gm * ry * inv3;
+            return false;
+        }
+        return true;
+    } else {
+        const float s = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2));
+        const float inv = __builtin_amdgcn_rsqf(s);
+        if (leaf || c.w * inv < p.theta) {
+            const float f = p.grav * c.z * (inv * inv * inv);
+            ax = __builtin_fmaf(f, rx, ax);
+            ay = __builtin_fmaf(f, ry, ay);
+            return false;
+        }
+        return true;
+    }
+}
+
+template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
+__global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_cap) {
+    extern __shared__ uint64_t s_dyn[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint64_t *s_mask = s_dyn + (size_t)wid * stack_cap;
+    uint32_t *s_node = reinterpret_cast<uint32_t *>(s_dyn + (size_t)(blockDim.x >> 6) * stack_cap) +
+                       (size_t)wid * stack_cap;
+
+    const uint32_t n = a.n;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;  // sorted position
+    const bool valid = t < n;
+    const uint32_t leaf0 = n - 1u;
+    const uint32_t my_leaf = leaf0 + t;
+    const float4 *__restrict__ cm = a.cm;
+    const_as<float> *cm_s = (const_as<float> *)a.cm;
+    const_as<uint32_t> *child_s = (const_as<uint32_t> *)a.child;
+
+    float px = 0.0f, py = 0.0f, m1 = 0.0f;
+    if (valid) {
+        const float4 me = cm[my_leaf];  // written by the fill: {pos, mass, 0}
+        px = me.x;
+        py = me.y;
+        m1 = me.z;
+    }
+    const uint64_t live = __ballot(valid);
+    if (live == 0) return;
+
+    const StepParams p = a.p;
+    const float eps2c = p.eps * p.eps;
+    float ax = 0.0f, ay = 0.0f;
+    uint32_t n_acc = 0, n_open = 0, n_drop = 0;
+    int sp = 0;
+    uint32_t node = 0u;
+    uint64_t mask = live;
+    if constexpr (kEnsemble) {
+        // Each lane starts at its sim's sub-tree root; group lanes by root.
+        uint32_t my_root = 0u;
+        if (valid) my_root = a.sim_root[(uint32_t)(a.keys64[t] >> kSimShift)];
+        uint64_t rem = live;
+        bool first = true;
+        while (rem) {
+            const int l0 = __builtin_ctzll(rem);
+            const uint32_t r = uniform_u32((uint32_t)__builtin_amdgcn_readlane((int)my_root, l0));
+            const uint64_t m = __ballot(valid && my_root == r) & rem;
+            rem &= ~m;
+            if (first) {
+                node = r;
+                mask = m;
+                first = false;
+            } else if (sp < stack_cap) {
+                if (lane == 0) {
+                    s_node[sp] = r;
+                    s_mask[sp] = m;
+                }
+                ++sp;
+            } else if (lane == 0) {
+                atomicOr(a.overflow, 1u);
+            }
+        }
+    }
+
+    // A wavefront visits each node at most once per lane group, so a walk of a
+    // valid tree ends within 4n iterations; the bound only guards against a
+    // corrupt tree and reports like a stack overflow.
+    const uint32_t max_iter = 4u * n + 64u;
+    for (uint32_t iter = 0;; ++iter) {
+        if (iter >= max_iter) {
+            if (lane == 0) atomicOr(a.overflow, 1u);
+            break;
+        }
+        const float4 c = make_float4(cm_s[4 * node], cm_s[4 * node + 1], cm_s[4 * node + 2],
+                                     cm_s[4 * node + 3]);
+        const bool leaf = node >= leaf0;
+        bool open = false;
+        if (((mask >> lane) & 1u) && !(leaf && node == my_leaf)) {
+            open = interact<kExact, kRefSoft>(c, leaf, px, py, m1, p, eps2c, ax, ay);
+            if constexpr (kStats) {
+                if (open)
+                    ++n_open;
+                else
+                    ++n_acc;
+            }
+        }
+        const uint64_t om = __ballot(open);
+        if (om) {
+            const uint2 ch = make_uint2(child_s[2 * node], child_s[2 * node + 1]);
+            if (sp < stack_cap) {
+                // Push left, continue with right: the reference pops right first.
+                if (lane == 0) {
+                    s_node[sp] = ch.x;
+                    s_mask[sp] = om;
+                }
+                ++sp;
+                node = ch.y;
+                mask = om;
+                continue;
+            }
+            if (lane == 0) atomicOr(a.overflow, 1u);
+            if constexpr (kStats) {
+                if (open) {
+                    --n_open;
+                    ++n_drop;
+                }
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = uniform_u32(s_node[sp]);
+        mask = uniform_u64(s_mask[sp]);
+    }
+
+    if (valid) {
+        const uint32_t bi = a.idx[t];
+        const float dt = p.dt;
+        float2 v = a.vel[bi];
+        float2 x;
+        if constexpr (kExact) {
+            v.x = v.x + ax * dt;  // barnes_hut.wgsl:110
+            v.y = v.y + ay * dt;
+            x.x = px + v.x * dt;  // barnes_hut.wgsl:120
+            x.y = py + v.y * dt;
+        } else {
+            v.x = __builtin_fmaf(ax, dt, v.x);
+            v.y = __builtin_fmaf(ay, dt, v.y);
+            x.x = __builtin_fmaf(v.x, dt, px);
+            x.y = __builtin_fmaf(v.y, dt, py);
+        }
+        a.vel[bi] = v;
+        a.pos[bi] = x;
+        if (a.accel) a.accel[bi] = make_float2(ax, ay);
+    }
+    if constexpr (kStats) {
+        // wave sums, one atomic per wave and counter
+        uint32_t s0 = n_acc, s1 = n_open, s2 = n_drop;
+        for (int o = 32; o > 0; o >>= 1) {
+            s0 += __shfl_xor(s0, o);
+            s1 += __shfl_xor(s1, o);
+            s2 += __shfl_xor(s2, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], (unsigned long long)s0);
+            atomicAdd(&a.stats[1], (unsigned long long)s1);
+            if (s2) atomicAdd(&a.stats[2], (unsigned long long)s2);
+        }
+    }
+}
+
+template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
+void launch_t(const WalkArgs &a, int block, int stack, hipStream_t s) {
+    const unsigned grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
+    const size_t lds = (size_t)(block / kWave) * (size_t)stack * (sizeof(uint64_t) + sizeof(uint32_t));
+    walk_kernel<kExact, kRefSoft, kEnsemble, kStats><<<grid, block, lds, s>>>(a, stack);
+}
+
+template <bool kExact, bool kRefSoft>
+void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, hipStream_t s) {
+    if (ens) {
+        if (stats)
+            launch_t<kExact, kRefSoft, true, true>(a, block, stack, s);
+        else
+            launch_t<kExact, kRefSoft, true, false>(a, block, stack, s);
+    } else {
+        if (stats)
+            launch_t<kExact, kRefSoft, false, true>(a, block, stack, s);
+        else
+            launch_t<kExact, kRefSoft, false, false>(a, block, stack, s);
+    }
+}
+
+}  // namespace
+
+void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
+                 hipStream_t s) {
+    if (a.n == 0) return;
+    const bool exact = a.p.math_mode != 0;
+    const bool refsoft = a.p.soft_mode == 0;
+    if (exact) {
+        if (refsoft)
+            launch_e<true, true>(a, ensemble, stats, block, stack, s);
+        else
+            launch_e<true, false>(a, ensemble, stats, block, stack, s);
+    } else {
+        if (refsoft)
+            launch_e<false, true>(a, ensemble, stats, block, stack, s);
+        else
+            launch_e<false, false>(a, ensemble, stats, block, stack, s);
+    }
+}
+
+}  // namespace nbs

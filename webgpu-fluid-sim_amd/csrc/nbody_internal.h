@@ -1,0 +1,84 @@
+// nbody_internal.h -- device-side types and kernel launchers shared by the
+// HIP translation units of libnbody_sim.so.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbs {
+
+constexpr int kWave = 64;
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+constexpr int kSimShift = 30;     // ensemble keys: sim id << 30 | 30-bit Morton code
+constexpr int kMaxSims = 256;     // 8-bit sim prefix
+constexpr int kBBoxChunk = 4096;  // bodies per bounding-box partial
+
+// Per-step physics parameters (the Metadata uniform, lbvh.wgsl:3-10).
+struct StepParams {
+    float grav;
+    float dt;
+    float eps_mult;
+    float theta;
+    float eps;  // constant softening length
+    int soft_mode;
+    int morton_mode;
+    int math_mode;
+};
+
+// ---------------------------------------------------------------- keys
+// bbox partials: grid (chunks, sims); then one wave per sim reduces them.
+void launch_bbox(const float2 *pos, const int64_t *sim_off, int num_sims, int max_chunks,
+                 float4 *partial, float4 *bbox, hipStream_t s);
+// Morton encode.  keys32 (plain) or keys64 (ensemble, body_sim != nullptr).
+void launch_encode(const float2 *pos, int64_t n, int morton_mode, const float4 *bbox,
+                   const uint8_t *body_sim, uint32_t *keys32, uint64_t *keys64, uint32_t *idx,
+                   hipStream_t s);
+
+// ---------------------------------------------------------------- sort
+struct SortScratch {
+    uint32_t *hist;      // [passes][256] global digit counts -> exclusive offsets
+    uint32_t *lookback;  // [passes][tiles][256] decoupled look-back words
+    uint32_t *counters;  // [passes] dynamic tile ids
+    int64_t words;       // total u32 words of the three arrays (cleared per sort)
+};
+int64_t sort_tiles(int64_t n);
+int64_t sort_scratch_words(int64_t n, int passes);
+void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t n, int passes);
+// Stable LSD sort of (keys, vals) on the low `bits` bits; ping-pongs between
+// buffers a and b and returns which buffer (0 = a, 1 = b) holds the result.
+int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
+                    int64_t n, int bits, const SortScratch &sc, hipStream_t s);
+int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,
+                    int64_t n, int bits, const SortScratch &sc, hipStream_t s);
+
+// ---------------------------------------------------------------- tree
+void launch_build_u32(const uint32_t *keys, int64_t n, uint2 *child, uint32_t *parent,
+                      uint32_t *status, hipStream_t s);
+void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, uint2 *child,
+                      uint32_t *parent, uint32_t *status, uint32_t *sim_root, hipStream_t s);
+void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
+                 const uint2 *child, const uint32_t *parent, uint32_t *status, float4 *cm,
+                 float4 *aabb, hipStream_t s);
+
+// ---------------------------------------------------------------- walk
+struct WalkArgs {
+    const float4 *cm;        // [2n-1] {com.x, com.y, mass, length}
+    const uint2 *child;      // [n-1]
+    const uint32_t *idx;     // [n] sorted position -> body
+    const uint64_t *keys64;  // ensemble: sorted keys (sim id in bits 30..37)
+    const uint32_t *sim_root;
+    float2 *vel;             // [n] by body
+    float2 *pos;             // [n] by body (drift output)
+    float2 *accel;           // optional [n] by body
+    unsigned long long *stats;  // optional {accepted, opened, overflowed}
+    uint32_t *overflow;         // set to 1 when a full stack drops nodes
+    uint32_t n;
+    StepParams p;
+};
+void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
+                 hipStream_t s);
+
+// ---------------------------------------------------------------- misc
+void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s);
+
+}  // namespace nbs
