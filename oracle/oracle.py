@@ -56,6 +56,11 @@ class Params:
 _lib = None
 
 
+def _threads():
+    """OpenMP threads for the oracle: the host cores, at most 16."""
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -181,7 +186,7 @@ def walk(idx, cm, left, right, mass, pos, params: Params, roots=None, stack_size
         rp = _p(roots)
     lib().orc_walk(ctypes.c_int64(n), _p(idx), _p(cm), _p(left), _p(right), rp, _p(mass),
                    _p(pos), ctypes.byref(pr), ctypes.c_int(stack_size), _p(acc), _p(stats),
-                   ctypes.c_int(threads or os.cpu_count() or 1))
+                   ctypes.c_int(threads or _threads()))
     return acc, stats
 
 
@@ -201,7 +206,7 @@ def step(mass, pos, vel, params: Params, nsteps=1, threads=None):
     vel = _f32(vel, (-1, 2)).copy()
     pr = params.c()
     rc = lib().orc_step(ctypes.c_int64(pos.shape[0]), ctypes.byref(pr), _p(mass), _p(pos),
-                        _p(vel), ctypes.c_int(nsteps), ctypes.c_int(threads or os.cpu_count() or 1))
+                        _p(vel), ctypes.c_int(nsteps), ctypes.c_int(threads or _threads()))
     if rc != 0:
         raise RuntimeError("orc_step failed")
     return pos, vel
@@ -229,7 +234,7 @@ def step_debug(mass, pos, vel, params: Params, threads=None):
         _p(out["idx"]), _p(out["left"]), _p(out["right"]), _p(out["parent"]),
         _p(out["node_cm"]), _p(out["node_aabb"]), _p(out["accel"]), _p(out["pos"]),
         _p(out["vel"]), _p(out["stats"]), _p(out["bbox"]),
-        ctypes.c_int(threads or os.cpu_count() or 1))
+        ctypes.c_int(threads or _threads()))
     if rc != 0:
         raise RuntimeError("orc_step_debug failed")
     out["left"] = out["left"][: max(n - 1, 0)]
@@ -251,5 +256,5 @@ def direct_sum_f64(pos, mass, params: Params, targets=None, threads=None):
     pr = params.c()
     lib().orc_direct_sum_f64(ctypes.c_int64(n), _p(pos), _p(mass), ctypes.byref(pr), tp,
                              ctypes.c_int64(nt), _p(acc),
-                             ctypes.c_int(threads or os.cpu_count() or 1))
+                             ctypes.c_int(threads or _threads()))
     return acc

@@ -65,7 +65,7 @@ def test_reference_keys_heavy_duplicates():
     cfg, P = params(theta=0.5, soft="constant", morton="reference")
     g = gpu_step(cfg, mass, pos, vel)
     r = oracle_step(P, mass, pos, vel)
-    assert len(np.unique(r["keys"])) < 200
+    assert len(np.unique(r["keys"])) < 0.6 * len(r["keys"])
     _compare_tree(g, r)
     assert np.array_equal(g["accel"], r["accel"])
 

@@ -10,8 +10,9 @@
 //   keys[2] u32|u64[C], vals[2] u32[C]             -- sort ping-pong
 //   cm float4[2C-1] {com.x, com.y, mass, length}   -- what the walk reads
 //   aabb float4[2C-1]                              -- what the fill merges
-//   child uint2[C-1], parent u32[2C-1], status u32[C-1]
-//   sort scratch, bbox partials, debug accel float2[C], counters
+//   child uint2[C-1], range uint2[C-1], parent u32[2C-1], status u32[C-1]
+//   fill frontier u32[C], sort scratch, bbox partials, debug accel float2[C],
+//   counters
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -63,9 +64,7 @@ struct nbody_sim {
     float2 *pos = nullptr, *vel = nullptr;
     void *keys[2] = {nullptr, nullptr};
     uint32_t *vals[2] = {nullptr, nullptr};
-    float4 *cm = nullptr, *aabb = nullptr;
-    uint2 *child = nullptr;
-    uint32_t *parent = nullptr, *status = nullptr;
+    TreeBufs tree{};
     uint32_t *sort_words = nullptr;
     SortScratch sort{};
     float4 *bbox_partial = nullptr, *bbox = nullptr;
@@ -168,11 +167,14 @@ int allocate(nbody_sim *s, int64_t cap) {
             s->keys[b] = key_size == 8 ? (void *)a.take<uint64_t>(cap) : (void *)a.take<uint32_t>(cap);
             s->vals[b] = a.take<uint32_t>(cap);
         }
-        s->cm = a.take<float4>(nodes);
-        s->aabb = a.take<float4>(nodes);
-        s->child = a.take<uint2>(cap);
-        s->parent = a.take<uint32_t>(nodes);
-        s->status = a.take<uint32_t>(cap);
+        s->tree.cm = a.take<float4>(nodes);
+        s->tree.aabb = a.take<float4>(nodes);
+        s->tree.child = a.take<uint2>(cap);
+        s->tree.range = a.take<uint2>(cap);
+        s->tree.parent = a.take<uint32_t>(nodes);
+        s->tree.status = a.take<uint32_t>(cap);
+        s->tree.frontier = a.take<uint32_t>(cap);
+        s->tree.frontier_count = a.take<uint32_t>(1);
         s->sort_words = a.take<uint32_t>(sort_words);
         s->bbox_partial = a.take<float4>((int64_t)s->max_chunks * s->num_sims);
         s->bbox = a.take<float4>(s->num_sims);
@@ -199,6 +201,7 @@ int allocate(nbody_sim *s, int64_t cap) {
     s->cap = cap;
     HIP_TRY(s, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->overflow, 0, sizeof(uint32_t), s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->tree.frontier_count, 0, sizeof(uint32_t), s->stream));
     return NBODY_OK;
 }
 
@@ -234,19 +237,18 @@ int enqueue_step(nbody_sim *s) {
     mark(3);
     // phase 3: Karras build (lbvh.wgsl:85-165)
     if (s->ensemble)
-        launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->child,
-                         s->parent, s->status, s->sim_root, st);
+        launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree,
+                         s->sim_root, st);
     else
-        launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->child, s->parent,
-                         s->status, st);
+        launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, st);
     mark(4);
     // phase 4: bottom-up fill (lbvh.wgsl:168-233)
-    launch_fill(idx, s->mass, s->pos, n, s->child, s->parent, s->status, s->cm, s->aabb, st);
+    launch_fill(idx, s->mass, s->pos, n, s->tree, st);
     mark(5);
     // phase 5: walk + kick + drift (barnes_hut.wgsl:48-121)
     WalkArgs w{};
-    w.cm = s->cm;
-    w.child = s->child;
+    w.cm = s->tree.cm;
+    w.child = s->tree.child;
     w.idx = idx;
     w.keys64 = s->ensemble ? (const uint64_t *)s->keys[s->result_buf] : nullptr;
     w.sim_root = s->sim_root;
@@ -571,19 +573,19 @@ int nbody_debug_get(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
             need = n * 4;
             break;
         case NBODY_BUF_CHILDREN:
-            src = s->child;
+            src = s->tree.child;
             need = std::max<int64_t>(n - 1, 0) * 8;
             break;
         case NBODY_BUF_PARENT:
-            src = s->parent;
+            src = s->tree.parent;
             need = std::max<int64_t>(2 * n - 1, 0) * 4;
             break;
         case NBODY_BUF_NODE_CM:
-            src = s->cm;
+            src = s->tree.cm;
             need = std::max<int64_t>(2 * n - 1, 0) * 16;
             break;
         case NBODY_BUF_NODE_AABB:
-            src = s->aabb;
+            src = s->tree.aabb;
             need = std::max<int64_t>(2 * n - 1, 0) * 16;
             break;
         case NBODY_BUF_ACCEL:

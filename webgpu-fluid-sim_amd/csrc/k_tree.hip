@@ -8,12 +8,25 @@
 //   order climbs toward the root; the first thread to reach a parent stops,
 //   the second merges both children.  The reference's arrival counter is a
 //   relaxed atomic with workgroup-scope barriers, so the second arriver may
-//   read stale sibling data (README.md:18).  Here the counter is an
-//   agent-scope acquire-release fetch-add: the first arriver's node writes
-//   happen-before the second arriver's reads on any XCD, and the merge
-//   formula does not depend on which child arrives last, so the result is
-//   deterministic.  The reference's random leaf rotation (`offset`,
-//   lbvh.wgsl:171) is therefore unnecessary and not applied.
+//   read stale sibling data (README.md:18).
+//
+//   Here the climb is split so that every arrival counter is ordered at
+//   workgroup scope, which on a multi-XCD MI355X needs no L2 write-back or
+//   invalidate (agent-scope acquire/release would flush the XCD's L2 on
+//   every level of every thread's climb):
+//     phase 1 (fill_local_kernel): workgroup b owns sorted leaves
+//       [b*1024, b*1024+1023] and completes every internal node whose leaf
+//       range lies inside that block, with the arrival counters in LDS.  A
+//       thread that completes a node whose parent spans blocks appends the
+//       node to a frontier list.
+//     phase 2 (fill_top_kernel): one workgroup climbs from the frontier
+//       nodes through the block-spanning nodes (a few per block) with
+//       workgroup-scope counters in global memory; the kernel boundary makes
+//       phase 1's nodes visible to it.
+//   The merge formula does not depend on which child arrives last, so the
+//   result is deterministic and equals a sequential post-order fill.  The
+//   reference's random leaf rotation (`offset`, lbvh.wgsl:171) is therefore
+//   unnecessary and not applied.
 #include "nbody_internal.h"
 
 namespace nbs {
@@ -84,35 +97,33 @@ __device__ __forceinline__ Split karras_node(const Delta &delta, int64_t i, int6
 }
 
 __global__ __launch_bounds__(256) void build_u32_kernel(const uint32_t *__restrict__ keys,
-                                                        int64_t n, uint2 *__restrict__ child,
-                                                        uint32_t *__restrict__ parent,
-                                                        uint32_t *__restrict__ status) {
+                                                        int64_t n, TreeBufs tb) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n - 1) return;
     DeltaU32 delta{keys, n};
     const Split sp = karras_node(delta, i, n);
-    child[i] = make_uint2(sp.lc, sp.rc);
-    parent[sp.lc] = (uint32_t)i;  // lbvh.wgsl:155-161
-    parent[sp.rc] = (uint32_t)i;
-    if (i == 0) parent[0] = kNoNode;
-    status[i] = 0u;  // lbvh.wgsl:164
+    tb.child[i] = make_uint2(sp.lc, sp.rc);
+    tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
+    tb.parent[sp.lc] = (uint32_t)i;  // lbvh.wgsl:155-161
+    tb.parent[sp.rc] = (uint32_t)i;
+    if (i == 0) tb.parent[0] = kNoNode;
+    tb.status[i] = 0u;  // lbvh.wgsl:164
 }
 
 __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restrict__ keys,
                                                         int64_t n,
                                                         const int64_t *__restrict__ sim_off,
-                                                        uint2 *__restrict__ child,
-                                                        uint32_t *__restrict__ parent,
-                                                        uint32_t *__restrict__ status,
+                                                        TreeBufs tb,
                                                         uint32_t *__restrict__ sim_root) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n - 1) return;
     DeltaU64 delta{keys, sim_off, n};
     const Split sp = karras_node(delta, i, n);
-    child[i] = make_uint2(sp.lc, sp.rc);
-    parent[sp.lc] = (uint32_t)i;
-    parent[sp.rc] = (uint32_t)i;
-    status[i] = 0u;
+    tb.child[i] = make_uint2(sp.lc, sp.rc);
+    tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
+    tb.parent[sp.lc] = (uint32_t)i;
+    tb.parent[sp.rc] = (uint32_t)i;
+    tb.status[i] = 0u;
     // A node spanning two sims sits above the sim level; a child whose
     // range lies inside one sim is that sim's root.
     const uint64_t s_lo = keys[sp.lo] >> kSimShift, s_hi = keys[sp.hi] >> kSimShift;
@@ -121,81 +132,116 @@ __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restri
         if ((keys[sp.gamma + 1] >> kSimShift) == s_hi) sim_root[s_hi] = sp.rc;
     }
     if (i == 0) {
-        parent[0] = kNoNode;
+        tb.parent[0] = kNoNode;
         const uint64_t s0 = keys[0] >> kSimShift;
         if (s0 == (keys[n - 1] >> kSimShift)) sim_root[s0] = 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void fill_kernel(const uint32_t *__restrict__ idx,
-                                                   const float *__restrict__ mass,
-                                                   const float2 *__restrict__ pos, int64_t n,
-                                                   const uint2 *__restrict__ child,
-                                                   const uint32_t *__restrict__ parent,
-                                                   uint32_t *status, float4 *cm, float4 *aabb) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    uint32_t cur = (uint32_t)(n - 1 + t);
+constexpr int kFillBlock = 1024;
+
+// lbvh.wgsl:205-228: node p from its two complete children.
+__device__ __forceinline__ void merge_children(uint32_t p, const uint2 ch, float4 *cm,
+                                               float4 *aabb) {
+    const float4 L = cm[ch.x], R = cm[ch.y];
+    const float4 LB = aabb[ch.x], RB = aabb[ch.y];
+    float4 B;
+    B.x = fminf(LB.x, RB.x);
+    B.y = fminf(LB.y, RB.y);
+    B.z = fmaxf(LB.z, RB.z);
+    B.w = fmaxf(LB.w, RB.w);
+    float4 C;
+    C.z = L.z + R.z;
+    if (C.z > 0.0f) {
+        C.x = (L.x * L.z + R.x * R.z) / C.z;
+        C.y = (L.y * L.z + R.y * R.z) / C.z;
+    } else {
+        C.x = 0.0f;
+        C.y = 0.0f;
+    }
+    C.w = fmaxf(B.z - B.x, B.w - B.y);
+    aabb[p] = B;
+    cm[p] = C;
+}
+
+__global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *__restrict__ idx,
+                                                                const float *__restrict__ mass,
+                                                                const float2 *__restrict__ pos,
+                                                                int64_t n, TreeBufs tb) {
+    __shared__ uint32_t s_arrive[kFillBlock];
+    s_arrive[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kFillBlock;
+    const uint32_t last = (uint32_t)((int64_t)base + kFillBlock - 1 < n - 1 ? base + kFillBlock - 1
+                                                                             : n - 1);
+    const uint32_t t = base + threadIdx.x;
+    if (t >= (uint64_t)n) return;
+    uint32_t cur = (uint32_t)(n - 1) + t;
     const uint32_t b = idx[t];
     const float2 p = pos[b];
     const float m = mass[b];
     // lbvh.wgsl:179-187
-    cm[cur] = make_float4(p.x, p.y, m, 0.0f);
-    aabb[cur] = make_float4(p.x, p.y, p.x, p.y);
-    // lbvh.wgsl:191-232
+    tb.cm[cur] = make_float4(p.x, p.y, m, 0.0f);
+    tb.aabb[cur] = make_float4(p.x, p.y, p.x, p.y);
+    // lbvh.wgsl:191-232, restricted to nodes inside this block's leaf range.
     // A valid tree reaches the root in <= 96 levels; the level bound and the
     // index check only guard against a corrupt parent array.
     for (int level = 0; cur != 0u && level < 128; ++level) {
-        const uint32_t par = parent[cur];
+        const uint32_t par = tb.parent[cur];
         if (par >= (uint32_t)(n - 1)) break;
-        const uint32_t prev =
-            __hip_atomic_fetch_add(&status[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0u) break;
-        const uint2 ch = child[par];
-        const float4 L = cm[ch.x], R = cm[ch.y];
-        const float4 LB = aabb[ch.x], RB = aabb[ch.y];
-        float4 B;
-        B.x = fminf(LB.x, RB.x);
-        B.y = fminf(LB.y, RB.y);
-        B.z = fmaxf(LB.z, RB.z);
-        B.w = fmaxf(LB.w, RB.w);
-        float4 C;
-        C.z = L.z + R.z;
-        if (C.z > 0.0f) {
-            C.x = (L.x * L.z + R.x * R.z) / C.z;
-            C.y = (L.y * L.z + R.y * R.z) / C.z;
-        } else {
-            C.x = 0.0f;
-            C.y = 0.0f;
+        const uint2 r = tb.range[par];
+        if (r.x < base || r.y > last) {
+            const uint32_t slot = atomicAdd(tb.frontier_count, 1u);
+            tb.frontier[slot] = cur;
+            break;
         }
-        C.w = fmaxf(B.z - B.x, B.w - B.y);
-        aabb[par] = B;
-        cm[par] = C;
+        const uint32_t prev = __hip_atomic_fetch_add(&s_arrive[par - base], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prev == 0u) break;
+        merge_children(par, tb.child[par], tb.cm, tb.aabb);
         cur = par;
     }
 }
 
-}  // namespace
-
-void launch_build_u32(const uint32_t *keys, int64_t n, uint2 *child, uint32_t *parent,
-                      uint32_t *status, hipStream_t s) {
-    if (n < 2) return;
-    build_u32_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, child, parent, status);
+// One workgroup: climb from the frontier through the block-spanning nodes.
+__global__ __launch_bounds__(kFillBlock) void fill_top_kernel(int64_t n, TreeBufs tb) {
+    const uint32_t count = *tb.frontier_count;
+    for (uint32_t k = threadIdx.x; k < count; k += kFillBlock) {
+        uint32_t cur = tb.frontier[k];
+        for (int level = 0; cur != 0u && level < 128; ++level) {
+            const uint32_t par = tb.parent[cur];
+            if (par >= (uint32_t)(n - 1)) break;
+            const uint32_t prev = __hip_atomic_fetch_add(&tb.status[par], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == 0u) break;
+            merge_children(par, tb.child[par], tb.cm, tb.aabb);
+            cur = par;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *tb.frontier_count = 0u;  // ready for the next step
 }
 
-void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, uint2 *child,
-                      uint32_t *parent, uint32_t *status, uint32_t *sim_root, hipStream_t s) {
+}  // namespace
+
+void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, hipStream_t s) {
     if (n < 2) return;
-    build_u64_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, sim_off, child,
-                                                                     parent, status, sim_root);
+    build_u32_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, t);
+}
+
+void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
+                      uint32_t *sim_root, hipStream_t s) {
+    if (n < 2) return;
+    build_u64_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, sim_off, t,
+                                                                     sim_root);
 }
 
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
-                 const uint2 *child, const uint32_t *parent, uint32_t *status, float4 *cm,
-                 float4 *aabb, hipStream_t s) {
+                 const TreeBufs &t, hipStream_t s) {
     if (n < 1) return;
-    fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(idx, mass, pos, n, child, parent,
-                                                            status, cm, aabb);
+    fill_local_kernel<<<(unsigned)((n + kFillBlock - 1) / kFillBlock), kFillBlock, 0, s>>>(
+        idx, mass, pos, n, t);
+    if (n > kFillBlock) fill_top_kernel<<<1, kFillBlock, 0, s>>>(n, t);
 }
 
 }  // namespace nbs

@@ -52,13 +52,21 @@ int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32
                     int64_t n, int bits, const SortScratch &sc, hipStream_t s);
 
 // ---------------------------------------------------------------- tree
-void launch_build_u32(const uint32_t *keys, int64_t n, uint2 *child, uint32_t *parent,
-                      uint32_t *status, hipStream_t s);
-void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, uint2 *child,
-                      uint32_t *parent, uint32_t *status, uint32_t *sim_root, hipStream_t s);
+struct TreeBufs {
+    uint2 *child;        // [n-1] {left, right}
+    uint2 *range;        // [n-1] {first, last} sorted leaf of each internal node
+    uint32_t *parent;    // [2n-1]
+    uint32_t *status;    // [n-1] arrival counters of the fill
+    float4 *cm;          // [2n-1] {com.x, com.y, mass, length}
+    float4 *aabb;        // [2n-1] {min.x, min.y, max.x, max.y}
+    uint32_t *frontier;  // [n] nodes completed by the first fill phase whose parent spans blocks
+    uint32_t *frontier_count;
+};
+void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, hipStream_t s);
+void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
+                      uint32_t *sim_root, hipStream_t s);
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
-                 const uint2 *child, const uint32_t *parent, uint32_t *status, float4 *cm,
-                 float4 *aabb, hipStream_t s);
+                 const TreeBufs &t, hipStream_t s);
 
 // ---------------------------------------------------------------- walk
 struct WalkArgs {
