@@ -79,6 +79,104 @@ def cpu_baseline(mass, pos, vel, budget_s=15.0):
                       f"state, oracle C restatement, OpenMP {threads} threads, {el:.1f} s"}
 
 
+def timed_steps(sim, steps):
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    return time.perf_counter() - t0
+
+
+def plain_config(scenario, n, theta, dt, eps, steps, seed, **ic):
+    """body-steps/s of a plain configuration (graph replay, bodies resident)."""
+    import nbody_sim
+
+    mass, pos, vel = nbody_sim.generate(scenario, n, seed=seed, **ic)
+    cfg = nbody_sim.Config(bhTheta=theta, deltaTime=dt, epsilon=eps, softeningMode="constant",
+                           mortonMode="bbox30", mathMode="fast")
+    sim = nbody_sim.Simulation(cfg)
+    sim.setBodies(mass, pos, vel)
+    sim.useGraph(True)
+    sim.step(3)
+    el = timed_steps(sim, steps)
+    sim.close()
+    return {"n": n, "steps": steps, "value": n * steps / el, "ms_per_step": 1e3 * el / steps}
+
+
+def ensemble_config(steps=200, num_sims=256, seed=4):
+    """256 independent sims (N log-uniform in [1024, 65536], uniform disk or
+    Plummer) stepped as one ensemble vs a loop of plain per-sim runs."""
+    import numpy as np
+
+    import nbody_sim
+
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(1024), np.log(65536), num_sims))).astype(np.int64)
+    kinds = rng.random(num_sims) < 0.5
+    parts = [nbody_sim.generate("plummer" if k else "uniform-disk", int(nn), seed=1000 + i)
+             for i, (nn, k) in enumerate(zip(sizes, kinds))]
+    cfg = nbody_sim.Config(bhTheta=0.6, deltaTime=1e-4, epsilon=1e-3, softeningMode="constant",
+                           mortonMode="bbox30", mathMode="fast")
+    total = int(sizes.sum())
+    ens = nbody_sim.Ensemble(cfg, sizes)
+    ens.setBodies(*[np.concatenate([p[i] for p in parts]) for i in range(3)])
+    ens.useGraph(True)
+    ens.step(3)
+    el_ens = timed_steps(ens, steps)
+    ens.close()
+    sims = []
+    for p in parts:
+        s = nbody_sim.Simulation(cfg)
+        s.setBodies(*p)
+        s.useGraph(True)
+        s.step(1)
+        s.synchronize()
+        sims.append(s)
+    t0 = time.perf_counter()
+    for s in sims:  # one plain run after the other
+        s.step(steps)
+        s.synchronize()
+    el_loop = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    return {"num_sims": num_sims, "total_bodies": total, "steps": steps,
+            "value": total * steps / el_ens, "plain_loop_value": total * steps / el_loop,
+            "speedup_vs_plain_loop": el_loop / el_ens}
+
+
+def adjoint_config(n=65536, steps=200, every=20, seed=5):
+    """Adjoint (reverse sweep incl. segment recomputation) vs the forward run."""
+    import numpy as np
+
+    import nbody_sim
+
+    mass, pos, vel = nbody_sim.generate("exp-disk", n, seed=seed, scale=0.2, dispersion=0.05,
+                                        mass_sigma=0.5)
+    cfg = nbody_sim.Config(bhTheta=0.6, deltaTime=1e-4, epsilon=1e-3, softeningMode="constant",
+                           mortonMode="bbox30", mathMode="fast")
+    sim = nbody_sim.Simulation(cfg)
+    sim.setBodies(mass, pos, vel)
+    el_fwd = timed_steps(sim, steps)              # plain forward, eager launches
+    sim.setBodies(mass, pos, vel)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.adjointForward(steps, every)
+    sim.synchronize()
+    el_cp = time.perf_counter() - t0
+    _, xT, _ = sim.getBodies()
+    g = np.random.default_rng(0).normal(size=xT.shape).astype(np.float32)
+    t0 = time.perf_counter()
+    gx, gv, gm = sim.adjointBackward(g, None)
+    el_bwd = time.perf_counter() - t0
+    sim.close()
+    ok = bool(np.all(np.isfinite(gx)) and np.all(np.isfinite(gv)) and np.all(np.isfinite(gm)))
+    return {"n": n, "steps": steps, "checkpoint_every": every,
+            "forward_ms_per_step": 1e3 * el_fwd / steps,
+            "checkpointed_forward_ms_per_step": 1e3 * el_cp / steps,
+            "adjoint_ms_per_step": 1e3 * el_bwd / steps,
+            "adjoint_over_forward": el_bwd / el_fwd, "gradients_finite": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +185,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other configurations, the ensemble, adjoint and error legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -168,6 +268,16 @@ def main():
             "opened_per_step": st["opened"] / args.profile_steps,
         },
     }
+    if not args.no_extras:
+        log("small plain config")
+        out["plain_small_16k"] = plain_config("uniform-disk", 16384, 0.5, 1e-4, 1e-3, 100, 1)
+        log("collision config")
+        out["plain_collision_4M"] = plain_config("collision", 4194304, 0.7, 2e-5, 5e-4, 100, 3,
+                                                 scale=0.15)
+        log("ensemble")
+        out["ensemble_256"] = ensemble_config()
+        log("adjoint")
+        out["adjoint_65k"] = adjoint_config()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mass, pos, vel, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -148,6 +148,19 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  * (64..1024, multiple of 64) and walk stack entries per wave (8..256). */
 int nbody_set_tuning(nbody_sim *sim, int32_t walk_block, int32_t walk_stack);
 
+/* Adjoint of the step (no reference counterpart: the reference only steps
+ * forward).  nbody_adjoint_forward runs `nsteps` steps from the current state
+ * like nbody_step, keeping a (pos, vel) checkpoint every `every` steps.
+ * nbody_adjoint_backward then takes dL/dpos and dL/dvel of the final state
+ * (host arrays, NULL = zero) and returns dL/dpos, dL/dvel and dL/dmass of
+ * the initial state, with the tree topology and opening decisions of every
+ * step frozen (each segment is recomputed from its checkpoint and every
+ * step's tree rebuilt from its positions).  The final state is restored.
+ * Plain handles, constant softening, n >= 2. */
+int nbody_adjoint_forward(nbody_sim *sim, int32_t nsteps, int32_t every);
+int nbody_adjoint_backward(nbody_sim *sim, const float *gpos_T, const float *gvel_T, float *gpos0,
+                           float *gvel0, float *gmass);
+
 /* Message of the last failure on `sim` (NULL: the last failure of a call
  * without a usable handle).  Never NULL. */
 const char *nbody_last_error(const nbody_sim *sim);

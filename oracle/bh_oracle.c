@@ -366,6 +366,50 @@ void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32
     }
 }
 
+int64_t orc_walk_lists(int64_t n, const uint32_t *idx, const float *node_cm, const uint32_t *left,
+                       const uint32_t *right, const float *mass, const float *pos_xy,
+                       const orc_params *p, int64_t *offsets, uint32_t *nodes, int64_t cap) {
+    /* Same traversal and test as orc_walk (barnes_hut.wgsl:67-106). */
+    const float g = p->grav_constant, dt = p->delta_time, theta = p->bh_theta;
+    const float mult = p->epsilon_multiplier, epsc = p->epsilon;
+    const int ref_soft = p->softening_mode == ORC_SOFTEN_REFERENCE;
+    const uint32_t leaf0 = (uint32_t)(n - 1);
+    int64_t total = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && total > cap) return total;
+        int64_t w = 0;
+        for (int64_t t = 0; t < n; ++t) {
+            if (pass == 0) offsets[t] = total;
+            uint32_t stack[256];
+            uint32_t bi = idx[t];
+            float p1x = pos_xy[2 * (size_t)bi], p1y = pos_xy[2 * (size_t)bi + 1], m1 = mass[bi];
+            int sp = 0;
+            stack[0] = 0;
+            while (sp >= 0) {
+                uint32_t node = stack[sp--];
+                const float *C = node_cm + 4 * (size_t)node;
+                float rx = C[0] - p1x, ry = C[1] - p1y;
+                float d2 = rx * rx + ry * ry;
+                int leaf = node >= leaf0;
+                if (leaf && idx[node - leaf0] == bi) continue;
+                float eps = ref_soft ? mult * sqrtf(g * fmaxf(m1, C[2]) * dt) : epsc;
+                float inv = 1.0f / sqrtf(d2 + eps * eps);
+                if (leaf || C[3] * inv < theta) {
+                    if (pass == 0)
+                        total++;
+                    else
+                        nodes[w++] = node;
+                } else if (sp + 2 < 256) {
+                    stack[++sp] = left[node];
+                    stack[++sp] = right[node];
+                }
+            }
+        }
+        if (pass == 0) offsets[n] = total;
+    }
+    return total;
+}
+
 void orc_kick_drift(int64_t n, const float *accel_xy, float dt, float *pos_xy, float *vel_xy) {
     for (int64_t i = 0; i < 2 * n; ++i) vel_xy[i] = vel_xy[i] + accel_xy[i] * dt; /* barnes_hut.wgsl:110 */
     for (int64_t i = 0; i < 2 * n; ++i) pos_xy[i] = pos_xy[i] + vel_xy[i] * dt;   /* barnes_hut.wgsl:120 */

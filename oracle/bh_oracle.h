@@ -92,6 +92,15 @@ void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32
               const float *pos_xy, const orc_params *p, int stack_size, float *accel_xy,
               int64_t *stats, int threads);
 
+/* The nodes each sorted position's walk accepts (barnes_hut.wgsl:94), in
+ * walk order, as CSR: offsets[n+1], nodes[offsets[t]..offsets[t+1]).
+ * Returns the total count; when it exceeds `cap` nothing is written to
+ * `nodes` and the caller retries with a larger buffer.  Used to freeze the
+ * opening decisions of a step for the adjoint's FP64 reference. */
+int64_t orc_walk_lists(int64_t n, const uint32_t *idx, const float *node_cm, const uint32_t *left,
+                       const uint32_t *right, const float *mass, const float *pos_xy,
+                       const orc_params *p, int64_t *offsets, uint32_t *nodes, int64_t cap);
+
 /* barnes_hut.wgsl:110 then barnes_hut.wgsl:113-121: v += a*dt, x += v*dt. */
 void orc_kick_drift(int64_t n, const float *accel_xy, float dt, float *pos_xy, float *vel_xy);
 

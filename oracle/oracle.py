@@ -258,3 +258,27 @@ def direct_sum_f64(pos, mass, params: Params, targets=None, threads=None):
                              ctypes.c_int64(nt), _p(acc),
                              ctypes.c_int(threads or _threads()))
     return acc
+
+
+def walk_lists(idx, cm, left, right, mass, pos, params: Params):
+    """Accepted nodes of each sorted position's walk (CSR offsets, nodes)."""
+    idx = np.ascontiguousarray(idx, np.uint32)
+    cm = _f32(cm, (-1, 4))
+    left = np.ascontiguousarray(left, np.uint32)
+    right = np.ascontiguousarray(right, np.uint32)
+    mass = _f32(mass)
+    pos = _f32(pos, (-1, 2))
+    n = idx.shape[0]
+    off = np.zeros(n + 1, np.int64)
+    pr = params.c()
+    f = lib().orc_walk_lists
+    f.restype = ctypes.c_int64
+    cap = 0
+    nodes = np.zeros(1, np.uint32)
+    while True:
+        total = f(ctypes.c_int64(n), _p(idx), _p(cm), _p(left), _p(right), _p(mass), _p(pos),
+                  ctypes.byref(pr), _p(off), _p(nodes), ctypes.c_int64(cap))
+        if total <= cap:
+            return off, nodes[:total]
+        cap = int(total)
+        nodes = np.zeros(cap, np.uint32)

@@ -1,0 +1,150 @@
+"""The hand-written adjoint (through the C ABI) against FP64 references:
+torch autograd of the frozen-decision Barnes-Hut trajectory, FP64 central
+finite differences of it (h = 1e-4 on positions and velocities), and
+autograd of an O(N^2) direct-sum trajectory at N = 4096 (theta = 0, where the
+Barnes-Hut walk is the direct sum)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import nbody_sim  # noqa: E402
+from frozen_bh import FrozenTrajectory, direct_trajectory  # noqa: E402
+from helpers import params  # noqa: E402
+from nbody_sim.adjoint import trajectory  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def gpu_grads(cfg, mass, pos, vel, steps, every, wx, wv):
+    sim = nbody_sim.Simulation(cfg)
+    sim.setBodies(mass, pos, vel)
+    sim.adjointForward(steps, every)
+    _, pT, vT = sim.getBodies()
+    g = sim.adjointBackward(wx, wv)
+    _, pT2, _ = sim.getBodies()
+    sim.close()
+    assert np.array_equal(pT, pT2)  # final state restored
+    return g, pT, vT
+
+
+@pytest.mark.parametrize("math", ["exact", "fast"])
+def test_adjoint_matches_frozen_fp64_autograd(math):
+    n, steps, every = 2048, 10, 4
+    mass, pos, vel = nbody_sim.generate("exp-disk", n, seed=6, dispersion=0.05, mass_sigma=0.5)
+    cfg, P = params(theta=0.6, dt=1e-4, eps=1e-3)
+    cfg = cfg.with_(mathMode=math)
+    rng = np.random.default_rng(0)
+    wx = rng.normal(size=(n, 2)).astype(np.float32)
+    wv = (0.3 * rng.normal(size=(n, 2))).astype(np.float32)
+    (gx, gv, gm), pT, _ = gpu_grads(cfg, mass, pos, vel, steps, every, wx, wv)
+
+    F = FrozenTrajectory(mass, pos, vel, P, steps)
+    if math == "exact":
+        assert np.array_equal(pT, F.final32[0])  # same decisions as the oracle
+    x0 = torch.from_numpy(pos.astype(np.float64)).requires_grad_()
+    v0 = torch.from_numpy(vel.astype(np.float64)).requires_grad_()
+    m0 = torch.from_numpy(mass.astype(np.float64)).requires_grad_()
+    xt, vt = F.run(x0, v0, m0)
+    L = (torch.from_numpy(wx.astype(np.float64)) * xt).sum() + \
+        (torch.from_numpy(wv.astype(np.float64)) * vt).sum()
+    rx, rv, rm = (t.numpy() for t in torch.autograd.grad(L, (x0, v0, m0)))
+    ex, ev, em = rel(gx, rx), rel(gv, rv), rel(gm, rm)
+    print(f"adjoint vs FP64 autograd ({math}): pos {ex:.2e} vel {ev:.2e} mass {em:.2e}")
+    tol = 1e-4 if math == "exact" else 1e-3
+    assert ex < tol and ev < tol and em < tol
+
+
+def test_adjoint_central_differences_64_coordinates():
+    """FP64 central differences (h = 1e-4) of the frozen trajectory at 64 random
+    coordinates; the loss is the mean squared distance to target positions from
+    a run with perturbed initial velocities."""
+    n, steps, every = 1024, 8, 3
+    mass, pos, vel = nbody_sim.generate("exp-disk", n, seed=8, dispersion=0.05, mass_sigma=0.5)
+    cfg, P = params(theta=0.6, dt=1e-4, eps=1e-3)
+    cfg = cfg.with_(mathMode="exact")
+    rng = np.random.default_rng(3)
+    vel_t = (vel + rng.normal(0, 0.01, vel.shape)).astype(np.float32)
+    sim = nbody_sim.Simulation(cfg)
+    sim.setBodies(mass, pos, vel_t)
+    sim.step(steps)
+    _, target, _ = sim.getBodies()
+    sim.close()
+
+    F = FrozenTrajectory(mass, pos, vel, P, steps)
+    xT32 = F.final32[0]
+    gx_T = (2.0 / n) * (xT32.astype(np.float64) - target)
+    (gx, gv, gm), _, _ = gpu_grads(cfg, mass, pos, vel, steps, every, gx_T.astype(np.float32), None)
+
+    tgt = torch.from_numpy(target.astype(np.float64))
+
+    def loss(x, v, m):
+        xt, _ = F.run(x, v, m)
+        return (((xt - tgt) ** 2).sum(1)).mean().item()
+
+    base = [torch.from_numpy(a.astype(np.float64)) for a in (pos, vel, mass)]
+    grads = {"x": gx.reshape(-1), "v": gv.reshape(-1), "m": gm}
+    picks = [("x", k) for k in rng.choice(2 * n, 24, replace=False)] + \
+            [("v", k) for k in rng.choice(2 * n, 24, replace=False)] + \
+            [("m", k) for k in rng.choice(n, 16, replace=False)]
+    worst = 0.0
+    scale = {w: np.abs(grads[w]).max() for w in grads}
+    for which, k in picks:
+        i = "xvm".index(which)
+        flat = base[i].reshape(-1)
+        h = 1e-4 if which != "m" else 1e-4 * float(flat[k])
+        args = [b.clone() for b in base]
+        args[i].reshape(-1)[k] += h
+        lp = loss(*args)
+        args[i].reshape(-1)[k] -= 2 * h
+        lm = loss(*args)
+        fd = (lp - lm) / (2 * h)
+        err = abs(fd - grads[which][k]) / scale[which]
+        worst = max(worst, err)
+    print(f"adjoint vs FP64 central differences: worst error / max|grad| = {worst:.2e}")
+    assert worst < 1e-3
+
+
+def test_adjoint_theta0_matches_direct_sum_autograd_n4096():
+    n, steps = 4096, 3
+    mass, pos, vel = nbody_sim.generate("uniform-disk", n, seed=9)
+    vel = (0.1 * np.random.default_rng(1).normal(size=(n, 2))).astype(np.float32)
+    cfg, _ = params(theta=0.0, dt=1e-4, eps=1e-3)
+    rng = np.random.default_rng(2)
+    wx = rng.normal(size=(n, 2)).astype(np.float32)
+    (gx, gv, gm), _, _ = gpu_grads(cfg.with_(mathMode="fast"), mass, pos, vel, steps, 2, wx, None)
+    x0 = torch.from_numpy(pos.astype(np.float64)).requires_grad_()
+    v0 = torch.from_numpy(vel.astype(np.float64)).requires_grad_()
+    m0 = torch.from_numpy(mass.astype(np.float64)).requires_grad_()
+    xt, _ = direct_trajectory(x0, v0, m0, 1.0, float(np.float32(1e-3)), float(np.float32(1e-4)),
+                              steps)
+    L = (torch.from_numpy(wx.astype(np.float64)) * xt).sum()
+    rx, rv, rm = (t.numpy() for t in torch.autograd.grad(L, (x0, v0, m0)))
+    ex, ev, em = rel(gx, rx), rel(gv, rv), rel(gm, rm)
+    print(f"adjoint (theta=0) vs direct-sum autograd: pos {ex:.2e} vel {ev:.2e} mass {em:.2e}")
+    assert ex < 1e-3 and ev < 1e-3 and em < 1e-3
+
+
+def test_autograd_function_wrapper():
+    n = 512
+    mass, pos, vel = nbody_sim.generate("plummer", n, seed=4)
+    cfg, _ = params(theta=0.6, dt=1e-4, eps=1e-2)
+    sim = nbody_sim.Simulation(cfg)
+    x0 = torch.from_numpy(pos).requires_grad_()
+    v0 = torch.from_numpy(vel).requires_grad_()
+    m0 = torch.from_numpy(mass).requires_grad_()
+    xT, vT = trajectory(x0, v0, m0, sim, 5, 2)
+    L = (xT ** 2).sum() + (vT ** 2).sum()
+    L.backward()
+    gx, gv, gm = sim.adjointBackward(2 * xT.detach().numpy(), 2 * vT.detach().numpy())
+    sim.close()
+    # node cotangents are summed with float atomics: equal up to summation order
+    assert np.allclose(x0.grad.numpy(), gx, rtol=1e-4, atol=1e-6 * np.abs(gx).max())
+    assert np.allclose(m0.grad.numpy(), gm, rtol=1e-4, atol=1e-6 * np.abs(gm).max())
+    assert np.all(np.isfinite(gv))

@@ -8,9 +8,10 @@
 // HBM layout (capacity C bodies, all 256-byte aligned):
 //   mass f32[C], pos float2[C], vel float2[C]      -- by body index
 //   keys[2] u32|u64[C], vals[2] u32[C]             -- sort ping-pong
-//   cm float4[2C-1] {com.x, com.y, mass, length}   -- what the walk reads
-//   aabb float4[2C-1]                              -- what the fill merges
-//   child uint2[C-1], range uint2[C-1], parent u32[2C-1], status u32[C-1]
+//   leaf float4[C] {x, y, mass, 0} in sorted order   -- the walk's own bodies
+//   pcm float4[2C] children's {com, mass, length} per internal node (walk)
+//   paabb float4[2C] children's AABBs per internal node (fill)
+//   root float4[2], child uint2[C], range uint2[C], parent u32[2C-1], status u32[C]
 //   fill frontier u32[C], sort scratch, bbox partials, debug accel float2[C],
 //   counters
 #include <hip/hip_runtime.h>
@@ -79,6 +80,7 @@ struct nbody_sim {
 
     int32_t debug = 0;
     int walk_block = 256, walk_stack = 128;
+    bool xcd_remap = true;
 
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;
@@ -89,6 +91,16 @@ struct nbody_sim {
     hipEvent_t ev[kNumPhases + 1] = {};
     float phase_ms[kNumPhases] = {};
     int64_t profiled_steps = 0;
+
+    // adjoint state (allocated by nbody_adjoint_forward)
+    void *adj_base = nullptr;
+    int64_t adj_n = 0;
+    int32_t adj_steps = 0, adj_every = 0, adj_cps = 0;
+    float2 *adj_cp = nullptr;    // [cps][2][n] checkpoints (pos, vel)
+    float2 *adj_seg = nullptr;   // [every][n] positions inside a segment
+    float2 *adj_lx = nullptr, *adj_lv = nullptr, *adj_la = nullptr, *adj_end = nullptr;
+    float *adj_lm = nullptr;
+    float4 *adj_cot = nullptr;
 
     std::string last_error;
 };
@@ -167,8 +179,10 @@ int allocate(nbody_sim *s, int64_t cap) {
             s->keys[b] = key_size == 8 ? (void *)a.take<uint64_t>(cap) : (void *)a.take<uint32_t>(cap);
             s->vals[b] = a.take<uint32_t>(cap);
         }
-        s->tree.cm = a.take<float4>(nodes);
-        s->tree.aabb = a.take<float4>(nodes);
+        s->tree.leaf = a.take<float4>(cap);
+        s->tree.pcm = a.take<float4>(2 * cap);
+        s->tree.paabb = a.take<float4>(2 * cap);
+        s->tree.root = a.take<float4>(2);
         s->tree.child = a.take<uint2>(cap);
         s->tree.range = a.take<uint2>(cap);
         s->tree.parent = a.take<uint32_t>(nodes);
@@ -205,13 +219,14 @@ int allocate(nbody_sim *s, int64_t cap) {
     return NBODY_OK;
 }
 
-// Enqueue one step: simulation.ts:393-421.
-int enqueue_step(nbody_sim *s) {
+// Enqueue one step: simulation.ts:393-421 (tree_only: phases 0-4, which
+// rebuild the tree of the current positions without moving the bodies).
+int enqueue_step(nbody_sim *s, bool tree_only = false) {
     const int64_t n = s->n;
     if (n <= 0) return NBODY_OK;
     hipStream_t st = s->stream;
     const StepParams p = step_params(s->cfg);
-    const bool prof = s->profiling && !s->use_graph;
+    const bool prof = s->profiling && !s->use_graph && !tree_only;
     auto mark = [&](int i) {
         if (prof) (void)hipEventRecord(s->ev[i], st);
     };
@@ -245,9 +260,13 @@ int enqueue_step(nbody_sim *s) {
     // phase 4: bottom-up fill (lbvh.wgsl:168-233)
     launch_fill(idx, s->mass, s->pos, n, s->tree, st);
     mark(5);
+    if (tree_only) return NBODY_OK;
     // phase 5: walk + kick + drift (barnes_hut.wgsl:48-121)
     WalkArgs w{};
-    w.cm = s->tree.cm;
+    w.leaf = s->tree.leaf;
+    w.pcm = s->tree.pcm;
+    w.root = s->tree.root;
+    w.parent = s->tree.parent;
     w.child = s->tree.child;
     w.idx = idx;
     w.keys64 = s->ensemble ? (const uint64_t *)s->keys[s->result_buf] : nullptr;
@@ -260,7 +279,7 @@ int enqueue_step(nbody_sim *s) {
     w.n = (uint32_t)n;
     w.p = p;
     launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
-                st);
+                s->xcd_remap, st);
     mark(6);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(s, e, "kernel launch");
@@ -357,6 +376,39 @@ nbody_sim *create_common(const nbody_config *cfg, int32_t device, int *err, bool
     return s;
 }
 
+// Per-node buffers in the reference's numbering (internal [0, n-2], leaves
+// [n-1, 2n-2]), reassembled on the host from the child-pair layout.
+int debug_node_buffer(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
+    const int64_t n = s->n;
+    const int64_t nodes = std::max<int64_t>(2 * n - 1, 0);
+    const int64_t need = nodes * (which == NBODY_BUF_PARENT ? 4 : 16);
+    if (bytes < need)
+        return fail(s, NBODY_ERR_INVALID_ARG,
+                    "destination too small: need " + std::to_string(need) + " bytes");
+    if (n <= 0) return NBODY_OK;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> parent((size_t)nodes);
+    HIP_TRY(s, hipMemcpy(parent.data(), s->tree.parent, 4 * (size_t)nodes, hipMemcpyDeviceToHost));
+    if (which == NBODY_BUF_PARENT) {
+        for (auto &v : parent)
+            if (v != kNoNode) v &= kParentMask;
+        std::memcpy(dst, parent.data(), 4 * (size_t)nodes);
+        return NBODY_OK;
+    }
+    std::vector<float4> pair((size_t)std::max<int64_t>(2 * (n - 1), 1)), root(2);
+    const float4 *src = which == NBODY_BUF_NODE_CM ? s->tree.pcm : s->tree.paabb;
+    if (n > 1)
+        HIP_TRY(s, hipMemcpy(pair.data(), src, 32 * (size_t)(n - 1), hipMemcpyDeviceToHost));
+    HIP_TRY(s, hipMemcpy(root.data(), s->tree.root, 32, hipMemcpyDeviceToHost));
+    float4 *out = static_cast<float4 *>(dst);
+    out[0] = which == NBODY_BUF_NODE_CM ? root[0] : root[1];
+    for (int64_t k = 1; k < nodes; ++k) {
+        const uint32_t pw = parent[(size_t)k];
+        out[k] = pair[2 * (size_t)(pw & kParentMask) + (pw >> 31)];
+    }
+    return NBODY_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -411,6 +463,7 @@ int nbody_destroy(nbody_sim *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     release_graph(s);
     if (s->arena.base) (void)hipFree(s->arena.base);
+    if (s->adj_base) (void)hipFree(s->adj_base);
     for (auto &ev : s->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
@@ -577,17 +630,9 @@ int nbody_debug_get(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
             need = std::max<int64_t>(n - 1, 0) * 8;
             break;
         case NBODY_BUF_PARENT:
-            src = s->tree.parent;
-            need = std::max<int64_t>(2 * n - 1, 0) * 4;
-            break;
         case NBODY_BUF_NODE_CM:
-            src = s->tree.cm;
-            need = std::max<int64_t>(2 * n - 1, 0) * 16;
-            break;
         case NBODY_BUF_NODE_AABB:
-            src = s->tree.aabb;
-            need = std::max<int64_t>(2 * n - 1, 0) * 16;
-            break;
+            return debug_node_buffer(s, which, dst, bytes);
         case NBODY_BUF_ACCEL:
             src = s->accel;
             need = n * 8;
@@ -656,6 +701,135 @@ int nbody_set_tuning(nbody_sim *s, int32_t walk_block, int32_t walk_stack) {
     }
     s->graph_dirty = true;
     return NBODY_OK;
+}
+
+int nbody_adjoint_forward(nbody_sim *s, int32_t nsteps, int32_t every) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (nsteps < 1 || every < 1) return fail(s, NBODY_ERR_INVALID_ARG, "nsteps and every must be >= 1");
+    if (s->ensemble) return fail(s, NBODY_ERR_INVALID_ARG, "the adjoint supports plain handles only");
+    if (s->cfg.softening_mode != NBODY_SOFTEN_CONSTANT)
+        return fail(s, NBODY_ERR_INVALID_ARG, "the adjoint needs constant softening");
+    if (s->n < 2) return fail(s, NBODY_ERR_INVALID_ARG, "the adjoint needs at least two bodies");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const int64_t n = s->n;
+    const int32_t cps = (nsteps + every - 1) / every;
+    // (re)allocate: checkpoints, segment positions, cotangents
+    if (s->adj_base) {
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        HIP_TRY(s, hipFree(s->adj_base));
+        s->adj_base = nullptr;
+    }
+    Arena a;
+    for (int pass = 0; pass < 2; ++pass) {
+        a.used = 0;
+        s->adj_cp = a.take<float2>((int64_t)cps * 2 * n);
+        s->adj_seg = a.take<float2>((int64_t)every * n);
+        s->adj_lx = a.take<float2>(n);
+        s->adj_lv = a.take<float2>(n);
+        s->adj_la = a.take<float2>(n);
+        s->adj_end = a.take<float2>(2 * n);
+        s->adj_lm = a.take<float>(n);
+        s->adj_cot = a.take<float4>(2 * n);
+        if (pass == 0) {
+            void *p = nullptr;
+            hipError_t e = hipMalloc(&p, a.used);
+            if (e != hipSuccess) return hip_fail(s, e, "hipMalloc (adjoint)");
+            a.base = static_cast<char *>(p);
+            s->adj_base = p;
+        }
+    }
+    s->adj_n = n;
+    s->adj_steps = nsteps;
+    s->adj_every = every;
+    s->adj_cps = cps;
+    for (int32_t k = 0; k < nsteps; ++k) {
+        if (k % every == 0) {
+            float2 *cp = s->adj_cp + (size_t)(k / every) * 2 * n;
+            HIP_TRY(s, hipMemcpyAsync(cp, s->pos, 8 * (size_t)n, hipMemcpyDeviceToDevice, s->stream));
+            HIP_TRY(s, hipMemcpyAsync(cp + n, s->vel, 8 * (size_t)n, hipMemcpyDeviceToDevice, s->stream));
+        }
+        int rc = enqueue_step(s);
+        if (rc != NBODY_OK) return rc;
+    }
+    return NBODY_OK;
+}
+
+int nbody_adjoint_backward(nbody_sim *s, const float *gpos_T, const float *gvel_T, float *gpos0,
+                           float *gvel0, float *gmass) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (!s->adj_base || s->adj_n != s->n)
+        return fail(s, NBODY_ERR_STATE, "call nbody_adjoint_forward first (same bodies)");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const int64_t n = s->n;
+    const float dt = s->cfg.delta_time;
+    hipStream_t st = s->stream;
+    // Final state, restored at the end.
+    HIP_TRY(s, hipMemcpyAsync(s->adj_end, s->pos, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(s, hipMemcpyAsync(s->adj_end + n, s->vel, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    if (gpos_T)
+        HIP_TRY(s, hipMemcpyAsync(s->adj_lx, gpos_T, 8 * (size_t)n, hipMemcpyHostToDevice, st));
+    else
+        HIP_TRY(s, hipMemsetAsync(s->adj_lx, 0, 8 * (size_t)n, st));
+    if (gvel_T)
+        HIP_TRY(s, hipMemcpyAsync(s->adj_lv, gvel_T, 8 * (size_t)n, hipMemcpyHostToDevice, st));
+    else
+        HIP_TRY(s, hipMemsetAsync(s->adj_lv, 0, 8 * (size_t)n, st));
+    HIP_TRY(s, hipMemsetAsync(s->adj_lm, 0, 4 * (size_t)n, st));
+
+    AdjointArgs a{};
+    a.leaf = s->tree.leaf;
+    a.pcm = s->tree.pcm;
+    a.root = s->tree.root;
+    a.parent = s->tree.parent;
+    a.child = s->tree.child;
+    a.la = s->adj_la;
+    a.lx = s->adj_lx;
+    a.lm = s->adj_lm;
+    a.cot = s->adj_cot;
+    a.overflow = s->overflow;
+    a.n = (uint32_t)n;
+    a.grav = s->cfg.grav_constant;
+    a.theta = s->cfg.bh_theta;
+    a.eps = s->cfg.epsilon;
+    const bool exact = s->cfg.math_mode == NBODY_MATH_EXACT;
+
+    for (int32_t seg = s->adj_cps - 1; seg >= 0; --seg) {
+        const int32_t k0 = seg * s->adj_every;
+        const int32_t len = std::min(s->adj_every, s->adj_steps - k0);
+        // Recompute the segment forward from its checkpoint, keeping x_k.
+        const float2 *cp = s->adj_cp + (size_t)seg * 2 * n;
+        HIP_TRY(s, hipMemcpyAsync(s->pos, cp, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(s, hipMemcpyAsync(s->vel, cp + n, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+        for (int32_t i = 0; i < len; ++i) {
+            HIP_TRY(s, hipMemcpyAsync(s->adj_seg + (size_t)i * n, s->pos, 8 * (size_t)n,
+                                      hipMemcpyDeviceToDevice, st));
+            if (i + 1 < len) {
+                int rc = enqueue_step(s);
+                if (rc != NBODY_OK) return rc;
+            }
+        }
+        // Reverse sweep over the segment.
+        for (int32_t i = len - 1; i >= 0; --i) {
+            HIP_TRY(s, hipMemcpyAsync(s->pos, s->adj_seg + (size_t)i * n, 8 * (size_t)n,
+                                      hipMemcpyDeviceToDevice, st));
+            int rc = enqueue_step(s, /*tree_only=*/true);
+            if (rc != NBODY_OK) return rc;
+            a.idx = s->vals[s->result_buf];
+            launch_adjoint_pre(s->adj_lx, s->adj_lv, s->adj_la, (uint32_t)n, dt, st);
+            launch_clear_u32(reinterpret_cast<uint32_t *>(s->adj_cot), 4 * (2 * n - 1), st);
+            launch_adjoint_walk(a, exact, s->walk_stack, st);
+            launch_adjoint_fold(a, st);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(s, e, "adjoint kernel launch");
+    if (gpos0) HIP_TRY(s, hipMemcpyAsync(gpos0, s->adj_lx, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+    if (gvel0) HIP_TRY(s, hipMemcpyAsync(gvel0, s->adj_lv, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+    if (gmass) HIP_TRY(s, hipMemcpyAsync(gmass, s->adj_lm, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(s, hipMemcpyAsync(s->pos, s->adj_end, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(s, hipMemcpyAsync(s->vel, s->adj_end + n, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    s->graph_dirty = true;
+    return nbody_synchronize(s);
 }
 
 const char *nbody_last_error(const nbody_sim *s) {

@@ -1,0 +1,244 @@
+// k_adjoint.hip -- reverse-mode (adjoint) kernels of the Barnes-Hut step.
+//
+// No reference counterpart: the reference only steps forward
+// (src/simulation.ts:386-425).  The adjoint differentiates the step with the
+// tree topology and every opening decision frozen at their forward values:
+// the tree is rebuilt from the checkpointed positions of the step (the
+// pipeline is deterministic, so it is the same tree) and walked with the
+// forward walk's own traversal and test.
+//
+// Forward step (constant softening):  a_i = sum_n G M_n r_in s^-3/2,
+//   r_in = c_n - x_i, s = |r|^2 + eps^2;  v' = v + a dt;  x' = x + v' dt.
+// Reverse step, given (lx, lv) = dL/d(x', v'):
+//   lv <- lv + lx dt;  la = lv dt                        (adjoint_pre_kernel)
+//   per accepted (i, n), g = la_i:
+//     h = G M_n (s^-3/2 g - 3 s^-5/2 (r.g) r)
+//     lx_i -= h;  lc_n += h;  lM_n += G s^-3/2 (r.g)    (adjoint_walk_kernel)
+//   node cotangents to leaves through c_n = sum m_j x_j / M_n, M_n = sum m_j:
+//     lx_j += m_j sum_{n ∋ j} lc_n / M_n
+//     lm_j += sum_{n ∋ j} [ x_j . lc_n / M_n - c_n . lc_n / M_n + lM_n ]
+//                                                        (adjoint_fold_kernel)
+#include "nbody_internal.h"
+
+namespace nbs {
+
+namespace {
+
+template <typename T>
+using const_as = const __attribute__((address_space(4))) T;
+
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = uniform_u32((uint32_t)v), hi = uniform_u32((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void adjoint_pre_kernel(const float2 *__restrict__ lx,
+                                                          float2 *__restrict__ lv,
+                                                          float2 *__restrict__ la, uint32_t n,
+                                                          float dt) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float2 x = lx[i];
+    float2 v = lv[i];
+    v.x = __builtin_fmaf(x.x, dt, v.x);
+    v.y = __builtin_fmaf(x.y, dt, v.y);
+    lv[i] = v;
+    la[i] = make_float2(v.x * dt, v.y * dt);
+}
+
+// Same traversal and opening test as walk_kernel (k_walk.hip), different
+// action on accepted nodes.
+template <bool kExact>
+__global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, int stack_cap) {
+    extern __shared__ float4 s_dyn[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int waves = blockDim.x >> 6;
+    float4 *s_data = s_dyn + (size_t)wid * stack_cap;
+    uint64_t *s_mask = reinterpret_cast<uint64_t *>(s_dyn + (size_t)waves * stack_cap) +
+                       (size_t)wid * stack_cap;
+    uint32_t *s_node = reinterpret_cast<uint32_t *>(reinterpret_cast<uint64_t *>(
+                           s_dyn + (size_t)waves * stack_cap) + (size_t)waves * stack_cap) +
+                       (size_t)wid * stack_cap;
+
+    const uint32_t n = a.n;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < n;
+    const uint32_t leaf0 = n - 1u;
+    const uint32_t my_leaf = leaf0 + t;
+    const_as<uint32_t> *child_s = (const_as<uint32_t> *)a.child;
+    const_as<float> *pcmf_s = (const_as<float> *)a.pcm;
+
+    float px = 0.0f, py = 0.0f;
+    float gx = 0.0f, gy = 0.0f;
+    uint32_t bi = 0;
+    if (valid) {
+        const float4 me = a.leaf[t];
+        px = me.x;
+        py = me.y;
+        bi = a.idx[t];
+        const float2 g = a.la[bi];
+        gx = g.x;
+        gy = g.y;
+    }
+    const uint64_t live = __ballot(valid);
+    if (live == 0) return;
+
+    const float G = a.grav, theta = a.theta, eps2 = a.eps * a.eps;
+    float hx_sum = 0.0f, hy_sum = 0.0f;
+    int sp = 0;
+    uint32_t node = 0u;
+    uint64_t mask = live;
+    float4 c = a.root[0];
+    const uint32_t max_iter = 4u * n + 64u;
+    for (uint32_t iter = 0;; ++iter) {
+        if (iter >= max_iter) {
+            if (lane == 0) atomicOr(a.overflow, 1u);
+            break;
+        }
+        const bool leaf = node >= leaf0;
+        bool open = false, acc = false;
+        float hx = 0.0f, hy = 0.0f, dm = 0.0f;
+        if (((mask >> lane) & 1u) && !(leaf && node == my_leaf)) {
+            const float rx = c.x - px, ry = c.y - py;
+            float inv;
+            if constexpr (kExact) {
+                const float d2 = rx * rx + ry * ry;
+                inv = 1.0f / sqrtf(d2 + eps2);
+            } else {
+                inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2)));
+            }
+            if (leaf || c.w * inv < theta) {
+                acc = true;
+                const float inv2 = inv * inv;
+                const float inv3 = inv2 * inv;
+                const float rg = rx * gx + ry * gy;
+                const float gm = G * c.z;
+                const float k3 = 3.0f * inv2 * rg;
+                hx = gm * inv3 * (gx - k3 * rx);
+                hy = gm * inv3 * (gy - k3 * ry);
+                dm = G * inv3 * rg;
+                hx_sum += hx;
+                hy_sum += hy;
+            } else {
+                open = true;
+            }
+        }
+        // Node cotangent: one atomic per wavefront and component.
+        if (__ballot(acc)) {
+            const float sx = wave_sum(hx), sy = wave_sum(hy), sm = wave_sum(dm);
+            if (lane == 0) {
+                float4 *cot = a.cot + node;
+                atomicAdd(&cot->x, sx);
+                atomicAdd(&cot->y, sy);
+                atomicAdd(&cot->z, sm);
+            }
+        }
+        const uint64_t om = __ballot(open);
+        if (om) {
+            const uint32_t pn = node;
+            const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
+                                          pcmf_s[8 * pn + 3]);
+            const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5],
+                                          pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
+            const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
+            if (sp < stack_cap) {
+                if (lane == 0) {
+                    s_node[sp] = chl;
+                    s_mask[sp] = om;
+                    s_data[sp] = cl;
+                }
+                ++sp;
+                node = chr;
+                mask = om;
+                c = cr;
+                continue;
+            }
+            if (lane == 0) atomicOr(a.overflow, 1u);
+        }
+        if (sp == 0) break;
+        --sp;
+        node = uniform_u32(s_node[sp]);
+        mask = uniform_u64(s_mask[sp]);
+        c = s_data[sp];
+    }
+    if (valid) {
+        float2 l = a.lx[bi];
+        l.x -= hx_sum;
+        l.y -= hy_sum;
+        a.lx[bi] = l;
+    }
+}
+
+// Record {com, mass, length} of a node: root record, leaf record, or the
+// child slot in its parent's pair.
+__device__ __forceinline__ float4 node_data(const AdjointArgs &a, uint32_t node) {
+    if (node == 0u) return a.root[0];
+    const uint32_t pw = a.parent[node];
+    return a.pcm[2u * (pw & kParentMask) + (pw >> 31)];
+}
+
+// One thread per sorted leaf: sum lc_n / M_n and lM_n - c_n . lc_n / M_n over
+// the leaf and its ancestors, then apply to the body (see the header).
+__global__ __launch_bounds__(256) void adjoint_fold_kernel(const AdjointArgs a) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = a.n;
+    if (t >= n) return;
+    const float4 me = a.leaf[t];
+    uint32_t cur = n - 1u + t;
+    float pax = 0.0f, pay = 0.0f, pb = 0.0f;
+    for (int level = 0; level < 128; ++level) {
+        const float4 c = (cur == n - 1u + t) ? me : node_data(a, cur);
+        const float4 l = a.cot[cur];
+        if (c.z > 0.0f) {
+            const float ax = l.x / c.z, ay = l.y / c.z;
+            pax += ax;
+            pay += ay;
+            pb += l.z - (c.x * ax + c.y * ay);
+        }
+        if (cur == 0u) break;
+        const uint32_t par = a.parent[cur] & kParentMask;
+        if (par >= n - 1u) break;
+        cur = par;
+    }
+    const uint32_t bi = a.idx[t];
+    float2 lx = a.lx[bi];
+    lx.x += me.z * pax;
+    lx.y += me.z * pay;
+    a.lx[bi] = lx;
+    a.lm[bi] += me.x * pax + me.y * pay + pb;
+}
+
+}  // namespace
+
+void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, float dt,
+                        hipStream_t s) {
+    if (n == 0) return;
+    adjoint_pre_kernel<<<(n + 255) / 256, 256, 0, s>>>(lx, lv, la, n, dt);
+}
+
+void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, hipStream_t s) {
+    if (a.n == 0) return;
+    const unsigned block = 256;
+    const unsigned grid = (a.n + block - 1) / block;
+    const size_t lds = (size_t)(block / kWave) * (size_t)stack *
+                       (sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t));
+    if (exact)
+        adjoint_walk_kernel<true><<<grid, block, lds, s>>>(a, stack);
+    else
+        adjoint_walk_kernel<false><<<grid, block, lds, s>>>(a, stack);
+}
+
+void launch_adjoint_fold(const AdjointArgs &a, hipStream_t s) {
+    if (a.n == 0) return;
+    adjoint_fold_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+}
+
+}  // namespace nbs

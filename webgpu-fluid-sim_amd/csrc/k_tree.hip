@@ -10,19 +10,19 @@
 //   relaxed atomic with workgroup-scope barriers, so the second arriver may
 //   read stale sibling data (README.md:18).
 //
-//   Here the climb is split so that every arrival counter is ordered at
-//   workgroup scope, which on a multi-XCD MI355X needs no L2 write-back or
-//   invalidate (agent-scope acquire/release would flush the XCD's L2 on
-//   every level of every thread's climb):
+//   Agent-scope acquire/release removes the race but, on a multi-XCD MI355X,
+//   writes back and invalidates the XCD's L2 at every counter update; done by
+//   every thread at every level it costs milliseconds.  The climb is split
+//   so that almost all of it is ordered at workgroup scope instead:
 //     phase 1 (fill_local_kernel): workgroup b owns sorted leaves
 //       [b*1024, b*1024+1023] and completes every internal node whose leaf
 //       range lies inside that block, with the arrival counters in LDS.  A
 //       thread that completes a node whose parent spans blocks appends the
 //       node to a frontier list.
-//     phase 2 (fill_top_kernel): one workgroup climbs from the frontier
-//       nodes through the block-spanning nodes (a few per block) with
-//       workgroup-scope counters in global memory; the kernel boundary makes
-//       phase 1's nodes visible to it.
+//     phase 2 (fill_top_kernel): a few workgroups climb from the frontier
+//       nodes through the block-spanning nodes (a few per block boundary)
+//       with agent-scope counters; the kernel boundary makes phase 1's nodes
+//       visible to them.
 //   The merge formula does not depend on which child arrives last, so the
 //   result is deterministic and equals a sequential post-order fill.  The
 //   reference's random leaf rotation (`offset`, lbvh.wgsl:171) is therefore
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void build_u32_kernel(const uint32_t *__restri
     tb.child[i] = make_uint2(sp.lc, sp.rc);
     tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
     tb.parent[sp.lc] = (uint32_t)i;  // lbvh.wgsl:155-161
-    tb.parent[sp.rc] = (uint32_t)i;
+    tb.parent[sp.rc] = (uint32_t)i | kRightChild;
     if (i == 0) tb.parent[0] = kNoNode;
     tb.status[i] = 0u;  // lbvh.wgsl:164
 }
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restri
     tb.child[i] = make_uint2(sp.lc, sp.rc);
     tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
     tb.parent[sp.lc] = (uint32_t)i;
-    tb.parent[sp.rc] = (uint32_t)i;
+    tb.parent[sp.rc] = (uint32_t)i | kRightChild;
     tb.status[i] = 0u;
     // A node spanning two sims sits above the sim level; a child whose
     // range lies inside one sim is that sim's root.
@@ -140,17 +140,16 @@ __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restri
 
 constexpr int kFillBlock = 1024;
 
-// lbvh.wgsl:205-228: node p from its two complete children.
-__device__ __forceinline__ void merge_children(uint32_t p, const uint2 ch, float4 *cm,
-                                               float4 *aabb) {
-    const float4 L = cm[ch.x], R = cm[ch.y];
-    const float4 LB = aabb[ch.x], RB = aabb[ch.y];
-    float4 B;
+// lbvh.wgsl:205-228: node p from its two complete children, read from
+// p's child-pair slots.
+__device__ __forceinline__ void merge_children(const TreeBufs &tb, uint32_t p, float4 &C,
+                                               float4 &B) {
+    const float4 L = tb.pcm[2 * p], R = tb.pcm[2 * p + 1];
+    const float4 LB = tb.paabb[2 * p], RB = tb.paabb[2 * p + 1];
     B.x = fminf(LB.x, RB.x);
     B.y = fminf(LB.y, RB.y);
     B.z = fmaxf(LB.z, RB.z);
     B.w = fmaxf(LB.w, RB.w);
-    float4 C;
     C.z = L.z + R.z;
     if (C.z > 0.0f) {
         C.x = (L.x * L.z + R.x * R.z) / C.z;
@@ -160,8 +159,20 @@ __device__ __forceinline__ void merge_children(uint32_t p, const uint2 ch, float
         C.y = 0.0f;
     }
     C.w = fmaxf(B.z - B.x, B.w - B.y);
-    aabb[p] = B;
-    cm[p] = C;
+}
+
+// Store a completed node's data where its readers look for it: the child
+// slot of its parent (parent word `pw`), or the root record.
+__device__ __forceinline__ void publish(const TreeBufs &tb, uint32_t node, uint32_t pw,
+                                        const float4 &C, const float4 &B) {
+    if (node == 0u) {
+        tb.root[0] = C;
+        tb.root[1] = B;
+        return;
+    }
+    const uint32_t slot = 2u * (pw & kParentMask) + (pw >> 31);
+    tb.pcm[slot] = C;
+    tb.paabb[slot] = B;
 }
 
 __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *__restrict__ idx,
@@ -176,19 +187,26 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
                                                                              : n - 1);
     const uint32_t t = base + threadIdx.x;
     if (t >= (uint64_t)n) return;
-    uint32_t cur = (uint32_t)(n - 1) + t;
+    const uint32_t nint = (uint32_t)(n - 1);  // internal nodes
+    uint32_t cur = nint + t;
     const uint32_t b = idx[t];
     const float2 p = pos[b];
-    const float m = mass[b];
     // lbvh.wgsl:179-187
-    tb.cm[cur] = make_float4(p.x, p.y, m, 0.0f);
-    tb.aabb[cur] = make_float4(p.x, p.y, p.x, p.y);
+    float4 C = make_float4(p.x, p.y, mass[b], 0.0f);
+    float4 B = make_float4(p.x, p.y, p.x, p.y);
+    tb.leaf[t] = C;
+    if (n == 1) {
+        publish(tb, 0u, 0u, C, B);
+        return;
+    }
     // lbvh.wgsl:191-232, restricted to nodes inside this block's leaf range.
     // A valid tree reaches the root in <= 96 levels; the level bound and the
     // index check only guard against a corrupt parent array.
-    for (int level = 0; cur != 0u && level < 128; ++level) {
-        const uint32_t par = tb.parent[cur];
-        if (par >= (uint32_t)(n - 1)) break;
+    for (int level = 0; level < 128; ++level) {
+        const uint32_t pw = tb.parent[cur];
+        const uint32_t par = pw & kParentMask;
+        if (par >= nint) break;
+        publish(tb, cur, pw, C, B);
         const uint2 r = tb.range[par];
         if (r.x < base || r.y > last) {
             const uint32_t slot = atomicAdd(tb.frontier_count, 1u);
@@ -198,29 +216,47 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
         const uint32_t prev = __hip_atomic_fetch_add(&s_arrive[par - base], 1u, __ATOMIC_ACQ_REL,
                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
         if (prev == 0u) break;
-        merge_children(par, tb.child[par], tb.cm, tb.aabb);
+        merge_children(tb, par, C, B);
         cur = par;
+        if (cur == 0u) {
+            publish(tb, 0u, 0u, C, B);
+            break;
+        }
     }
 }
 
-// One workgroup: climb from the frontier through the block-spanning nodes.
-__global__ __launch_bounds__(kFillBlock) void fill_top_kernel(int64_t n, TreeBufs tb) {
+// Phase 2: climb from the frontier through the block-spanning nodes (a few
+// per block boundary).  Threads of several workgroups (on any XCD) meet at
+// these nodes, so their counters are agent-scope acquire-release; with only
+// the frontier's few thousand threads doing so, the L2 write-back and
+// invalidate they imply stay cheap.  Each frontier node's data is already
+// in its parent's child slot.
+constexpr int kTopBlocks = 32;
+constexpr int kTopThreads = 256;
+
+__global__ __launch_bounds__(kTopThreads) void fill_top_kernel(int64_t n, TreeBufs tb) {
     const uint32_t count = *tb.frontier_count;
-    for (uint32_t k = threadIdx.x; k < count; k += kFillBlock) {
+    const uint32_t nint = (uint32_t)(n - 1);
+    for (uint32_t k = blockIdx.x * kTopThreads + threadIdx.x; k < count;
+         k += kTopBlocks * kTopThreads) {
         uint32_t cur = tb.frontier[k];
-        for (int level = 0; cur != 0u && level < 128; ++level) {
-            const uint32_t par = tb.parent[cur];
-            if (par >= (uint32_t)(n - 1)) break;
+        for (int level = 0; level < 128; ++level) {
+            const uint32_t par = tb.parent[cur] & kParentMask;
+            if (par >= nint) break;
             const uint32_t prev = __hip_atomic_fetch_add(&tb.status[par], 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                                                         __HIP_MEMORY_SCOPE_AGENT);
             if (prev == 0u) break;
-            merge_children(par, tb.child[par], tb.cm, tb.aabb);
+            float4 C, B;
+            merge_children(tb, par, C, B);
             cur = par;
+            publish(tb, cur, cur ? tb.parent[cur] : 0u, C, B);
+            if (cur == 0u) break;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) *tb.frontier_count = 0u;  // ready for the next step
 }
+
+// Reset the frontier counter for the next step (after phase 2 read it).
+__global__ void fill_reset_kernel(uint32_t *count) { *count = 0u; }
 
 }  // namespace
 
@@ -241,7 +277,10 @@ void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int6
     if (n < 1) return;
     fill_local_kernel<<<(unsigned)((n + kFillBlock - 1) / kFillBlock), kFillBlock, 0, s>>>(
         idx, mass, pos, n, t);
-    if (n > kFillBlock) fill_top_kernel<<<1, kFillBlock, 0, s>>>(n, t);
+    if (n > kFillBlock) {
+        fill_top_kernel<<<kTopBlocks, kTopThreads, 0, s>>>(n, t);
+        fill_reset_kernel<<<1, 1, 0, s>>>(t.frontier_count);
+    }
 }
 
 }  // namespace nbs

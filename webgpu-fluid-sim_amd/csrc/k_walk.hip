@@ -85,26 +85,45 @@ __device__ __forceinline__ bool interact(const float4 c, bool leaf, float px, fl
     }
 }
 
+// Record of node `node` for the walk start: the root record, or the child
+// slot in its parent's pair (ensemble sub-tree roots).
+__device__ __forceinline__ float4 node_record(const WalkArgs &a, uint32_t node) {
+    if (node == 0u) return a.root[0];
+    const uint32_t pw = a.parent[node];
+    return a.pcm[2u * (pw & kParentMask) + (pw >> 31)];
+}
+
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
-__global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_cap) {
-    extern __shared__ uint64_t s_dyn[];
+__global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_cap,
+                                                    uint32_t xcd_per) {
+    extern __shared__ float4 s_dyn[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint64_t *s_mask = s_dyn + (size_t)wid * stack_cap;
-    uint32_t *s_node = reinterpret_cast<uint32_t *>(s_dyn + (size_t)(blockDim.x >> 6) * stack_cap) +
+    const int waves = blockDim.x >> 6;
+    // LDS stack of one wavefront: (node record, lane mask, node id) entries.
+    float4 *s_data = s_dyn + (size_t)wid * stack_cap;
+    uint64_t *s_mask = reinterpret_cast<uint64_t *>(s_dyn + (size_t)waves * stack_cap) +
+                       (size_t)wid * stack_cap;
+    uint32_t *s_node = reinterpret_cast<uint32_t *>(reinterpret_cast<uint64_t *>(
+                           s_dyn + (size_t)waves * stack_cap) + (size_t)waves * stack_cap) +
                        (size_t)wid * stack_cap;
 
+    // XCD-aware order: hardware deals workgroups round-robin over the 8 XCDs;
+    // remap so each XCD walks one contiguous run of Morton-ordered bodies and
+    // its L2 holds that region's sub-tree.
+    uint32_t block = blockIdx.x;
+    if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
+
     const uint32_t n = a.n;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;  // sorted position
+    const uint32_t t = block * blockDim.x + threadIdx.x;  // sorted position
     const bool valid = t < n;
     const uint32_t leaf0 = n - 1u;
     const uint32_t my_leaf = leaf0 + t;
-    const float4 *__restrict__ cm = a.cm;
-    const_as<float> *cm_s = (const_as<float> *)a.cm;
     const_as<uint32_t> *child_s = (const_as<uint32_t> *)a.child;
+    const_as<float> *pcmf_s = (const_as<float> *)a.pcm;
 
     float px = 0.0f, py = 0.0f, m1 = 0.0f;
     if (valid) {
-        const float4 me = cm[my_leaf];  // written by the fill: {pos, mass, 0}
+        const float4 me = a.leaf[t];  // written by the fill: {pos, mass, 0}
         px = me.x;
         py = me.y;
         m1 = me.z;
@@ -119,6 +138,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
     int sp = 0;
     uint32_t node = 0u;
     uint64_t mask = live;
+    float4 c;
     if constexpr (kEnsemble) {
         // Each lane starts at its sim's sub-tree root; group lanes by root.
         uint32_t my_root = 0u;
@@ -130,20 +150,25 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
             const uint32_t r = uniform_u32((uint32_t)__builtin_amdgcn_readlane((int)my_root, l0));
             const uint64_t m = __ballot(valid && my_root == r) & rem;
             rem &= ~m;
+            const float4 rc = node_record(a, r);
             if (first) {
                 node = r;
                 mask = m;
+                c = rc;
                 first = false;
             } else if (sp < stack_cap) {
                 if (lane == 0) {
                     s_node[sp] = r;
                     s_mask[sp] = m;
+                    s_data[sp] = rc;
                 }
                 ++sp;
             } else if (lane == 0) {
                 atomicOr(a.overflow, 1u);
             }
         }
+    } else {
+        c = a.root[0];
     }
 
     // A wavefront visits each node at most once per lane group, so a walk of a
@@ -155,9 +180,15 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
             if (lane == 0) atomicOr(a.overflow, 1u);
             break;
         }
-        const float4 c = make_float4(cm_s[4 * node], cm_s[4 * node + 1], cm_s[4 * node + 2],
-                                     cm_s[4 * node + 3]);
         const bool leaf = node >= leaf0;
+        // Speculative scalar loads of the children (used if the node opens);
+        // a leaf reads pair 0, which always exists when n >= 2.
+        const uint32_t pn = leaf ? 0u : node;
+        const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
+                                      pcmf_s[8 * pn + 3]);
+        const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5], pcmf_s[8 * pn + 6],
+                                      pcmf_s[8 * pn + 7]);
+        const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
         bool open = false;
         if (((mask >> lane) & 1u) && !(leaf && node == my_leaf)) {
             open = interact<kExact, kRefSoft>(c, leaf, px, py, m1, p, eps2c, ax, ay);
@@ -170,16 +201,17 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
         }
         const uint64_t om = __ballot(open);
         if (om) {
-            const uint2 ch = make_uint2(child_s[2 * node], child_s[2 * node + 1]);
             if (sp < stack_cap) {
                 // Push left, continue with right: the reference pops right first.
                 if (lane == 0) {
-                    s_node[sp] = ch.x;
+                    s_node[sp] = chl;
                     s_mask[sp] = om;
+                    s_data[sp] = cl;
                 }
                 ++sp;
-                node = ch.y;
+                node = chr;
                 mask = om;
+                c = cr;
                 continue;
             }
             if (lane == 0) atomicOr(a.overflow, 1u);
@@ -194,6 +226,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
         --sp;
         node = uniform_u32(s_node[sp]);
         mask = uniform_u64(s_mask[sp]);
+        c = s_data[sp];
     }
 
     if (valid) {
@@ -233,44 +266,51 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
 }
 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
-void launch_t(const WalkArgs &a, int block, int stack, hipStream_t s) {
-    const unsigned grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
-    const size_t lds = (size_t)(block / kWave) * (size_t)stack * (sizeof(uint64_t) + sizeof(uint32_t));
-    walk_kernel<kExact, kRefSoft, kEnsemble, kStats><<<grid, block, lds, s>>>(a, stack);
+void launch_t(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s) {
+    uint32_t grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
+    uint32_t per = 0;
+    if (xcd && grid >= 16) {
+        per = (grid + 7u) / 8u;
+        grid = per * 8u;  // blocks past the bodies exit at once
+    }
+    const size_t lds = (size_t)(block / kWave) * (size_t)stack *
+                       (sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t));
+    walk_kernel<kExact, kRefSoft, kEnsemble, kStats><<<grid, block, lds, s>>>(a, stack, per);
 }
 
 template <bool kExact, bool kRefSoft>
-void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, hipStream_t s) {
+void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, bool xcd,
+              hipStream_t s) {
     if (ens) {
         if (stats)
-            launch_t<kExact, kRefSoft, true, true>(a, block, stack, s);
+            launch_t<kExact, kRefSoft, true, true>(a, block, stack, xcd, s);
         else
-            launch_t<kExact, kRefSoft, true, false>(a, block, stack, s);
+            launch_t<kExact, kRefSoft, true, false>(a, block, stack, xcd, s);
     } else {
         if (stats)
-            launch_t<kExact, kRefSoft, false, true>(a, block, stack, s);
+            launch_t<kExact, kRefSoft, false, true>(a, block, stack, xcd, s);
         else
-            launch_t<kExact, kRefSoft, false, false>(a, block, stack, s);
+            launch_t<kExact, kRefSoft, false, false>(a, block, stack, xcd, s);
     }
 }
 
 }  // namespace
 
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 hipStream_t s) {
+                 bool xcd_remap, hipStream_t s) {
     if (a.n == 0) return;
     const bool exact = a.p.math_mode != 0;
     const bool refsoft = a.p.soft_mode == 0;
     if (exact) {
         if (refsoft)
-            launch_e<true, true>(a, ensemble, stats, block, stack, s);
+            launch_e<true, true>(a, ensemble, stats, block, stack, xcd_remap, s);
         else
-            launch_e<true, false>(a, ensemble, stats, block, stack, s);
+            launch_e<true, false>(a, ensemble, stats, block, stack, xcd_remap, s);
     } else {
         if (refsoft)
-            launch_e<false, true>(a, ensemble, stats, block, stack, s);
+            launch_e<false, true>(a, ensemble, stats, block, stack, xcd_remap, s);
         else
-            launch_e<false, false>(a, ensemble, stats, block, stack, s);
+            launch_e<false, false>(a, ensemble, stats, block, stack, xcd_remap, s);
     }
 }
 

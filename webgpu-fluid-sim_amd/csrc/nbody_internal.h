@@ -52,13 +52,22 @@ int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32
                     int64_t n, int bits, const SortScratch &sc, hipStream_t s);
 
 // ---------------------------------------------------------------- tree
+// Node data is stored in child pairs: each internal node p holds the
+// {com.x, com.y, mass, length} records of its two children in pcm[2p] (left)
+// and pcm[2p+1] (right), and their AABBs in paabb likewise.  Opening a node
+// therefore fetches both children's data with one 32-byte load.  The root,
+// which is nobody's child, has its own record.
+constexpr uint32_t kRightChild = 0x80000000u;  // parent word: index | side << 31
+constexpr uint32_t kParentMask = 0x7FFFFFFFu;
 struct TreeBufs {
     uint2 *child;        // [n-1] {left, right}
     uint2 *range;        // [n-1] {first, last} sorted leaf of each internal node
-    uint32_t *parent;    // [2n-1]
+    uint32_t *parent;    // [2n-1] parent index | kRightChild for right children
     uint32_t *status;    // [n-1] arrival counters of the fill
-    float4 *cm;          // [2n-1] {com.x, com.y, mass, length}
-    float4 *aabb;        // [2n-1] {min.x, min.y, max.x, max.y}
+    float4 *leaf;        // [n] sorted bodies {x, y, mass, 0}
+    float4 *pcm;         // [2(n-1)] children's {com.x, com.y, mass, length}
+    float4 *paabb;       // [2(n-1)] children's {min.x, min.y, max.x, max.y}
+    float4 *root;        // [2] root {cm, aabb}
     uint32_t *frontier;  // [n] nodes completed by the first fill phase whose parent spans blocks
     uint32_t *frontier_count;
 };
@@ -70,7 +79,10 @@ void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int6
 
 // ---------------------------------------------------------------- walk
 struct WalkArgs {
-    const float4 *cm;        // [2n-1] {com.x, com.y, mass, length}
+    const float4 *leaf;      // [n] sorted bodies {x, y, mass, 0}
+    const float4 *pcm;       // [2(n-1)] child-pair records
+    const float4 *root;      // root record
+    const uint32_t *parent;  // parent words (ensemble sub-tree roots)
     const uint2 *child;      // [n-1]
     const uint32_t *idx;     // [n] sorted position -> body
     const uint64_t *keys64;  // ensemble: sorted keys (sim id in bits 30..37)
@@ -84,7 +96,28 @@ struct WalkArgs {
     StepParams p;
 };
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 hipStream_t s);
+                 bool xcd_remap, hipStream_t s);
+
+// ---------------------------------------------------------------- adjoint
+struct AdjointArgs {
+    const float4 *leaf;
+    const float4 *pcm;
+    const float4 *root;
+    const uint32_t *parent;
+    const uint2 *child;
+    const uint32_t *idx;
+    const float2 *la;   // [n] dL/da by body
+    float2 *lx;         // [n] dL/dx by body (accumulated)
+    float *lm;          // [n] dL/dm by body (accumulated)
+    float4 *cot;        // [2n-1] node cotangents {lc.x, lc.y, lM, -}
+    uint32_t *overflow;
+    uint32_t n;
+    float grav, theta, eps;
+};
+void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, float dt,
+                        hipStream_t s);
+void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, hipStream_t s);
+void launch_adjoint_fold(const AdjointArgs &a, hipStream_t s);
 
 // ---------------------------------------------------------------- misc
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s);

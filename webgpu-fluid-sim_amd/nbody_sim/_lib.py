@@ -57,7 +57,8 @@ EXPORTS = (
     "nbody_get_num_bodies", "nbody_step", "nbody_synchronize", "nbody_set_stream",
     "nbody_use_graph", "nbody_set_debug", "nbody_debug_get", "nbody_walk_stats",
     "nbody_set_profiling", "nbody_phase_times", "nbody_set_tuning", "nbody_last_error",
-    "nbody_generate", "nbody_philox4x32", "nbody_version",
+    "nbody_generate", "nbody_philox4x32", "nbody_version", "nbody_adjoint_forward",
+    "nbody_adjoint_backward",
 )
 
 
@@ -133,6 +134,8 @@ def lib() -> ctypes.CDLL:
         "nbody_philox4x32": (None, [ctypes.POINTER(u32), ctypes.POINTER(u32),
                                     ctypes.POINTER(u32)]),
         "nbody_version": (ctypes.c_char_p, []),
+        "nbody_adjoint_forward": (ctypes.c_int, [vp, i32, i32]),
+        "nbody_adjoint_backward": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -129,6 +129,24 @@ class _Handle:
         self._check(_lib.lib().nbody_walk_stats(self.handle, out, int(reset)))
         return {"accepted": int(out[0]), "opened": int(out[1]), "overflowed": int(out[2])}
 
+    # -- adjoint ------------------------------------------------------------
+    def adjointForward(self, nsteps: int, every: int = 20):
+        """Run nsteps steps keeping a checkpoint every `every` steps."""
+        self._check(_lib.lib().nbody_adjoint_forward(self.handle, int(nsteps), int(every)))
+
+    def adjointBackward(self, gpos_T=None, gvel_T=None):
+        """dL/d(pos0, vel0, mass) from dL/d(pos_T, vel_T), decisions frozen."""
+        n = self.getNumBodies()
+        gp = np.zeros((n, 2), np.float32)
+        gv = np.zeros((n, 2), np.float32)
+        gm = np.zeros(n, np.float32)
+        a = None if gpos_T is None else np.ascontiguousarray(gpos_T, np.float32).reshape(n, 2)
+        b = None if gvel_T is None else np.ascontiguousarray(gvel_T, np.float32).reshape(n, 2)
+        self._check(_lib.lib().nbody_adjoint_backward(
+            self.handle, None if a is None else _ptr(a), None if b is None else _ptr(b),
+            _ptr(gp), _ptr(gv), _ptr(gm)))
+        return gp, gv, gm
+
     def debugBuffer(self, which: str) -> np.ndarray:
         """Intermediate buffers of the last step, as numpy arrays."""
         n = self.getNumBodies()
