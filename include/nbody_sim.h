@@ -144,9 +144,11 @@ int nbody_walk_stats(nbody_sim *sim, uint64_t *out3, int32_t reset);
 int nbody_set_profiling(nbody_sim *sim, int32_t enable);
 int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t reset);
 
-/* Tunables of the launch (0 keeps the current value): walk workgroup size
- * (64..1024, multiple of 64) and walk stack entries per wave (8..256). */
-int nbody_set_tuning(nbody_sim *sim, int32_t walk_block, int32_t walk_stack);
+/* Tunables of the launch: walk workgroup size (64..1024, multiple of 64;
+ * 0 keeps), walk stack entries per wavefront (8..256; 0 keeps) and walk
+ * variant (0 LDS-stack walk, 1 register-stack walk; -1 keeps).  Every
+ * setting gives the same results. */
+int nbody_set_tuning(nbody_sim *sim, int32_t walk_block, int32_t walk_stack, int32_t walk_variant);
 
 /* Adjoint of the step (no reference counterpart: the reference only steps
  * forward).  nbody_adjoint_forward runs `nsteps` steps from the current state

@@ -149,3 +149,24 @@ def test_stats_match_oracle():
     assert g["stats"]["accepted"] == int(r["stats"][:, 0].sum())
     assert g["stats"]["opened"] == int(r["stats"][:, 1].sum())
     assert g["stats"]["overflowed"] == 0
+
+
+GOLDEN = sorted(__import__("glob").glob(__import__("os").path.join(
+    __import__("os").path.dirname(__file__), "golden", "step_*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[p.rsplit("/", 1)[-1] for p in GOLDEN])
+def test_gpu_matches_golden_fixture(path):
+    f = np.load(path)
+    g, dt, em, th, eps, sm, mm = f["params"]
+    cfg = nbody_sim.Config(gravConstant=float(g), deltaTime=float(dt), epsilonMultiplier=float(em),
+                           bhTheta=float(th), epsilon=float(eps),
+                           softeningMode="constant" if int(sm) == 1 else "reference",
+                           mortonMode="bbox30" if int(mm) == 1 else "reference")
+    out = gpu_step(cfg, f["mass"], f["pos_in"], f["vel_in"])
+    ref = {k: f[k] for k in f.files}
+    _compare_tree(out, {"keys": ref["keys"], "idx": ref["idx"], "left": ref["left"],
+                        "right": ref["right"], "parent": ref["parent"],
+                        "node_cm": ref["node_cm"], "node_aabb": ref["node_aabb"]})
+    assert np.array_equal(out["accel"], ref["accel"])
+    assert np.array_equal(out["pos"], ref["pos"]) and np.array_equal(out["vel"], ref["vel"])

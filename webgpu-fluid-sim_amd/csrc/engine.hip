@@ -80,6 +80,7 @@ struct nbody_sim {
 
     int32_t debug = 0;
     int walk_block = 256, walk_stack = 128;
+    int walk_variant = 0;  // 0: LDS-stack walk, 1: register-stack walk
     bool xcd_remap = true;
 
     bool use_graph = false;
@@ -279,7 +280,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     w.n = (uint32_t)n;
     w.p = p;
     launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
-                s->xcd_remap, st);
+                s->xcd_remap, s->walk_variant, st);
     mark(6);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(s, e, "kernel launch");
@@ -687,8 +688,10 @@ int nbody_phase_times(nbody_sim *s, float *ms_out, int64_t *steps, int32_t reset
     return NBODY_OK;
 }
 
-int nbody_set_tuning(nbody_sim *s, int32_t walk_block, int32_t walk_stack) {
+int nbody_set_tuning(nbody_sim *s, int32_t walk_block, int32_t walk_stack, int32_t walk_variant) {
     if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (walk_variant > 1) return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0 or 1");
+    if (walk_variant >= 0) s->walk_variant = walk_variant;
     if (walk_block) {
         if (walk_block < 64 || walk_block > 1024 || walk_block % 64)
             return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");

@@ -265,13 +265,241 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
     }
 }
 
+// Test of one lane against node record c, computed for every lane (the
+// caller selects with `act`), so the wavefront runs it without branches.
+template <bool kExact, bool kRefSoft>
+__device__ __forceinline__ void lane_test(const float4 c, bool leaf, bool act, float px, float py,
+                                          float m1, const StepParams &p, float eps2c, float &ax,
+                                          float &ay, bool &acc, bool &open) {
+    const float rx = c.x - px;
+    const float ry = c.y - py;
+    float eps2;
+    if constexpr (kRefSoft) {
+        const float m_eff = fmaxf(m1, c.z);
+        const float eps = p.eps_mult * sqrtf(p.grav * m_eff * p.dt);
+        eps2 = eps * eps;
+    } else {
+        eps2 = eps2c;
+    }
+    float inv, nx, ny;
+    if constexpr (kExact) {
+        const float d2 = rx * rx + ry * ry;
+        inv = 1.0f / sqrtf(d2 + eps2);
+        const float inv3 = inv * inv * inv;
+        const float gm = p.grav * c.z;
+        nx = ax + gm * rx * inv3;
+        ny = ay + gm * ry * inv3;
+    } else {
+        inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2)));
+        const float f = p.grav * c.z * (inv * inv * inv);
+        nx = __builtin_fmaf(f, rx, ax);
+        ny = __builtin_fmaf(f, ry, ay);
+    }
+    const bool near = !(c.w * inv < p.theta);
+    acc = act && (leaf || !near);
+    open = act && !leaf && near;
+    ax = acc ? nx : ax;
+    ay = acc ? ny : ay;
+}
+
+// Register-resident stack of one wavefront: entry k lives in lane k & 63 of
+// row k >> 6, one VGPR per field, so a push or pop needs no LDS access, no
+// exec-mask change and no readfirstlane.  A push is seven v_writelane_b32
+// (lane k takes the wave-uniform SGPR value; the other lanes keep theirs)
+// with the lane select in M0 -- GFX9 VALU instructions read at most one SGPR
+// through the constant bus besides M0.  No memory is accessed.
+#define NBS_WRITELANES(a0, a1, a2, a3, a4, a5, a6)                                          \
+    asm("s_mov_b32 m0, %14\n\t"                                                              \
+        "v_writelane_b32 %0, %7, m0\n\t"                                                     \
+        "v_writelane_b32 %1, %8, m0\n\t"                                                     \
+        "v_writelane_b32 %2, %9, m0\n\t"                                                     \
+        "v_writelane_b32 %3, %10, m0\n\t"                                                    \
+        "v_writelane_b32 %4, %11, m0\n\t"                                                    \
+        "v_writelane_b32 %5, %12, m0\n\t"                                                    \
+        "v_writelane_b32 %6, %13, m0"                                                         \
+        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6)                \
+        : "s"(v0), "s"(v1), "s"(v2), "s"(v3), "s"(v4), "s"(v5), "s"(v6), "s"(l)               \
+        : "m0")
+
+struct RegStack {
+    int node[2], mlo[2], mhi[2], cx[2], cy[2], cm[2], cw[2];
+
+    __device__ __forceinline__ void put(int row, int lane_sel, uint32_t nd, uint64_t m,
+                                        const float4 &c) {
+        const int v0 = __builtin_amdgcn_readfirstlane((int)nd);
+        const int v1 = __builtin_amdgcn_readfirstlane((int)(uint32_t)m);
+        const int v2 = __builtin_amdgcn_readfirstlane((int)(uint32_t)(m >> 32));
+        const int v3 = __builtin_amdgcn_readfirstlane(__float_as_int(c.x));
+        const int v4 = __builtin_amdgcn_readfirstlane(__float_as_int(c.y));
+        const int v5 = __builtin_amdgcn_readfirstlane(__float_as_int(c.z));
+        const int v6 = __builtin_amdgcn_readfirstlane(__float_as_int(c.w));
+        const int l = __builtin_amdgcn_readfirstlane(lane_sel);
+        if (row == 0)
+            NBS_WRITELANES(node[0], mlo[0], mhi[0], cx[0], cy[0], cm[0], cw[0]);
+        else
+            NBS_WRITELANES(node[1], mlo[1], mhi[1], cx[1], cy[1], cm[1], cw[1]);
+    }
+    __device__ __forceinline__ void get(int row, int lane_sel, uint32_t &nd, uint64_t &m,
+                                        float4 &c) const {
+        const int r = row;
+        nd = (uint32_t)__builtin_amdgcn_readlane(r ? node[1] : node[0], lane_sel);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(r ? mlo[1] : mlo[0], lane_sel);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(r ? mhi[1] : mhi[0], lane_sel);
+        m = ((uint64_t)hi << 32) | lo;
+        c.x = __int_as_float(__builtin_amdgcn_readlane(r ? cx[1] : cx[0], lane_sel));
+        c.y = __int_as_float(__builtin_amdgcn_readlane(r ? cy[1] : cy[0], lane_sel));
+        c.z = __int_as_float(__builtin_amdgcn_readlane(r ? cm[1] : cm[0], lane_sel));
+        c.w = __int_as_float(__builtin_amdgcn_readlane(r ? cw[1] : cw[0], lane_sel));
+    }
+};
+
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
-void launch_t(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s) {
+__global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int stack_cap,
+                                                       uint32_t xcd_per) {
+    const int lane = threadIdx.x & 63;
+    uint32_t block = blockIdx.x;
+    if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
+    const uint32_t n = a.n;
+    const uint32_t t = block * blockDim.x + threadIdx.x;  // sorted position
+    const bool valid = t < n;
+    const uint32_t leaf0 = n - 1u;
+    const uint32_t my_leaf = leaf0 + t;
+    const_as<uint32_t> *child_s = (const_as<uint32_t> *)a.child;
+    const_as<float> *pcmf_s = (const_as<float> *)a.pcm;
+
+    float px = 0.0f, py = 0.0f, m1 = 0.0f;
+    if (valid) {
+        const float4 me = a.leaf[t];  // written by the fill: {pos, mass, 0}
+        px = me.x;
+        py = me.y;
+        m1 = me.z;
+    }
+    const uint64_t live = __ballot(valid);
+    if (live == 0) return;
+
+    const StepParams p = a.p;
+    const float eps2c = p.eps * p.eps;
+    float ax = 0.0f, ay = 0.0f;
+    uint32_t n_acc = 0, n_open = 0, n_drop = 0;
+    RegStack st;
+    int sp = 0;
+    uint32_t node = 0u;
+    uint64_t mask = live;
+    float4 c;
+    if constexpr (kEnsemble) {
+        // Each lane starts at its sim's sub-tree root; group lanes by root.
+        uint32_t my_root = 0u;
+        if (valid) my_root = a.sim_root[(uint32_t)(a.keys64[t] >> kSimShift)];
+        uint64_t rem = live;
+        bool first = true;
+        while (rem) {
+            const int l0 = __builtin_ctzll(rem);
+            const uint32_t r = uniform_u32((uint32_t)__builtin_amdgcn_readlane((int)my_root, l0));
+            const uint64_t m = __ballot(valid && my_root == r) & rem;
+            rem &= ~m;
+            const float4 rc = node_record(a, r);
+            if (first) {
+                node = r;
+                mask = m;
+                c = rc;
+                first = false;
+            } else if (sp < stack_cap) {
+                st.put(sp >> 6, sp & 63, r, m, rc);
+                ++sp;
+            } else if (lane == 0) {
+                atomicOr(a.overflow, 1u);
+            }
+        }
+    } else {
+        c = a.root[0];
+    }
+
+    for (;;) {
+        const bool leaf = node >= leaf0;
+        const bool act = ((mask >> lane) & 1u) && !(leaf && node == my_leaf);
+        bool acc, open;
+        lane_test<kExact, kRefSoft>(c, leaf, act, px, py, m1, p, eps2c, ax, ay, acc, open);
+        if constexpr (kStats) {
+            n_acc += acc ? 1u : 0u;
+            n_open += open ? 1u : 0u;
+        }
+        const uint64_t om = __ballot(open);
+        if (om) {
+            // Both children's records in one scalar load; push left, go right
+            // (the reference pops the right child first).
+            const uint32_t pn = node;
+            const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
+                                          pcmf_s[8 * pn + 3]);
+            const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5],
+                                          pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
+            const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
+            if (sp < stack_cap) {
+                st.put(sp >> 6, sp & 63, chl, om, cl);
+                ++sp;
+                node = chr;
+                mask = om;
+                c = cr;
+                continue;
+            }
+            // full stack: drop both children like barnes_hut.wgsl:100-105
+            if (lane == 0) atomicOr(a.overflow, 1u);
+            if constexpr (kStats) {
+                n_open -= open ? 1u : 0u;
+                n_drop += open ? 1u : 0u;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        st.get(sp >> 6, sp & 63, node, mask, c);
+    }
+
+    if (valid) {
+        const uint32_t bi = a.idx[t];
+        const float dt = p.dt;
+        float2 v = a.vel[bi];
+        float2 x;
+        if constexpr (kExact) {
+            v.x = v.x + ax * dt;  // barnes_hut.wgsl:110
+            v.y = v.y + ay * dt;
+            x.x = px + v.x * dt;  // barnes_hut.wgsl:120
+            x.y = py + v.y * dt;
+        } else {
+            v.x = __builtin_fmaf(ax, dt, v.x);
+            v.y = __builtin_fmaf(ay, dt, v.y);
+            x.x = __builtin_fmaf(v.x, dt, px);
+            x.y = __builtin_fmaf(v.y, dt, py);
+        }
+        a.vel[bi] = v;
+        a.pos[bi] = x;
+        if (a.accel) a.accel[bi] = make_float2(ax, ay);
+    }
+    if constexpr (kStats) {
+        uint32_t s0 = n_acc, s1 = n_open, s2 = n_drop;
+        for (int o = 32; o > 0; o >>= 1) {
+            s0 += __shfl_xor(s0, o);
+            s1 += __shfl_xor(s1, o);
+            s2 += __shfl_xor(s2, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], (unsigned long long)s0);
+            atomicAdd(&a.stats[1], (unsigned long long)s1);
+            if (s2) atomicAdd(&a.stats[2], (unsigned long long)s2);
+        }
+    }
+}
+
+template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
+void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, hipStream_t s) {
     uint32_t grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
     uint32_t per = 0;
     if (xcd && grid >= 16) {
         per = (grid + 7u) / 8u;
         grid = per * 8u;  // blocks past the bodies exit at once
+    }
+    if (variant == 1) {
+        walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats><<<grid, block, 0, s>>>(
+            a, stack < 128 ? stack : 128, per);
+        return;
     }
     const size_t lds = (size_t)(block / kWave) * (size_t)stack *
                        (sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t));
@@ -280,37 +508,37 @@ void launch_t(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s) 
 
 template <bool kExact, bool kRefSoft>
 void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, bool xcd,
-              hipStream_t s) {
+              int variant, hipStream_t s) {
     if (ens) {
         if (stats)
-            launch_t<kExact, kRefSoft, true, true>(a, block, stack, xcd, s);
+            launch_t<kExact, kRefSoft, true, true>(a, block, stack, xcd, variant, s);
         else
-            launch_t<kExact, kRefSoft, true, false>(a, block, stack, xcd, s);
+            launch_t<kExact, kRefSoft, true, false>(a, block, stack, xcd, variant, s);
     } else {
         if (stats)
-            launch_t<kExact, kRefSoft, false, true>(a, block, stack, xcd, s);
+            launch_t<kExact, kRefSoft, false, true>(a, block, stack, xcd, variant, s);
         else
-            launch_t<kExact, kRefSoft, false, false>(a, block, stack, xcd, s);
+            launch_t<kExact, kRefSoft, false, false>(a, block, stack, xcd, variant, s);
     }
 }
 
 }  // namespace
 
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 bool xcd_remap, hipStream_t s) {
+                 bool xcd_remap, int variant, hipStream_t s) {
     if (a.n == 0) return;
     const bool exact = a.p.math_mode != 0;
     const bool refsoft = a.p.soft_mode == 0;
     if (exact) {
         if (refsoft)
-            launch_e<true, true>(a, ensemble, stats, block, stack, xcd_remap, s);
+            launch_e<true, true>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
         else
-            launch_e<true, false>(a, ensemble, stats, block, stack, xcd_remap, s);
+            launch_e<true, false>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
     } else {
         if (refsoft)
-            launch_e<false, true>(a, ensemble, stats, block, stack, xcd_remap, s);
+            launch_e<false, true>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
         else
-            launch_e<false, false>(a, ensemble, stats, block, stack, xcd_remap, s);
+            launch_e<false, false>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
     }
 }
 

@@ -96,7 +96,7 @@ struct WalkArgs {
     StepParams p;
 };
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 bool xcd_remap, hipStream_t s);
+                 bool xcd_remap, int variant, hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint
 struct AdjointArgs {

@@ -127,7 +127,7 @@ def lib() -> ctypes.CDLL:
         "nbody_set_profiling": (ctypes.c_int, [vp, i32]),
         "nbody_phase_times": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float),
                                              ctypes.POINTER(ctypes.c_int64), i32]),
-        "nbody_set_tuning": (ctypes.c_int, [vp, i32, i32]),
+        "nbody_set_tuning": (ctypes.c_int, [vp, i32, i32, i32]),
         "nbody_last_error": (ctypes.c_char_p, [vp]),
         "nbody_generate": (ctypes.c_int, [i32, i64, ctypes.c_uint64,
                                           ctypes.POINTER(NbodyIcParams), vp, vp, vp]),

@@ -112,8 +112,10 @@ class _Handle:
         flags = (_lib.DEBUG_ACCEL if accel else 0) | (_lib.DEBUG_STATS if stats else 0)
         self._check(_lib.lib().nbody_set_debug(self.handle, flags))
 
-    def setTuning(self, walk_block: int = 0, walk_stack: int = 0):
-        self._check(_lib.lib().nbody_set_tuning(self.handle, int(walk_block), int(walk_stack)))
+    def setTuning(self, walk_block: int = 0, walk_stack: int = 0, walk_variant: int = -1):
+        """Launch tunables (same results for every setting); 0 / -1 keep the current value."""
+        self._check(_lib.lib().nbody_set_tuning(self.handle, int(walk_block), int(walk_stack),
+                                                int(walk_variant)))
 
     def setProfiling(self, enable: bool = True):
         self._check(_lib.lib().nbody_set_profiling(self.handle, int(bool(enable))))
