@@ -144,11 +144,21 @@ int nbody_walk_stats(nbody_sim *sim, uint64_t *out3, int32_t reset);
 int nbody_set_profiling(nbody_sim *sim, int32_t enable);
 int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t reset);
 
-/* Tunables of the launch: walk workgroup size (64..1024, multiple of 64;
- * 0 keeps), walk stack entries per wavefront (8..256; 0 keeps) and walk
- * variant (0 LDS-stack walk, 1 register-stack walk; -1 keeps).  Every
- * setting gives the same results. */
-int nbody_set_tuning(nbody_sim *sim, int32_t walk_block, int32_t walk_stack, int32_t walk_variant);
+/* Launch tunables.  Every setting computes the same result bit for bit.
+ *   walk_block   walk workgroup size (64..1024, multiple of 64)
+ *   walk_stack   walk stack entries per wavefront (8..256; the register stack
+ *                holds at most 128)
+ *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default)
+ *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
+ *                of Morton-ordered bodies (default); 0: hardware order */
+typedef struct nbody_tuning {
+    int32_t walk_block;
+    int32_t walk_stack;
+    int32_t walk_variant;
+    int32_t xcd_order;
+} nbody_tuning;
+int nbody_set_tuning(nbody_sim *sim, const nbody_tuning *t);
+int nbody_get_tuning(const nbody_sim *sim, nbody_tuning *out);
 
 /* Adjoint of the step (no reference counterpart: the reference only steps
  * forward).  nbody_adjoint_forward runs `nsteps` steps from the current state

@@ -33,7 +33,8 @@ def test_walk_variants_identical(math):
     cfg, _ = params(theta=0.7, dt=5e-5)
     cfg = cfg.with_(mathMode=math)
     ref = run(cfg, bodies, (256, 128, 0))
-    for tuning in [(64, 64, 0), (1024, 256, 0), (256, 128, 1), (64, 64, 1), (512, 128, 1)]:
+    for tuning in [(64, 64, 0, 1), (1024, 256, 0, 0), (256, 128, 1, 0), (64, 64, 1, 1),
+                   (512, 128, 1, 1)]:
         got = run(cfg, bodies, tuning)
         assert np.array_equal(got[0], ref[0]), tuning
         assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), tuning
@@ -60,7 +61,7 @@ def test_autotune_picks_a_working_configuration():
     cfg, _ = params(theta=0.7, dt=5e-5)
     sim = nbody_sim.Simulation(cfg)
     sim.setBodies(*bodies)
-    best, table = autotune(sim, steps=2, blocks=(128, 256), stacks=(128,))
+    best, table = autotune(sim, steps=2, blocks=(128, 256), stacks=(128,), xcd=(1,))
     assert best in table and all(v > 0 for v in table.values())
     _, p0, _ = sim.getBodies()
     assert np.array_equal(p0, bodies[1])  # state restored

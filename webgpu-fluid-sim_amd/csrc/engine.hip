@@ -80,7 +80,7 @@ struct nbody_sim {
 
     int32_t debug = 0;
     int walk_block = 256, walk_stack = 128;
-    int walk_variant = 0;  // 0: LDS-stack walk, 1: register-stack walk
+    int walk_variant = 1;  // 0: LDS-stack walk, 1: register-stack walk
     bool xcd_remap = true;
 
     bool use_graph = false;
@@ -688,21 +688,28 @@ int nbody_phase_times(nbody_sim *s, float *ms_out, int64_t *steps, int32_t reset
     return NBODY_OK;
 }
 
-int nbody_set_tuning(nbody_sim *s, int32_t walk_block, int32_t walk_stack, int32_t walk_variant) {
-    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
-    if (walk_variant > 1) return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0 or 1");
-    if (walk_variant >= 0) s->walk_variant = walk_variant;
-    if (walk_block) {
-        if (walk_block < 64 || walk_block > 1024 || walk_block % 64)
-            return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");
-        s->walk_block = walk_block;
-    }
-    if (walk_stack) {
-        if (walk_stack < 8 || walk_stack > 256)
-            return fail(s, NBODY_ERR_INVALID_ARG, "walk_stack must be in [8, 256]");
-        s->walk_stack = walk_stack;
-    }
+int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
+    if (!s || !t) return fail(s, NBODY_ERR_INVALID_ARG, "NULL argument");
+    if (t->walk_block < 64 || t->walk_block > 1024 || t->walk_block % 64)
+        return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");
+    if (t->walk_stack < 8 || t->walk_stack > 256)
+        return fail(s, NBODY_ERR_INVALID_ARG, "walk_stack must be in [8, 256]");
+    if (t->walk_variant != 0 && t->walk_variant != 1)
+        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0 or 1");
+    s->walk_block = t->walk_block;
+    s->walk_stack = t->walk_stack;
+    s->walk_variant = t->walk_variant;
+    s->xcd_remap = t->xcd_order != 0;
     s->graph_dirty = true;
+    return NBODY_OK;
+}
+
+int nbody_get_tuning(const nbody_sim *s, nbody_tuning *out) {
+    if (!s || !out) return fail(nullptr, NBODY_ERR_INVALID_ARG, "NULL argument");
+    out->walk_block = s->walk_block;
+    out->walk_stack = s->walk_stack;
+    out->walk_variant = s->walk_variant;
+    out->xcd_order = s->xcd_remap ? 1 : 0;
     return NBODY_OK;
 }
 

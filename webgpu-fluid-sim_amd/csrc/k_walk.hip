@@ -433,19 +433,38 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
             const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5],
                                           pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
             const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
-            if (sp < stack_cap) {
+            if (chr >= leaf0) {
+                // A leaf child has no subtree: its lanes take it in this
+                // iteration, at the same place in the order.
+                const bool inr = ((om >> lane) & 1u) != 0u;
+                bool a2, o2;
+                lane_test<kExact, kRefSoft>(cr, true, inr && chr != my_leaf, px, py, m1, p,
+                                            eps2c, ax, ay, a2, o2);
+                if constexpr (kStats) n_acc += a2 ? 1u : 0u;
+                if (chl < leaf0) {
+                    node = chl;
+                    mask = om;
+                    c = cl;
+                    continue;
+                }
+                lane_test<kExact, kRefSoft>(cl, true, inr && chl != my_leaf, px, py, m1, p,
+                                            eps2c, ax, ay, a2, o2);
+                if constexpr (kStats) n_acc += a2 ? 1u : 0u;
+                // both children done: fall through to the pop
+            } else if (sp < stack_cap) {
                 st.put(sp >> 6, sp & 63, chl, om, cl);
                 ++sp;
                 node = chr;
                 mask = om;
                 c = cr;
                 continue;
-            }
-            // full stack: drop both children like barnes_hut.wgsl:100-105
-            if (lane == 0) atomicOr(a.overflow, 1u);
-            if constexpr (kStats) {
-                n_open -= open ? 1u : 0u;
-                n_drop += open ? 1u : 0u;
+            } else {
+                // full stack: drop both children like barnes_hut.wgsl:100-105
+                if (lane == 0) atomicOr(a.overflow, 1u);
+                if constexpr (kStats) {
+                    n_open -= open ? 1u : 0u;
+                    n_drop += open ? 1u : 0u;
+                }
             }
         }
         if (sp == 0) break;

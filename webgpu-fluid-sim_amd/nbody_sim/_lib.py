@@ -56,7 +56,8 @@ EXPORTS = (
     "nbody_set_config", "nbody_get_config", "nbody_set_bodies", "nbody_get_bodies",
     "nbody_get_num_bodies", "nbody_step", "nbody_synchronize", "nbody_set_stream",
     "nbody_use_graph", "nbody_set_debug", "nbody_debug_get", "nbody_walk_stats",
-    "nbody_set_profiling", "nbody_phase_times", "nbody_set_tuning", "nbody_last_error",
+    "nbody_set_profiling", "nbody_phase_times", "nbody_set_tuning", "nbody_get_tuning",
+    "nbody_last_error",
     "nbody_generate", "nbody_philox4x32", "nbody_version", "nbody_adjoint_forward",
     "nbody_adjoint_backward",
 )
@@ -74,6 +75,15 @@ class NbodyConfig(ctypes.Structure):
         ("morton_mode", ctypes.c_int32),
         ("math_mode", ctypes.c_int32),
         ("offset", ctypes.c_uint32),
+    ]
+
+
+class NbodyTuning(ctypes.Structure):
+    _fields_ = [
+        ("walk_block", ctypes.c_int32),
+        ("walk_stack", ctypes.c_int32),
+        ("walk_variant", ctypes.c_int32),
+        ("xcd_order", ctypes.c_int32),
     ]
 
 
@@ -127,7 +137,8 @@ def lib() -> ctypes.CDLL:
         "nbody_set_profiling": (ctypes.c_int, [vp, i32]),
         "nbody_phase_times": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float),
                                              ctypes.POINTER(ctypes.c_int64), i32]),
-        "nbody_set_tuning": (ctypes.c_int, [vp, i32, i32, i32]),
+        "nbody_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(NbodyTuning)]),
+        "nbody_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(NbodyTuning)]),
         "nbody_last_error": (ctypes.c_char_p, [vp]),
         "nbody_generate": (ctypes.c_int, [i32, i64, ctypes.c_uint64,
                                           ctypes.POINTER(NbodyIcParams), vp, vp, vp]),

@@ -112,10 +112,20 @@ class _Handle:
         flags = (_lib.DEBUG_ACCEL if accel else 0) | (_lib.DEBUG_STATS if stats else 0)
         self._check(_lib.lib().nbody_set_debug(self.handle, flags))
 
-    def setTuning(self, walk_block: int = 0, walk_stack: int = 0, walk_variant: int = -1):
-        """Launch tunables (same results for every setting); 0 / -1 keep the current value."""
-        self._check(_lib.lib().nbody_set_tuning(self.handle, int(walk_block), int(walk_stack),
-                                                int(walk_variant)))
+    def getTuning(self) -> dict:
+        t = _lib.NbodyTuning()
+        self._check(_lib.lib().nbody_get_tuning(self.handle, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.NbodyTuning._fields_}
+
+    def setTuning(self, walk_block=None, walk_stack=None, walk_variant=None, xcd_order=None):
+        """Launch tunables (the results are the same for every setting); None keeps."""
+        cur = self.getTuning()
+        for k, v in (("walk_block", walk_block), ("walk_stack", walk_stack),
+                     ("walk_variant", walk_variant), ("xcd_order", xcd_order)):
+            if v is not None:
+                cur[k] = int(v)
+        t = _lib.NbodyTuning(**cur)
+        self._check(_lib.lib().nbody_set_tuning(self.handle, ctypes.byref(t)))
 
     def setProfiling(self, enable: bool = True):
         self._check(_lib.lib().nbody_set_profiling(self.handle, int(bool(enable))))
