@@ -12,7 +12,7 @@
 //   pcm float4[2C] children's {com, mass, length} per internal node (walk)
 //   paabb float4[2C] children's AABBs per internal node (fill)
 //   root float4[2], child uint2[C], range uint2[C], parent u32[2C-1], status u32[C]
-//   fill frontier u32[C], sort scratch, bbox partials, debug accel float2[C],
+//   fill hand-off lists u32[2C] + counts, sort scratch, bbox partials, debug accel float2[C],
 //   counters
 #include <hip/hip_runtime.h>
 
@@ -188,8 +188,9 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->tree.range = a.take<uint2>(cap);
         s->tree.parent = a.take<uint32_t>(nodes);
         s->tree.status = a.take<uint32_t>(cap);
-        s->tree.frontier = a.take<uint32_t>(cap);
-        s->tree.frontier_count = a.take<uint32_t>(1);
+        s->tree.frontier = a.take<uint32_t>(fill_frontier_words(cap));
+        s->tree.top = a.take<uint32_t>(cap);
+        s->tree.counts = a.take<uint32_t>(fill_count_words(cap));
         s->sort_words = a.take<uint32_t>(sort_words);
         s->bbox_partial = a.take<float4>((int64_t)s->max_chunks * s->num_sims);
         s->bbox = a.take<float4>(s->num_sims);
@@ -216,7 +217,7 @@ int allocate(nbody_sim *s, int64_t cap) {
     s->cap = cap;
     HIP_TRY(s, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->overflow, 0, sizeof(uint32_t), s->stream));
-    HIP_TRY(s, hipMemsetAsync(s->tree.frontier_count, 0, sizeof(uint32_t), s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->tree.counts, 0, 4 * (size_t)fill_count_words(cap), s->stream));
     return NBODY_OK;
 }
 

@@ -12,17 +12,20 @@
 //
 //   Agent-scope acquire/release removes the race but, on a multi-XCD MI355X,
 //   writes back and invalidates the XCD's L2 at every counter update; done by
-//   every thread at every level it costs milliseconds.  The climb is split
-//   so that almost all of it is ordered at workgroup scope instead:
+//   every thread at every level it costs milliseconds.  The climb is split in
+//   three phases so that every counter is ordered at workgroup scope:
 //     phase 1 (fill_local_kernel): workgroup b owns sorted leaves
-//       [b*1024, b*1024+1023] and completes every internal node whose leaf
-//       range lies inside that block, with the arrival counters in LDS.  A
-//       thread that completes a node whose parent spans blocks appends the
-//       node to a frontier list.
-//     phase 2 (fill_top_kernel): a few workgroups climb from the frontier
-//       nodes through the block-spanning nodes (a few per block boundary)
-//       with agent-scope counters; the kernel boundary makes phase 1's nodes
-//       visible to them.
+//       [b*1024, b*1024+1023].  It loads its slice of the child, range and
+//       parent arrays into LDS, completes every node whose leaf range lies in
+//       the block with LDS-resident node data and LDS arrival counters, then
+//       writes all completed nodes out cooperatively.  A node whose parent
+//       spans blocks goes to its super-block's (32 blocks) frontier bucket.
+//     phase 2 (fill_mid_kernel): one workgroup per super-block climbs from
+//       its bucket through the nodes inside the super-block (global arrival
+//       counters, workgroup scope); nodes whose parent spans super-blocks go
+//       to the top list.
+//     phase 3 (fill_top_kernel): one workgroup finishes the top of the tree.
+//   Kernel boundaries order the phases.
 //   The merge formula does not depend on which child arrives last, so the
 //   result is deterministic and equals a sequential post-order fill.  The
 //   reference's random leaf rotation (`offset`, lbvh.wgsl:171) is therefore
@@ -140,12 +143,9 @@ __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restri
 
 constexpr int kFillBlock = 1024;
 
-// lbvh.wgsl:205-228: node p from its two complete children, read from
-// p's child-pair slots.
-__device__ __forceinline__ void merge_children(const TreeBufs &tb, uint32_t p, float4 &C,
-                                               float4 &B) {
-    const float4 L = tb.pcm[2 * p], R = tb.pcm[2 * p + 1];
-    const float4 LB = tb.paabb[2 * p], RB = tb.paabb[2 * p + 1];
+// lbvh.wgsl:205-228: a node from its two complete children.
+__device__ __forceinline__ void merge_values(const float4 &L, const float4 &LB, const float4 &R,
+                                             const float4 &RB, float4 &C, float4 &B) {
     B.x = fminf(LB.x, RB.x);
     B.y = fminf(LB.y, RB.y);
     B.z = fmaxf(LB.z, RB.z);
@@ -159,6 +159,12 @@ __device__ __forceinline__ void merge_children(const TreeBufs &tb, uint32_t p, f
         C.y = 0.0f;
     }
     C.w = fmaxf(B.z - B.x, B.w - B.y);
+}
+
+// Node p from its children's records in p's child-pair slots.
+__device__ __forceinline__ void merge_children(const TreeBufs &tb, uint32_t p, float4 &C,
+                                               float4 &B) {
+    merge_values(tb.pcm[2 * p], tb.paabb[2 * p], tb.pcm[2 * p + 1], tb.paabb[2 * p + 1], C, B);
 }
 
 // Store a completed node's data where its readers look for it: the child
@@ -175,88 +181,151 @@ __device__ __forceinline__ void publish(const TreeBufs &tb, uint32_t node, uint3
     tb.paabb[slot] = B;
 }
 
+constexpr int kSuperBlocks = 32;                     // phase-1 blocks per phase-2 workgroup
+constexpr int kSuper = kFillBlock * kSuperBlocks;   // leaves per super-block
+
 __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *__restrict__ idx,
                                                                 const float *__restrict__ mass,
                                                                 const float2 *__restrict__ pos,
                                                                 int64_t n, TreeBufs tb) {
+    __shared__ uint2 s_child[kFillBlock];     // internal node base+j
+    __shared__ uint2 s_range[kFillBlock];
+    __shared__ uint32_t s_par_int[kFillBlock];
+    __shared__ uint32_t s_par_leaf[kFillBlock];  // leaf of sorted position base+j
+    __shared__ float4 s_cm_int[kFillBlock];
+    __shared__ float4 s_bb_int[kFillBlock];
+    __shared__ float4 s_cm_leaf[kFillBlock];
     __shared__ uint32_t s_arrive[kFillBlock];
-    s_arrive[threadIdx.x] = 0u;
-    __syncthreads();
+    __shared__ uint32_t s_done[kFillBlock];
+
+    const uint32_t j = threadIdx.x;
     const uint32_t base = blockIdx.x * kFillBlock;
-    const uint32_t last = (uint32_t)((int64_t)base + kFillBlock - 1 < n - 1 ? base + kFillBlock - 1
-                                                                             : n - 1);
-    const uint32_t t = base + threadIdx.x;
-    if (t >= (uint64_t)n) return;
     const uint32_t nint = (uint32_t)(n - 1);  // internal nodes
-    uint32_t cur = nint + t;
-    const uint32_t b = idx[t];
-    const float2 p = pos[b];
-    // lbvh.wgsl:179-187
-    float4 C = make_float4(p.x, p.y, mass[b], 0.0f);
-    float4 B = make_float4(p.x, p.y, p.x, p.y);
-    tb.leaf[t] = C;
-    if (n == 1) {
-        publish(tb, 0u, 0u, C, B);
-        return;
+    const uint32_t last = base + kFillBlock - 1 < (uint32_t)(n - 1) ? base + kFillBlock - 1
+                                                                   : (uint32_t)(n - 1);
+    const uint32_t t = base + j;
+    s_arrive[j] = 0u;
+    s_done[j] = 0u;
+    if (t < nint) {
+        s_child[j] = tb.child[t];
+        s_range[j] = tb.range[t];
+        s_par_int[j] = tb.parent[t];
     }
-    // lbvh.wgsl:191-232, restricted to nodes inside this block's leaf range.
-    // A valid tree reaches the root in <= 96 levels; the level bound and the
-    // index check only guard against a corrupt parent array.
+    float4 C, B;
+    if (t < (uint64_t)n) {
+        s_par_leaf[j] = n > 1 ? tb.parent[nint + t] : kNoNode;
+        const uint32_t b = idx[t];
+        const float2 p = pos[b];
+        // lbvh.wgsl:179-187
+        C = make_float4(p.x, p.y, mass[b], 0.0f);
+        B = make_float4(p.x, p.y, p.x, p.y);
+        s_cm_leaf[j] = C;
+        tb.leaf[t] = C;
+    }
+    __syncthreads();
+
+    // lbvh.wgsl:191-232, restricted to the nodes inside this block.
+    if (t < (uint64_t)n && n > 1) {
+        uint32_t cur = nint + t;
+        uint32_t pw = s_par_leaf[j];
+        for (int level = 0; level < 128; ++level) {
+            const uint32_t par = pw & kParentMask;
+            if (par >= nint) break;  // corrupt parent word
+            const uint32_t lp = par - base;
+            if (par < base || par > last || s_range[lp].x < base || s_range[lp].y > last) {
+                const uint32_t sb = base / kSuper;
+                const uint32_t slot = atomicAdd(&tb.counts[sb], 1u);
+                tb.frontier[(size_t)sb * kSuper + slot] = cur;
+                break;
+            }
+            const uint32_t prev = __hip_atomic_fetch_add(&s_arrive[lp], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == 0u) break;
+            const uint2 ch = s_child[lp];
+            float4 L, LB, R, RB;
+            if (ch.x >= nint) {
+                L = s_cm_leaf[ch.x - nint - base];
+                LB = make_float4(L.x, L.y, L.x, L.y);
+            } else {
+                L = s_cm_int[ch.x - base];
+                LB = s_bb_int[ch.x - base];
+            }
+            if (ch.y >= nint) {
+                R = s_cm_leaf[ch.y - nint - base];
+                RB = make_float4(R.x, R.y, R.x, R.y);
+            } else {
+                R = s_cm_int[ch.y - base];
+                RB = s_bb_int[ch.y - base];
+            }
+            merge_values(L, LB, R, RB, C, B);
+            s_cm_int[lp] = C;
+            s_bb_int[lp] = B;
+            s_done[lp] = 1u;
+            cur = par;
+            if (cur == 0u) break;
+            pw = s_par_int[lp];
+        }
+    }
+    __syncthreads();
+
+    // Write the block's completed nodes where their readers look for them.
+    if (t < (uint64_t)n) {
+        const float4 c = s_cm_leaf[j];
+        if (n == 1)
+            publish(tb, 0u, 0u, c, make_float4(c.x, c.y, c.x, c.y));
+        else
+            publish(tb, nint + t, s_par_leaf[j], c, make_float4(c.x, c.y, c.x, c.y));
+    }
+    if (t < nint && s_done[j]) publish(tb, t, t ? s_par_int[j] : 0u, s_cm_int[j], s_bb_int[j]);
+}
+
+// Climb from `cur` (its data already published) while the parents stay in
+// [lo, hi]; a node whose parent leaves that range goes to out[atomic(*out_n)].
+__device__ __forceinline__ void climb_wg(const TreeBufs &tb, uint32_t nint, uint32_t cur,
+                                         uint32_t lo, uint32_t hi, uint32_t *out,
+                                         uint32_t *out_n) {
     for (int level = 0; level < 128; ++level) {
-        const uint32_t pw = tb.parent[cur];
-        const uint32_t par = pw & kParentMask;
+        const uint32_t par = tb.parent[cur] & kParentMask;
         if (par >= nint) break;
-        publish(tb, cur, pw, C, B);
         const uint2 r = tb.range[par];
-        if (r.x < base || r.y > last) {
-            const uint32_t slot = atomicAdd(tb.frontier_count, 1u);
-            tb.frontier[slot] = cur;
+        if (r.x < lo || r.y > hi) {
+            out[atomicAdd(out_n, 1u)] = cur;
             break;
         }
-        const uint32_t prev = __hip_atomic_fetch_add(&s_arrive[par - base], 1u, __ATOMIC_ACQ_REL,
+        const uint32_t prev = __hip_atomic_fetch_add(&tb.status[par], 1u, __ATOMIC_ACQ_REL,
                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
         if (prev == 0u) break;
+        float4 C, B;
         merge_children(tb, par, C, B);
         cur = par;
-        if (cur == 0u) {
-            publish(tb, 0u, 0u, C, B);
-            break;
-        }
+        publish(tb, cur, cur ? tb.parent[cur] : 0u, C, B);
+        if (cur == 0u) break;
     }
 }
 
-// Phase 2: climb from the frontier through the block-spanning nodes (a few
-// per block boundary).  Threads of several workgroups (on any XCD) meet at
-// these nodes, so their counters are agent-scope acquire-release; with only
-// the frontier's few thousand threads doing so, the L2 write-back and
-// invalidate they imply stay cheap.  Each frontier node's data is already
-// in its parent's child slot.
-constexpr int kTopBlocks = 32;
-constexpr int kTopThreads = 256;
-
-__global__ __launch_bounds__(kTopThreads) void fill_top_kernel(int64_t n, TreeBufs tb) {
-    const uint32_t count = *tb.frontier_count;
+// Phase 2: workgroup sb owns super-block sb's leaves.
+__global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBufs tb,
+                                                              uint32_t num_super) {
+    const uint32_t sb = blockIdx.x;
     const uint32_t nint = (uint32_t)(n - 1);
-    for (uint32_t k = blockIdx.x * kTopThreads + threadIdx.x; k < count;
-         k += kTopBlocks * kTopThreads) {
-        uint32_t cur = tb.frontier[k];
-        for (int level = 0; level < 128; ++level) {
-            const uint32_t par = tb.parent[cur] & kParentMask;
-            if (par >= nint) break;
-            const uint32_t prev = __hip_atomic_fetch_add(&tb.status[par], 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == 0u) break;
-            float4 C, B;
-            merge_children(tb, par, C, B);
-            cur = par;
-            publish(tb, cur, cur ? tb.parent[cur] : 0u, C, B);
-            if (cur == 0u) break;
-        }
-    }
+    const uint32_t lo = sb * (uint32_t)kSuper;
+    const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
+    const uint32_t count = tb.counts[sb];
+    for (uint32_t k = threadIdx.x; k < count; k += kFillBlock)
+        climb_wg(tb, nint, tb.frontier[(size_t)sb * kSuper + k], lo, hi, tb.top,
+                 &tb.counts[num_super]);
 }
 
-// Reset the frontier counter for the next step (after phase 2 read it).
-__global__ void fill_reset_kernel(uint32_t *count) { *count = 0u; }
+// Phase 3: one workgroup finishes the tree, then clears the hand-off counts.
+__global__ __launch_bounds__(kFillBlock) void fill_top_kernel(int64_t n, TreeBufs tb,
+                                                              uint32_t num_super) {
+    const uint32_t nint = (uint32_t)(n - 1);
+    const uint32_t count = tb.counts[num_super];
+    for (uint32_t k = threadIdx.x; k < count; k += kFillBlock)
+        climb_wg(tb, nint, tb.top[k], 0u, nint, tb.top, &tb.counts[num_super]);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k <= num_super; k += kFillBlock) tb.counts[k] = 0u;
+}
 
 }  // namespace
 
@@ -275,12 +344,16 @@ void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, c
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
                  const TreeBufs &t, hipStream_t s) {
     if (n < 1) return;
-    fill_local_kernel<<<(unsigned)((n + kFillBlock - 1) / kFillBlock), kFillBlock, 0, s>>>(
-        idx, mass, pos, n, t);
+    const unsigned blocks = (unsigned)((n + kFillBlock - 1) / kFillBlock);
+    fill_local_kernel<<<blocks, kFillBlock, 0, s>>>(idx, mass, pos, n, t);
     if (n > kFillBlock) {
-        fill_top_kernel<<<kTopBlocks, kTopThreads, 0, s>>>(n, t);
-        fill_reset_kernel<<<1, 1, 0, s>>>(t.frontier_count);
+        const uint32_t num_super = (uint32_t)((n + kSuper - 1) / kSuper);
+        fill_mid_kernel<<<num_super, kFillBlock, 0, s>>>(n, t, num_super);
+        fill_top_kernel<<<1, kFillBlock, 0, s>>>(n, t, num_super);
     }
 }
+
+int64_t fill_frontier_words(int64_t cap) { return ((cap + kSuper - 1) / kSuper) * kSuper; }
+int64_t fill_count_words(int64_t cap) { return (cap + kSuper - 1) / kSuper + 1; }
 
 }  // namespace nbs

@@ -26,6 +26,9 @@
 // (barnes_hut.wgsl:100-105) and raises an overflow counter.
 #include "nbody_internal.h"
 
+// v_writelane_b32 through its LLVM intrinsic (no clang builtin in this toolchain).
+__device__ int nbs_writelane(int value, int lane_sel, int old) __asm("llvm.amdgcn.writelane.i32");
+
 namespace nbs {
 
 namespace {
@@ -304,40 +307,34 @@ __device__ __forceinline__ void lane_test(const float4 c, bool leaf, bool act, f
 
 // Register-resident stack of one wavefront: entry k lives in lane k & 63 of
 // row k >> 6, one VGPR per field, so a push or pop needs no LDS access, no
-// exec-mask change and no readfirstlane.  A push is seven v_writelane_b32
-// (lane k takes the wave-uniform SGPR value; the other lanes keep theirs)
-// with the lane select in M0 -- GFX9 VALU instructions read at most one SGPR
-// through the constant bus besides M0.  No memory is accessed.
-#define NBS_WRITELANES(a0, a1, a2, a3, a4, a5, a6)                                          \
-    asm("s_mov_b32 m0, %14\n\t"                                                              \
-        "v_writelane_b32 %0, %7, m0\n\t"                                                     \
-        "v_writelane_b32 %1, %8, m0\n\t"                                                     \
-        "v_writelane_b32 %2, %9, m0\n\t"                                                     \
-        "v_writelane_b32 %3, %10, m0\n\t"                                                    \
-        "v_writelane_b32 %4, %11, m0\n\t"                                                    \
-        "v_writelane_b32 %5, %12, m0\n\t"                                                    \
-        "v_writelane_b32 %6, %13, m0"                                                         \
-        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6)                \
-        : "s"(v0), "s"(v1), "s"(v2), "s"(v3), "s"(v4), "s"(v5), "s"(v6), "s"(l)               \
-        : "m0")
+// exec-mask change and no readfirstlane.  A push writes lane k of each field
+// with v_writelane_b32 (the LLVM intrinsic, called by its IR name; the
+// backend puts the lane select in M0 as GFX9 requires), a pop reads it back
+// with v_readlane_b32.  No memory is accessed.
 
 struct RegStack {
     int node[2], mlo[2], mhi[2], cx[2], cy[2], cm[2], cw[2];
 
     __device__ __forceinline__ void put(int row, int lane_sel, uint32_t nd, uint64_t m,
                                         const float4 &c) {
-        const int v0 = __builtin_amdgcn_readfirstlane((int)nd);
-        const int v1 = __builtin_amdgcn_readfirstlane((int)(uint32_t)m);
-        const int v2 = __builtin_amdgcn_readfirstlane((int)(uint32_t)(m >> 32));
-        const int v3 = __builtin_amdgcn_readfirstlane(__float_as_int(c.x));
-        const int v4 = __builtin_amdgcn_readfirstlane(__float_as_int(c.y));
-        const int v5 = __builtin_amdgcn_readfirstlane(__float_as_int(c.z));
-        const int v6 = __builtin_amdgcn_readfirstlane(__float_as_int(c.w));
-        const int l = __builtin_amdgcn_readfirstlane(lane_sel);
-        if (row == 0)
-            NBS_WRITELANES(node[0], mlo[0], mhi[0], cx[0], cy[0], cm[0], cw[0]);
-        else
-            NBS_WRITELANES(node[1], mlo[1], mhi[1], cx[1], cy[1], cm[1], cw[1]);
+        const int l = lane_sel;
+        if (row == 0) {
+            node[0] = nbs_writelane((int)nd, l, node[0]);
+            mlo[0] = nbs_writelane((int)(uint32_t)m, l, mlo[0]);
+            mhi[0] = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi[0]);
+            cx[0] = nbs_writelane(__float_as_int(c.x), l, cx[0]);
+            cy[0] = nbs_writelane(__float_as_int(c.y), l, cy[0]);
+            cm[0] = nbs_writelane(__float_as_int(c.z), l, cm[0]);
+            cw[0] = nbs_writelane(__float_as_int(c.w), l, cw[0]);
+        } else {
+            node[1] = nbs_writelane((int)nd, l, node[1]);
+            mlo[1] = nbs_writelane((int)(uint32_t)m, l, mlo[1]);
+            mhi[1] = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi[1]);
+            cx[1] = nbs_writelane(__float_as_int(c.x), l, cx[1]);
+            cy[1] = nbs_writelane(__float_as_int(c.y), l, cy[1]);
+            cm[1] = nbs_writelane(__float_as_int(c.z), l, cm[1]);
+            cw[1] = nbs_writelane(__float_as_int(c.w), l, cw[1]);
+        }
     }
     __device__ __forceinline__ void get(int row, int lane_sel, uint32_t &nd, uint64_t &m,
                                         float4 &c) const {
