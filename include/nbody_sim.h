@@ -148,7 +148,8 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *   walk_block   walk workgroup size (64..1024, multiple of 64)
  *   walk_stack   walk stack entries per wavefront (8..256; the register stack
  *                holds at most 128)
- *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default)
+ *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default); 2: as 1,
+ *                with the children's records loaded before the opening test
  *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
  *                of Morton-ordered bodies (default); 0: hardware order */
 typedef struct nbody_tuning {

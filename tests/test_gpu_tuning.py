@@ -34,7 +34,7 @@ def test_walk_variants_identical(math):
     cfg = cfg.with_(mathMode=math)
     ref = run(cfg, bodies, (256, 128, 0))
     for tuning in [(64, 64, 0, 1), (1024, 256, 0, 0), (256, 128, 1, 0), (64, 64, 1, 1),
-                   (512, 128, 1, 1)]:
+                   (512, 128, 1, 1), (256, 128, 2, 1), (128, 64, 2, 0)]:
         got = run(cfg, bodies, tuning)
         assert np.array_equal(got[0], ref[0]), tuning
         assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), tuning
@@ -47,13 +47,15 @@ def test_register_stack_walk_ensemble_and_reference_modes():
     bodies = [np.concatenate([p[i] for p in parts]) for i in range(3)]
     cfg, _ = params(theta=0.6)
     a = run(cfg.with_(mathMode="exact"), bodies, (256, 128, 0), ensemble_sizes=sizes)
-    b = run(cfg.with_(mathMode="exact"), bodies, (128, 128, 1), ensemble_sizes=sizes)
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for variant in (1, 2):
+        b = run(cfg.with_(mathMode="exact"), bodies, (128, 128, variant), ensemble_sizes=sizes)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), variant
     bodies = nbody_sim.generate("two-clusters", 5000, seed=3)
     cfg, _ = params(theta=0.6, dt=0.1 / 60, soft="reference", morton="reference")
     a = run(cfg, bodies, (256, 128, 0))
-    b = run(cfg, bodies, (256, 128, 1))
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for variant in (1, 2):
+        b = run(cfg, bodies, (256, 128, variant))
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), variant
 
 
 def test_autotune_picks_a_working_configuration():

@@ -695,8 +695,8 @@ int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");
     if (t->walk_stack < 8 || t->walk_stack > 256)
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_stack must be in [8, 256]");
-    if (t->walk_variant != 0 && t->walk_variant != 1)
-        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0 or 1");
+    if (t->walk_variant < 0 || t->walk_variant > 2)
+        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0, 1 or 2");
     s->walk_block = t->walk_block;
     s->walk_stack = t->walk_stack;
     s->walk_variant = t->walk_variant;

@@ -18,22 +18,11 @@
 //     lx_j += m_j sum_{n ∋ j} lc_n / M_n
 //     lm_j += sum_{n ∋ j} [ x_j . lc_n / M_n - c_n . lc_n / M_n + lM_n ]
 //                                                        (adjoint_fold_kernel)
-#include "nbody_internal.h"
+#include "walk_common.h"
 
 namespace nbs {
 
 namespace {
-
-template <typename T>
-using const_as = const __attribute__((address_space(4))) T;
-
-__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-    const uint32_t lo = uniform_u32((uint32_t)v), hi = uniform_u32((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
 
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -54,30 +43,18 @@ __global__ __launch_bounds__(256) void adjoint_pre_kernel(const float2 *__restri
     la[i] = make_float2(v.x * dt, v.y * dt);
 }
 
-// Same traversal and opening test as walk_kernel (k_walk.hip), different
-// action on accepted nodes.
+// The forward walk's traversal and opening test (walk_common.h), with the
+// VJP of the pull on accepted nodes instead of the pull.
 template <bool kExact>
 __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, int stack_cap) {
-    extern __shared__ float4 s_dyn[];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int waves = blockDim.x >> 6;
-    float4 *s_data = s_dyn + (size_t)wid * stack_cap;
-    uint64_t *s_mask = reinterpret_cast<uint64_t *>(s_dyn + (size_t)waves * stack_cap) +
-                       (size_t)wid * stack_cap;
-    uint32_t *s_node = reinterpret_cast<uint32_t *>(reinterpret_cast<uint64_t *>(
-                           s_dyn + (size_t)waves * stack_cap) + (size_t)waves * stack_cap) +
-                       (size_t)wid * stack_cap;
-
+    const int lane = threadIdx.x & 63;
     const uint32_t n = a.n;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = t < n;
     const uint32_t leaf0 = n - 1u;
     const uint32_t my_leaf = leaf0 + t;
-    const_as<uint32_t> *child_s = (const_as<uint32_t> *)a.child;
-    const_as<float> *pcmf_s = (const_as<float> *)a.pcm;
 
-    float px = 0.0f, py = 0.0f;
-    float gx = 0.0f, gy = 0.0f;
+    float px = 0.0f, py = 0.0f, gx = 0.0f, gy = 0.0f;
     uint32_t bi = 0;
     if (valid) {
         const float4 me = a.leaf[t];
@@ -93,82 +70,36 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
 
     const float G = a.grav, theta = a.theta, eps2 = a.eps * a.eps;
     float hx_sum = 0.0f, hy_sum = 0.0f;
-    int sp = 0;
-    uint32_t node = 0u;
-    uint64_t mask = live;
-    float4 c = a.root[0];
-    const uint32_t max_iter = 4u * n + 64u;
-    for (uint32_t iter = 0;; ++iter) {
-        if (iter >= max_iter) {
-            if (lane == 0) atomicOr(a.overflow, 1u);
-            break;
-        }
-        const bool leaf = node >= leaf0;
-        bool open = false, acc = false;
-        float hx = 0.0f, hy = 0.0f, dm = 0.0f;
-        if (((mask >> lane) & 1u) && !(leaf && node == my_leaf)) {
+    RegStack st;
+    walk_tree<false>(
+        a.pcm, a.child, leaf0, my_leaf, lane, 0u, live, a.root[0], st, 0, stack_cap, a.overflow,
+        [&](const float4 &c, bool leaf, uint32_t node, bool act, bool &acc, bool &open) {
             const float rx = c.x - px, ry = c.y - py;
-            float inv;
-            if constexpr (kExact) {
-                const float d2 = rx * rx + ry * ry;
-                inv = 1.0f / sqrtf(d2 + eps2);
-            } else {
-                inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2)));
-            }
-            if (leaf || c.w * inv < theta) {
-                acc = true;
-                const float inv2 = inv * inv;
-                const float inv3 = inv2 * inv;
-                const float rg = rx * gx + ry * gy;
-                const float gm = G * c.z;
-                const float k3 = 3.0f * inv2 * rg;
-                hx = gm * inv3 * (gx - k3 * rx);
-                hy = gm * inv3 * (gy - k3 * ry);
-                dm = G * inv3 * rg;
-                hx_sum += hx;
-                hy_sum += hy;
-            } else {
-                open = true;
-            }
-        }
-        // Node cotangent: one atomic per wavefront and component.
-        if (__ballot(acc)) {
-            const float sx = wave_sum(hx), sy = wave_sum(hy), sm = wave_sum(dm);
-            if (lane == 0) {
-                float4 *cot = a.cot + node;
-                atomicAdd(&cot->x, sx);
-                atomicAdd(&cot->y, sy);
-                atomicAdd(&cot->z, sm);
-            }
-        }
-        const uint64_t om = __ballot(open);
-        if (om) {
-            const uint32_t pn = node;
-            const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
-                                          pcmf_s[8 * pn + 3]);
-            const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5],
-                                          pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
-            const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
-            if (sp < stack_cap) {
+            const float inv = inv_dist<kExact>(rx, ry, eps2);
+            const bool near = !(c.w * inv < theta);
+            acc = act && (leaf || !near);
+            open = act && !leaf && near;
+            const float inv2 = inv * inv;
+            const float inv3 = inv2 * inv;
+            const float rg = rx * gx + ry * gy;
+            const float gm = G * c.z;
+            const float k3 = 3.0f * inv2 * rg;
+            const float hx = acc ? gm * inv3 * (gx - k3 * rx) : 0.0f;
+            const float hy = acc ? gm * inv3 * (gy - k3 * ry) : 0.0f;
+            const float dm = acc ? G * inv3 * rg : 0.0f;
+            hx_sum += hx;
+            hy_sum += hy;
+            // node cotangent: one atomic per wavefront and component
+            if (__ballot(acc)) {
+                const float sx = wave_sum(hx), sy = wave_sum(hy), sm = wave_sum(dm);
                 if (lane == 0) {
-                    s_node[sp] = chl;
-                    s_mask[sp] = om;
-                    s_data[sp] = cl;
+                    float4 *cot = a.cot + node;
+                    atomicAdd(&cot->x, sx);
+                    atomicAdd(&cot->y, sy);
+                    atomicAdd(&cot->z, sm);
                 }
-                ++sp;
-                node = chr;
-                mask = om;
-                c = cr;
-                continue;
             }
-            if (lane == 0) atomicOr(a.overflow, 1u);
-        }
-        if (sp == 0) break;
-        --sp;
-        node = uniform_u32(s_node[sp]);
-        mask = uniform_u64(s_mask[sp]);
-        c = s_data[sp];
-    }
+        });
     if (valid) {
         float2 l = a.lx[bi];
         l.x -= hx_sum;
@@ -228,12 +159,11 @@ void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, hipStream_
     if (a.n == 0) return;
     const unsigned block = 256;
     const unsigned grid = (a.n + block - 1) / block;
-    const size_t lds = (size_t)(block / kWave) * (size_t)stack *
-                       (sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t));
+    const int cap = stack < 128 ? stack : 128;
     if (exact)
-        adjoint_walk_kernel<true><<<grid, block, lds, s>>>(a, stack);
+        adjoint_walk_kernel<true><<<grid, block, 0, s>>>(a, cap);
     else
-        adjoint_walk_kernel<false><<<grid, block, lds, s>>>(a, stack);
+        adjoint_walk_kernel<false><<<grid, block, 0, s>>>(a, cap);
 }
 
 void launch_adjoint_fold(const AdjointArgs &a, hipStream_t s) {

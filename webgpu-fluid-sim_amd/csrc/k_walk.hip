@@ -24,28 +24,11 @@
 // ensemble's 38-bit keys) and the stack never holds more than depth + 1
 // entries.  A full stack drops both children like the reference
 // (barnes_hut.wgsl:100-105) and raises an overflow counter.
-#include "nbody_internal.h"
-
-// v_writelane_b32 through its LLVM intrinsic (no clang builtin in this toolchain).
-__device__ int nbs_writelane(int value, int lane_sel, int old) __asm("llvm.amdgcn.writelane.i32");
+#include "walk_common.h"
 
 namespace nbs {
 
 namespace {
-
-// Tree records are read-only during the walk and indexed by a wave-uniform
-// node id; reading them through the constant address space makes them
-// scalar loads (s_load) issued once per wavefront.
-template <typename T>
-using const_as = const __attribute__((address_space(4))) T;
-
-__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-    const uint32_t lo = uniform_u32((uint32_t)v), hi = uniform_u32((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
 
 // One interaction test of body (px, py, m1) against node record c.
 // Returns true when the lane must open the node; otherwise adds the
@@ -305,52 +288,7 @@ __device__ __forceinline__ void lane_test(const float4 c, bool leaf, bool act, f
     ay = acc ? ny : ay;
 }
 
-// Register-resident stack of one wavefront: entry k lives in lane k & 63 of
-// row k >> 6, one VGPR per field, so a push or pop needs no LDS access, no
-// exec-mask change and no readfirstlane.  A push writes lane k of each field
-// with v_writelane_b32 (the LLVM intrinsic, called by its IR name; the
-// backend puts the lane select in M0 as GFX9 requires), a pop reads it back
-// with v_readlane_b32.  No memory is accessed.
-
-struct RegStack {
-    int node[2], mlo[2], mhi[2], cx[2], cy[2], cm[2], cw[2];
-
-    __device__ __forceinline__ void put(int row, int lane_sel, uint32_t nd, uint64_t m,
-                                        const float4 &c) {
-        const int l = lane_sel;
-        if (row == 0) {
-            node[0] = nbs_writelane((int)nd, l, node[0]);
-            mlo[0] = nbs_writelane((int)(uint32_t)m, l, mlo[0]);
-            mhi[0] = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi[0]);
-            cx[0] = nbs_writelane(__float_as_int(c.x), l, cx[0]);
-            cy[0] = nbs_writelane(__float_as_int(c.y), l, cy[0]);
-            cm[0] = nbs_writelane(__float_as_int(c.z), l, cm[0]);
-            cw[0] = nbs_writelane(__float_as_int(c.w), l, cw[0]);
-        } else {
-            node[1] = nbs_writelane((int)nd, l, node[1]);
-            mlo[1] = nbs_writelane((int)(uint32_t)m, l, mlo[1]);
-            mhi[1] = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi[1]);
-            cx[1] = nbs_writelane(__float_as_int(c.x), l, cx[1]);
-            cy[1] = nbs_writelane(__float_as_int(c.y), l, cy[1]);
-            cm[1] = nbs_writelane(__float_as_int(c.z), l, cm[1]);
-            cw[1] = nbs_writelane(__float_as_int(c.w), l, cw[1]);
-        }
-    }
-    __device__ __forceinline__ void get(int row, int lane_sel, uint32_t &nd, uint64_t &m,
-                                        float4 &c) const {
-        const int r = row;
-        nd = (uint32_t)__builtin_amdgcn_readlane(r ? node[1] : node[0], lane_sel);
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(r ? mlo[1] : mlo[0], lane_sel);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(r ? mhi[1] : mhi[0], lane_sel);
-        m = ((uint64_t)hi << 32) | lo;
-        c.x = __int_as_float(__builtin_amdgcn_readlane(r ? cx[1] : cx[0], lane_sel));
-        c.y = __int_as_float(__builtin_amdgcn_readlane(r ? cy[1] : cy[0], lane_sel));
-        c.z = __int_as_float(__builtin_amdgcn_readlane(r ? cm[1] : cm[0], lane_sel));
-        c.w = __int_as_float(__builtin_amdgcn_readlane(r ? cw[1] : cw[0], lane_sel));
-    }
-};
-
-template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
+template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, bool kPrefetch>
 __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int stack_cap,
                                                        uint32_t xcd_per) {
     const int lane = threadIdx.x & 63;
@@ -361,8 +299,6 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     const bool valid = t < n;
     const uint32_t leaf0 = n - 1u;
     const uint32_t my_leaf = leaf0 + t;
-    const_as<uint32_t> *child_s = (const_as<uint32_t> *)a.child;
-    const_as<float> *pcmf_s = (const_as<float> *)a.pcm;
 
     float px = 0.0f, py = 0.0f, m1 = 0.0f;
     if (valid) {
@@ -411,63 +347,16 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
         c = a.root[0];
     }
 
-    for (;;) {
-        const bool leaf = node >= leaf0;
-        const bool act = ((mask >> lane) & 1u) && !(leaf && node == my_leaf);
-        bool acc, open;
-        lane_test<kExact, kRefSoft>(c, leaf, act, px, py, m1, p, eps2c, ax, ay, acc, open);
-        if constexpr (kStats) {
-            n_acc += acc ? 1u : 0u;
-            n_open += open ? 1u : 0u;
-        }
-        const uint64_t om = __ballot(open);
-        if (om) {
-            // Both children's records in one scalar load; push left, go right
-            // (the reference pops the right child first).
-            const uint32_t pn = node;
-            const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
-                                          pcmf_s[8 * pn + 3]);
-            const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5],
-                                          pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
-            const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
-            if (chr >= leaf0) {
-                // A leaf child has no subtree: its lanes take it in this
-                // iteration, at the same place in the order.
-                const bool inr = ((om >> lane) & 1u) != 0u;
-                bool a2, o2;
-                lane_test<kExact, kRefSoft>(cr, true, inr && chr != my_leaf, px, py, m1, p,
-                                            eps2c, ax, ay, a2, o2);
-                if constexpr (kStats) n_acc += a2 ? 1u : 0u;
-                if (chl < leaf0) {
-                    node = chl;
-                    mask = om;
-                    c = cl;
-                    continue;
-                }
-                lane_test<kExact, kRefSoft>(cl, true, inr && chl != my_leaf, px, py, m1, p,
-                                            eps2c, ax, ay, a2, o2);
-                if constexpr (kStats) n_acc += a2 ? 1u : 0u;
-                // both children done: fall through to the pop
-            } else if (sp < stack_cap) {
-                st.put(sp >> 6, sp & 63, chl, om, cl);
-                ++sp;
-                node = chr;
-                mask = om;
-                c = cr;
-                continue;
-            } else {
-                // full stack: drop both children like barnes_hut.wgsl:100-105
-                if (lane == 0) atomicOr(a.overflow, 1u);
-                if constexpr (kStats) {
-                    n_open -= open ? 1u : 0u;
-                    n_drop += open ? 1u : 0u;
-                }
-            }
-        }
-        if (sp == 0) break;
-        --sp;
-        st.get(sp >> 6, sp & 63, node, mask, c);
-    }
+    walk_tree<kPrefetch>(a.pcm, a.child, leaf0, my_leaf, lane, node, mask, c, st, sp, stack_cap,
+                         a.overflow,
+                         [&](const float4 &cc, bool leaf, uint32_t, bool act, bool &acc, bool &open) {
+                             lane_test<kExact, kRefSoft>(cc, leaf, act, px, py, m1, p, eps2c, ax,
+                                                         ay, acc, open);
+                             if constexpr (kStats) {
+                                 n_acc += acc ? 1u : 0u;
+                                 n_open += open ? 1u : 0u;
+                             }
+                         });
 
     if (valid) {
         const uint32_t bi = a.idx[t];
@@ -513,7 +402,12 @@ void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, hi
         grid = per * 8u;  // blocks past the bodies exit at once
     }
     if (variant == 1) {
-        walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats><<<grid, block, 0, s>>>(
+        walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats, false><<<grid, block, 0, s>>>(
+            a, stack < 128 ? stack : 128, per);
+        return;
+    }
+    if (variant == 2) {
+        walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats, true><<<grid, block, 0, s>>>(
             a, stack < 128 ? stack : 128, per);
         return;
     }
