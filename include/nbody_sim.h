@@ -148,15 +148,19 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *   walk_block   walk workgroup size (64..1024, multiple of 64)
  *   walk_stack   walk stack entries per wavefront (8..256; the register stack
  *                holds at most 128)
- *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default); 2: as 1,
- *                with the children's records loaded before the opening test
+ *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default)
  *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
- *                of Morton-ordered bodies (default); 0: hardware order */
+ *                of Morton-ordered bodies (default); 0: hardware order
+ *   lane_bodies  bodies per lane of the VGPR-stack walk and the adjoint walk:
+ *                1, 2 (default) or 4; a wavefront walks 64 * lane_bodies
+ *                Morton-consecutive bodies together (the LDS walk ignores it,
+ *                walking one body per lane) */
 typedef struct nbody_tuning {
     int32_t walk_block;
     int32_t walk_stack;
     int32_t walk_variant;
     int32_t xcd_order;
+    int32_t lane_bodies;
 } nbody_tuning;
 int nbody_set_tuning(nbody_sim *sim, const nbody_tuning *t);
 int nbody_get_tuning(const nbody_sim *sim, nbody_tuning *out);

@@ -22,9 +22,11 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def gpu_grads(cfg, mass, pos, vel, steps, every, wx, wv):
+def gpu_grads(cfg, mass, pos, vel, steps, every, wx, wv, lane_bodies=None):
     sim = nbody_sim.Simulation(cfg)
     sim.setBodies(mass, pos, vel)
+    if lane_bodies is not None:
+        sim.setTuning(lane_bodies=lane_bodies)
     sim.adjointForward(steps, every)
     _, pT, vT = sim.getBodies()
     g = sim.adjointBackward(wx, wv)
@@ -59,6 +61,22 @@ def test_adjoint_matches_frozen_fp64_autograd(math):
     print(f"adjoint vs FP64 autograd ({math}): pos {ex:.2e} vel {ev:.2e} mass {em:.2e}")
     tol = 1e-4 if math == "exact" else 1e-3
     assert ex < tol and ev < tol and em < tol
+
+
+def test_adjoint_lane_bodies_agree():
+    """Bodies per lane change only the order of the node-cotangent atomics."""
+    n, steps, every = 5000, 6, 3
+    mass, pos, vel = nbody_sim.generate("plummer", n, seed=9)
+    cfg, _ = params(theta=0.6, dt=1e-4, eps=1e-3)
+    rng = np.random.default_rng(1)
+    wx = rng.normal(size=(n, 2)).astype(np.float32)
+    wv = rng.normal(size=(n, 2)).astype(np.float32)
+    ref, pT, _ = gpu_grads(cfg, mass, pos, vel, steps, every, wx, wv, lane_bodies=1)
+    for lb in (2, 4):
+        got, pT2, _ = gpu_grads(cfg, mass, pos, vel, steps, every, wx, wv, lane_bodies=lb)
+        assert np.array_equal(pT, pT2)
+        for a, b in zip(got, ref):
+            assert rel(a, b) < 1e-5, lb
 
 
 def test_adjoint_central_differences_64_coordinates():

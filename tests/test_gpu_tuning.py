@@ -1,5 +1,6 @@
-"""Launch tunables (walk variant, workgroup size, stack depth, XCD order)
-never change results, and the autotuner picks a configuration that runs."""
+"""Launch tunables (walk variant, workgroup size, stack depth, XCD order,
+bodies per lane) never change results, and the autotuner picks a
+configuration that runs."""
 import numpy as np
 import pytest
 
@@ -33,8 +34,9 @@ def test_walk_variants_identical(math):
     cfg, _ = params(theta=0.7, dt=5e-5)
     cfg = cfg.with_(mathMode=math)
     ref = run(cfg, bodies, (256, 128, 0))
-    for tuning in [(64, 64, 0, 1), (1024, 256, 0, 0), (256, 128, 1, 0), (64, 64, 1, 1),
-                   (512, 128, 1, 1), (256, 128, 2, 1), (128, 64, 2, 0)]:
+    for tuning in [(64, 64, 0, 1), (1024, 256, 0, 0), (256, 128, 1, 0, 1), (64, 64, 1, 1, 1),
+                   (512, 128, 1, 1, 2), (256, 128, 1, 0, 2), (128, 64, 1, 1, 4),
+                   (1024, 128, 1, 0, 4)]:
         got = run(cfg, bodies, tuning)
         assert np.array_equal(got[0], ref[0]), tuning
         assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), tuning
@@ -47,15 +49,15 @@ def test_register_stack_walk_ensemble_and_reference_modes():
     bodies = [np.concatenate([p[i] for p in parts]) for i in range(3)]
     cfg, _ = params(theta=0.6)
     a = run(cfg.with_(mathMode="exact"), bodies, (256, 128, 0), ensemble_sizes=sizes)
-    for variant in (1, 2):
-        b = run(cfg.with_(mathMode="exact"), bodies, (128, 128, variant), ensemble_sizes=sizes)
-        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), variant
+    for lb in (1, 2, 4):
+        b = run(cfg.with_(mathMode="exact"), bodies, (128, 128, 1, 1, lb), ensemble_sizes=sizes)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), lb
     bodies = nbody_sim.generate("two-clusters", 5000, seed=3)
     cfg, _ = params(theta=0.6, dt=0.1 / 60, soft="reference", morton="reference")
     a = run(cfg, bodies, (256, 128, 0))
-    for variant in (1, 2):
-        b = run(cfg, bodies, (256, 128, variant))
-        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), variant
+    for lb in (1, 2, 4):
+        b = run(cfg, bodies, (256, 128, 1, 1, lb))
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), lb
 
 
 def test_autotune_picks_a_working_configuration():
@@ -63,7 +65,8 @@ def test_autotune_picks_a_working_configuration():
     cfg, _ = params(theta=0.7, dt=5e-5)
     sim = nbody_sim.Simulation(cfg)
     sim.setBodies(*bodies)
-    best, table = autotune(sim, steps=2, blocks=(128, 256), stacks=(128,), xcd=(1,))
+    best, table = autotune(sim, steps=2, blocks=(128, 256), stacks=(128,), xcd=(1,),
+                           lane_bodies=(1, 2))
     assert best in table and all(v > 0 for v in table.values())
     _, p0, _ = sim.getBodies()
     assert np.array_equal(p0, bodies[1])  # state restored

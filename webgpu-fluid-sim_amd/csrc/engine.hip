@@ -81,6 +81,7 @@ struct nbody_sim {
     int32_t debug = 0;
     int walk_block = 256, walk_stack = 128;
     int walk_variant = 1;  // 0: LDS-stack walk, 1: register-stack walk
+    int lane_bodies = 2;   // bodies per lane in the register-stack walks (1, 2, 4)
     bool xcd_remap = true;
 
     bool use_graph = false;
@@ -281,7 +282,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     w.n = (uint32_t)n;
     w.p = p;
     launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
-                s->xcd_remap, s->walk_variant, st);
+                s->xcd_remap, s->walk_variant, s->lane_bodies, st);
     mark(6);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(s, e, "kernel launch");
@@ -695,9 +696,12 @@ int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");
     if (t->walk_stack < 8 || t->walk_stack > 256)
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_stack must be in [8, 256]");
-    if (t->walk_variant < 0 || t->walk_variant > 2)
-        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0, 1 or 2");
+    if (t->walk_variant < 0 || t->walk_variant > 1)
+        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0 or 1");
+    if (t->lane_bodies != 1 && t->lane_bodies != 2 && t->lane_bodies != 4)
+        return fail(s, NBODY_ERR_INVALID_ARG, "lane_bodies must be 1, 2 or 4");
     s->walk_block = t->walk_block;
+    s->lane_bodies = t->lane_bodies;
     s->walk_stack = t->walk_stack;
     s->walk_variant = t->walk_variant;
     s->xcd_remap = t->xcd_order != 0;
@@ -711,6 +715,7 @@ int nbody_get_tuning(const nbody_sim *s, nbody_tuning *out) {
     out->walk_stack = s->walk_stack;
     out->walk_variant = s->walk_variant;
     out->xcd_order = s->xcd_remap ? 1 : 0;
+    out->lane_bodies = s->lane_bodies;
     return NBODY_OK;
 }
 
@@ -828,7 +833,7 @@ int nbody_adjoint_backward(nbody_sim *s, const float *gpos_T, const float *gvel_
             a.idx = s->vals[s->result_buf];
             launch_adjoint_pre(s->adj_lx, s->adj_lv, s->adj_la, (uint32_t)n, dt, st);
             launch_clear_u32(reinterpret_cast<uint32_t *>(s->adj_cot), 4 * (2 * n - 1), st);
-            launch_adjoint_walk(a, exact, s->walk_stack, st);
+            launch_adjoint_walk(a, exact, s->walk_stack, s->lane_bodies, st);
             launch_adjoint_fold(a, st);
         }
     }

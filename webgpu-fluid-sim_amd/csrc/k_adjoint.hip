@@ -44,52 +44,62 @@ __global__ __launch_bounds__(256) void adjoint_pre_kernel(const float2 *__restri
 }
 
 // The forward walk's traversal and opening test (walk_common.h), with the
-// VJP of the pull on accepted nodes instead of the pull.
-template <bool kExact>
+// VJP of the pull on accepted nodes instead of the pull; kB bodies per lane
+// as in the forward walk.
+template <bool kExact, int kB>
 __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, int stack_cap) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = a.n;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t < n;
+    const uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (64u * kB) + lane;
     const uint32_t leaf0 = n - 1u;
-    const uint32_t my_leaf = leaf0 + t;
 
-    float px = 0.0f, py = 0.0f, gx = 0.0f, gy = 0.0f;
-    uint32_t bi = 0;
-    if (valid) {
-        const float4 me = a.leaf[t];
-        px = me.x;
-        py = me.y;
-        bi = a.idx[t];
-        const float2 g = a.la[bi];
-        gx = g.x;
-        gy = g.y;
+    uint32_t t[kB], my_leaf[kB], bi[kB];
+    bool valid[kB];
+    float px[kB], py[kB], gx[kB], gy[kB], hx_sum[kB], hy_sum[kB];
+    uint64_t mask[kB];
+    uint64_t live = 0;
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+        t[b] = base + 64u * b;
+        valid[b] = t[b] < n;
+        my_leaf[b] = leaf0 + t[b];
+        px[b] = py[b] = gx[b] = gy[b] = hx_sum[b] = hy_sum[b] = 0.0f;
+        bi[b] = 0;
+        if (valid[b]) {
+            const float4 me = a.leaf[t[b]];
+            px[b] = me.x;
+            py[b] = me.y;
+            bi[b] = a.idx[t[b]];
+            const float2 g = a.la[bi[b]];
+            gx[b] = g.x;
+            gy[b] = g.y;
+        }
+        mask[b] = __ballot(valid[b]);
+        live |= mask[b];
     }
-    const uint64_t live = __ballot(valid);
     if (live == 0) return;
 
     const float G = a.grav, theta = a.theta, eps2 = a.eps * a.eps;
-    float hx_sum = 0.0f, hy_sum = 0.0f;
-    RegStack st;
-    walk_tree<false>(
-        a.pcm, a.child, leaf0, my_leaf, lane, 0u, live, a.root[0], st, 0, stack_cap, a.overflow,
-        [&](const float4 &c, bool leaf, uint32_t node, bool act, bool &acc, bool &open) {
-            const float rx = c.x - px, ry = c.y - py;
+    RegStack<kB> st;
+    walk_tree<kB>(
+        a.pcm, a.child, leaf0, my_leaf, lane, 0u, mask, a.root[0], st, 0, stack_cap, a.overflow,
+        [&](const float4 &c, bool leaf, uint32_t node, int b, bool act, bool &acc, bool &open) {
+            const float rx = c.x - px[b], ry = c.y - py[b];
             const float inv = inv_dist<kExact>(rx, ry, eps2);
             const bool near = !(c.w * inv < theta);
             acc = act && (leaf || !near);
             open = act && !leaf && near;
             const float inv2 = inv * inv;
             const float inv3 = inv2 * inv;
-            const float rg = rx * gx + ry * gy;
+            const float rg = rx * gx[b] + ry * gy[b];
             const float gm = G * c.z;
             const float k3 = 3.0f * inv2 * rg;
-            const float hx = acc ? gm * inv3 * (gx - k3 * rx) : 0.0f;
-            const float hy = acc ? gm * inv3 * (gy - k3 * ry) : 0.0f;
+            const float hx = acc ? gm * inv3 * (gx[b] - k3 * rx) : 0.0f;
+            const float hy = acc ? gm * inv3 * (gy[b] - k3 * ry) : 0.0f;
             const float dm = acc ? G * inv3 * rg : 0.0f;
-            hx_sum += hx;
-            hy_sum += hy;
-            // node cotangent: one atomic per wavefront and component
+            hx_sum[b] += hx;
+            hy_sum[b] += hy;
+            // node cotangent: one atomic per wavefront, slot and component
             if (__ballot(acc)) {
                 const float sx = wave_sum(hx), sy = wave_sum(hy), sm = wave_sum(dm);
                 if (lane == 0) {
@@ -99,12 +109,15 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
                     atomicAdd(&cot->z, sm);
                 }
             }
-        });
-    if (valid) {
-        float2 l = a.lx[bi];
-        l.x -= hx_sum;
-        l.y -= hy_sum;
-        a.lx[bi] = l;
+        },
+        [](int, bool) {});
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+        if (!valid[b]) continue;
+        float2 l = a.lx[bi[b]];
+        l.x -= hx_sum[b];
+        l.y -= hy_sum[b];
+        a.lx[bi[b]] = l;
     }
 }
 
@@ -147,6 +160,13 @@ __global__ __launch_bounds__(256) void adjoint_fold_kernel(const AdjointArgs a) 
     a.lm[bi] += me.x * pax + me.y * pay + pb;
 }
 
+template <bool kExact, int kB>
+void launch_adjoint_walk_t(const AdjointArgs &a, int cap, hipStream_t s) {
+    const unsigned block = 256, per_block = block * kB;
+    const unsigned grid = (a.n + per_block - 1) / per_block;
+    adjoint_walk_kernel<kExact, kB><<<grid, block, 0, s>>>(a, cap);
+}
+
 }  // namespace
 
 void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, float dt,
@@ -155,15 +175,17 @@ void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, fl
     adjoint_pre_kernel<<<(n + 255) / 256, 256, 0, s>>>(lx, lv, la, n, dt);
 }
 
-void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, hipStream_t s) {
+void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, int lane_bodies,
+                         hipStream_t s) {
     if (a.n == 0) return;
-    const unsigned block = 256;
-    const unsigned grid = (a.n + block - 1) / block;
     const int cap = stack < 128 ? stack : 128;
-    if (exact)
-        adjoint_walk_kernel<true><<<grid, block, 0, s>>>(a, cap);
-    else
-        adjoint_walk_kernel<false><<<grid, block, 0, s>>>(a, cap);
+    if (lane_bodies == 2) {
+        if (exact) launch_adjoint_walk_t<true, 2>(a, cap, s);
+        else launch_adjoint_walk_t<false, 2>(a, cap, s);
+    } else {
+        if (exact) launch_adjoint_walk_t<true, 1>(a, cap, s);
+        else launch_adjoint_walk_t<false, 1>(a, cap, s);
+    }
 }
 
 void launch_adjoint_fold(const AdjointArgs &a, hipStream_t s) {

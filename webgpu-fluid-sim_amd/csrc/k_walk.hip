@@ -288,95 +288,128 @@ __device__ __forceinline__ void lane_test(const float4 c, bool leaf, bool act, f
     ay = acc ? ny : ay;
 }
 
-template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, bool kPrefetch>
+// Register-stack walk: each lane holds kB bodies, slot b of wavefront w
+// being sorted position w * 64 * kB + 64 * b + lane, so the wavefront's
+// 64 * kB Morton-consecutive bodies share one traversal (walk_common.h).
+template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, int kB>
 __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int stack_cap,
                                                        uint32_t xcd_per) {
     const int lane = threadIdx.x & 63;
     uint32_t block = blockIdx.x;
     if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
     const uint32_t n = a.n;
-    const uint32_t t = block * blockDim.x + threadIdx.x;  // sorted position
-    const bool valid = t < n;
+    const uint32_t base = ((block * blockDim.x + threadIdx.x) >> 6) * (64u * kB) + lane;
     const uint32_t leaf0 = n - 1u;
-    const uint32_t my_leaf = leaf0 + t;
 
-    float px = 0.0f, py = 0.0f, m1 = 0.0f;
-    if (valid) {
-        const float4 me = a.leaf[t];  // written by the fill: {pos, mass, 0}
-        px = me.x;
-        py = me.y;
-        m1 = me.z;
+    uint32_t t[kB], my_leaf[kB];
+    bool valid[kB];
+    float px[kB], py[kB], m1[kB], ax[kB], ay[kB];
+    uint64_t mask[kB];
+    uint64_t live = 0;
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+        t[b] = base + 64u * b;  // sorted position
+        valid[b] = t[b] < n;
+        my_leaf[b] = leaf0 + t[b];
+        px[b] = py[b] = m1[b] = ax[b] = ay[b] = 0.0f;
+        if (valid[b]) {
+            const float4 me = a.leaf[t[b]];  // written by the fill: {pos, mass, 0}
+            px[b] = me.x;
+            py[b] = me.y;
+            m1[b] = me.z;
+        }
+        mask[b] = __ballot(valid[b]);
+        live |= mask[b];
     }
-    const uint64_t live = __ballot(valid);
     if (live == 0) return;
 
     const StepParams p = a.p;
     const float eps2c = p.eps * p.eps;
-    float ax = 0.0f, ay = 0.0f;
     uint32_t n_acc = 0, n_open = 0, n_drop = 0;
-    RegStack st;
+    RegStack<kB> st;
     int sp = 0;
     uint32_t node = 0u;
-    uint64_t mask = live;
     float4 c;
     if constexpr (kEnsemble) {
-        // Each lane starts at its sim's sub-tree root; group lanes by root.
-        uint32_t my_root = 0u;
-        if (valid) my_root = a.sim_root[(uint32_t)(a.keys64[t] >> kSimShift)];
-        uint64_t rem = live;
+        // Each body starts at its sim's sub-tree root; one entry per distinct
+        // root, with the slot masks of the bodies under it.
+        uint32_t my_root[kB];
+        uint64_t rem[kB];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            my_root[b] = valid[b] ? a.sim_root[(uint32_t)(a.keys64[t[b]] >> kSimShift)] : 0u;
+            rem[b] = mask[b];
+        }
         bool first = true;
-        while (rem) {
-            const int l0 = __builtin_ctzll(rem);
-            const uint32_t r = uniform_u32((uint32_t)__builtin_amdgcn_readlane((int)my_root, l0));
-            const uint64_t m = __ballot(valid && my_root == r) & rem;
-            rem &= ~m;
-            const float4 rc = node_record(a, r);
-            if (first) {
-                node = r;
-                mask = m;
-                c = rc;
-                first = false;
-            } else if (sp < stack_cap) {
-                st.put(sp >> 6, sp & 63, r, m, rc);
-                ++sp;
-            } else if (lane == 0) {
-                atomicOr(a.overflow, 1u);
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            while (rem[b]) {
+                const int l0 = __builtin_ctzll(rem[b]);
+                const uint32_t r =
+                    uniform_u32((uint32_t)__builtin_amdgcn_readlane((int)my_root[b], l0));
+                uint64_t m[kB];
+#pragma unroll
+                for (int b2 = 0; b2 < kB; ++b2) {
+                    m[b2] = __ballot(valid[b2] && my_root[b2] == r) & rem[b2];
+                    rem[b2] &= ~m[b2];
+                }
+                const float4 rc = node_record(a, r);
+                if (first) {
+                    node = r;
+#pragma unroll
+                    for (int b2 = 0; b2 < kB; ++b2) mask[b2] = m[b2];
+                    c = rc;
+                    first = false;
+                } else if (sp < stack_cap) {
+                    st.put(sp >> 6, sp & 63, r, m, rc);
+                    ++sp;
+                } else if (lane == 0) {
+                    atomicOr(a.overflow, 1u);
+                }
             }
         }
     } else {
         c = a.root[0];
     }
 
-    walk_tree<kPrefetch>(a.pcm, a.child, leaf0, my_leaf, lane, node, mask, c, st, sp, stack_cap,
-                         a.overflow,
-                         [&](const float4 &cc, bool leaf, uint32_t, bool act, bool &acc, bool &open) {
-                             lane_test<kExact, kRefSoft>(cc, leaf, act, px, py, m1, p, eps2c, ax,
-                                                         ay, acc, open);
-                             if constexpr (kStats) {
-                                 n_acc += acc ? 1u : 0u;
-                                 n_open += open ? 1u : 0u;
-                             }
-                         });
+    walk_tree<kB>(
+        a.pcm, a.child, leaf0, my_leaf, lane, node, mask, c, st, sp, stack_cap, a.overflow,
+        [&](const float4 &cc, bool leaf, uint32_t, int b, bool act, bool &acc, bool &open) {
+            lane_test<kExact, kRefSoft>(cc, leaf, act, px[b], py[b], m1[b], p, eps2c, ax[b], ay[b],
+                                        acc, open);
+            if constexpr (kStats) {
+                n_acc += acc ? 1u : 0u;
+                n_open += open ? 1u : 0u;
+            }
+        },
+        [&](int, bool dropped) {
+            if constexpr (kStats) {
+                n_open -= dropped ? 1u : 0u;
+                n_drop += dropped ? 1u : 0u;
+            }
+        });
 
-    if (valid) {
-        const uint32_t bi = a.idx[t];
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+        if (!valid[b]) continue;
+        const uint32_t bi = a.idx[t[b]];
         const float dt = p.dt;
         float2 v = a.vel[bi];
         float2 x;
         if constexpr (kExact) {
-            v.x = v.x + ax * dt;  // barnes_hut.wgsl:110
-            v.y = v.y + ay * dt;
-            x.x = px + v.x * dt;  // barnes_hut.wgsl:120
-            x.y = py + v.y * dt;
+            v.x = v.x + ax[b] * dt;  // barnes_hut.wgsl:110
+            v.y = v.y + ay[b] * dt;
+            x.x = px[b] + v.x * dt;  // barnes_hut.wgsl:120
+            x.y = py[b] + v.y * dt;
         } else {
-            v.x = __builtin_fmaf(ax, dt, v.x);
-            v.y = __builtin_fmaf(ay, dt, v.y);
-            x.x = __builtin_fmaf(v.x, dt, px);
-            x.y = __builtin_fmaf(v.y, dt, py);
+            v.x = __builtin_fmaf(ax[b], dt, v.x);
+            v.y = __builtin_fmaf(ay[b], dt, v.y);
+            x.x = __builtin_fmaf(v.x, dt, px[b]);
+            x.y = __builtin_fmaf(v.y, dt, py[b]);
         }
         a.vel[bi] = v;
         a.pos[bi] = x;
-        if (a.accel) a.accel[bi] = make_float2(ax, ay);
+        if (a.accel) a.accel[bi] = make_float2(ax[b], ay[b]);
     }
     if constexpr (kStats) {
         uint32_t s0 = n_acc, s1 = n_open, s2 = n_drop;
@@ -393,23 +426,36 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     }
 }
 
-template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
-void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, hipStream_t s) {
-    uint32_t grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
+template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, int kB>
+void launch_rs(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s) {
+    const uint32_t per_block = (uint32_t)block * kB;
+    uint32_t grid = (a.n + per_block - 1u) / per_block;
     uint32_t per = 0;
     if (xcd && grid >= 16) {
         per = (grid + 7u) / 8u;
         grid = per * 8u;  // blocks past the bodies exit at once
     }
+    walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats, kB>
+        <<<grid, block, 0, s>>>(a, stack < 128 ? stack : 128, per);
+}
+
+template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
+void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, int lane_bodies,
+              hipStream_t s) {
     if (variant == 1) {
-        walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats, false><<<grid, block, 0, s>>>(
-            a, stack < 128 ? stack : 128, per);
+        if (lane_bodies == 4)
+            launch_rs<kExact, kRefSoft, kEnsemble, kStats, 4>(a, block, stack, xcd, s);
+        else if (lane_bodies == 2)
+            launch_rs<kExact, kRefSoft, kEnsemble, kStats, 2>(a, block, stack, xcd, s);
+        else
+            launch_rs<kExact, kRefSoft, kEnsemble, kStats, 1>(a, block, stack, xcd, s);
         return;
     }
-    if (variant == 2) {
-        walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats, true><<<grid, block, 0, s>>>(
-            a, stack < 128 ? stack : 128, per);
-        return;
+    uint32_t grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
+    uint32_t per = 0;
+    if (xcd && grid >= 16) {
+        per = (grid + 7u) / 8u;
+        grid = per * 8u;  // blocks past the bodies exit at once
     }
     const size_t lds = (size_t)(block / kWave) * (size_t)stack *
                        (sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t));
@@ -418,37 +464,37 @@ void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, hi
 
 template <bool kExact, bool kRefSoft>
 void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, bool xcd,
-              int variant, hipStream_t s) {
+              int variant, int lb, hipStream_t s) {
     if (ens) {
         if (stats)
-            launch_t<kExact, kRefSoft, true, true>(a, block, stack, xcd, variant, s);
+            launch_t<kExact, kRefSoft, true, true>(a, block, stack, xcd, variant, lb, s);
         else
-            launch_t<kExact, kRefSoft, true, false>(a, block, stack, xcd, variant, s);
+            launch_t<kExact, kRefSoft, true, false>(a, block, stack, xcd, variant, lb, s);
     } else {
         if (stats)
-            launch_t<kExact, kRefSoft, false, true>(a, block, stack, xcd, variant, s);
+            launch_t<kExact, kRefSoft, false, true>(a, block, stack, xcd, variant, lb, s);
         else
-            launch_t<kExact, kRefSoft, false, false>(a, block, stack, xcd, variant, s);
+            launch_t<kExact, kRefSoft, false, false>(a, block, stack, xcd, variant, lb, s);
     }
 }
 
 }  // namespace
 
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 bool xcd_remap, int variant, hipStream_t s) {
+                 bool xcd_remap, int variant, int lane_bodies, hipStream_t s) {
     if (a.n == 0) return;
     const bool exact = a.p.math_mode != 0;
     const bool refsoft = a.p.soft_mode == 0;
     if (exact) {
         if (refsoft)
-            launch_e<true, true>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
+            launch_e<true, true>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
         else
-            launch_e<true, false>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
+            launch_e<true, false>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
     } else {
         if (refsoft)
-            launch_e<false, true>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
+            launch_e<false, true>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
         else
-            launch_e<false, false>(a, ensemble, stats, block, stack, xcd_remap, variant, s);
+            launch_e<false, false>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
     }
 }
 

@@ -99,7 +99,7 @@ struct WalkArgs {
     StepParams p;
 };
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 bool xcd_remap, int variant, hipStream_t s);
+                 bool xcd_remap, int variant, int lane_bodies, hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint
 struct AdjointArgs {
@@ -119,7 +119,8 @@ struct AdjointArgs {
 };
 void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, float dt,
                         hipStream_t s);
-void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, hipStream_t s);
+void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, int lane_bodies,
+                         hipStream_t s);
 void launch_adjoint_fold(const AdjointArgs &a, hipStream_t s);
 
 // ---------------------------------------------------------------- misc

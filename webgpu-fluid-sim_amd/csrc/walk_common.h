@@ -1,10 +1,11 @@
 // walk_common.h -- the wavefront-coherent tree traversal shared by the
 // forward walk (k_walk.hip) and the adjoint walk (k_adjoint.hip).
 //
-// The 64 lanes of a wavefront hold 64 consecutive Morton-ordered bodies and
-// walk one depth-first traversal together.  A traversal entry is (node, lane
-// mask, node record); every lane in the mask applies the opening test of its
-// own body, and the lanes that open a node form the mask of its children.
+// The lanes of a wavefront hold 64 * kB consecutive Morton-ordered bodies
+// (kB per lane) and walk one depth-first traversal together.  A traversal
+// entry is (node, node record, one lane mask per body slot); every body in
+// a mask applies its own opening test, and the bodies that open a node form
+// the masks of its children.
 // Right children are visited before left ones, as the reference's stack pops
 // them (barnes_hut.wgsl:100-105), so every lane meets exactly the nodes of its
 // private reference walk, in the same order.
@@ -36,41 +37,47 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
 // exec-mask change and no readfirstlane.  A push writes lane k of each field
 // with v_writelane_b32 (the backend puts the lane select in M0 as GFX9
 // requires), a pop reads it back with v_readlane_b32.  No memory is accessed.
+// An entry is (node, node record, one lane mask per body slot).
+template <int kB>
 struct RegStack {
-    int node[2], mlo[2], mhi[2], cx[2], cy[2], cm[2], cw[2];
+    int node[2], cx[2], cy[2], cm[2], cw[2];
+    int mlo[kB][2], mhi[kB][2];
 
-    __device__ __forceinline__ void put(int row, int lane_sel, uint32_t nd, uint64_t m,
-                                        const float4 &c) {
-        const int l = lane_sel;
-        if (row == 0) {
-            node[0] = nbs_writelane((int)nd, l, node[0]);
-            mlo[0] = nbs_writelane((int)(uint32_t)m, l, mlo[0]);
-            mhi[0] = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi[0]);
-            cx[0] = nbs_writelane(__float_as_int(c.x), l, cx[0]);
-            cy[0] = nbs_writelane(__float_as_int(c.y), l, cy[0]);
-            cm[0] = nbs_writelane(__float_as_int(c.z), l, cm[0]);
-            cw[0] = nbs_writelane(__float_as_int(c.w), l, cw[0]);
-        } else {
-            node[1] = nbs_writelane((int)nd, l, node[1]);
-            mlo[1] = nbs_writelane((int)(uint32_t)m, l, mlo[1]);
-            mhi[1] = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi[1]);
-            cx[1] = nbs_writelane(__float_as_int(c.x), l, cx[1]);
-            cy[1] = nbs_writelane(__float_as_int(c.y), l, cy[1]);
-            cm[1] = nbs_writelane(__float_as_int(c.z), l, cm[1]);
-            cw[1] = nbs_writelane(__float_as_int(c.w), l, cw[1]);
+    template <int r>
+    __device__ __forceinline__ void put_row(int l, uint32_t nd, const uint64_t (&m)[kB],
+                                            const float4 &c) {
+        node[r] = nbs_writelane((int)nd, l, node[r]);
+        cx[r] = nbs_writelane(__float_as_int(c.x), l, cx[r]);
+        cy[r] = nbs_writelane(__float_as_int(c.y), l, cy[r]);
+        cm[r] = nbs_writelane(__float_as_int(c.z), l, cm[r]);
+        cw[r] = nbs_writelane(__float_as_int(c.w), l, cw[r]);
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            mlo[b][r] = nbs_writelane((int)(uint32_t)m[b], l, mlo[b][r]);
+            mhi[b][r] = nbs_writelane((int)(uint32_t)(m[b] >> 32), l, mhi[b][r]);
         }
     }
-    __device__ __forceinline__ void get(int row, int lane_sel, uint32_t &nd, uint64_t &m,
+    __device__ __forceinline__ void put(int row, int l, uint32_t nd, const uint64_t (&m)[kB],
+                                        const float4 &c) {
+        if (row == 0)
+            put_row<0>(l, nd, m, c);
+        else
+            put_row<1>(l, nd, m, c);
+    }
+    __device__ __forceinline__ void get(int row, int l, uint32_t &nd, uint64_t (&m)[kB],
                                         float4 &c) const {
-        const int r = row;
-        nd = (uint32_t)__builtin_amdgcn_readlane(r ? node[1] : node[0], lane_sel);
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(r ? mlo[1] : mlo[0], lane_sel);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(r ? mhi[1] : mhi[0], lane_sel);
-        m = ((uint64_t)hi << 32) | lo;
-        c.x = __int_as_float(__builtin_amdgcn_readlane(r ? cx[1] : cx[0], lane_sel));
-        c.y = __int_as_float(__builtin_amdgcn_readlane(r ? cy[1] : cy[0], lane_sel));
-        c.z = __int_as_float(__builtin_amdgcn_readlane(r ? cm[1] : cm[0], lane_sel));
-        c.w = __int_as_float(__builtin_amdgcn_readlane(r ? cw[1] : cw[0], lane_sel));
+        const bool r = row != 0;
+        nd = (uint32_t)__builtin_amdgcn_readlane(r ? node[1] : node[0], l);
+        c.x = __int_as_float(__builtin_amdgcn_readlane(r ? cx[1] : cx[0], l));
+        c.y = __int_as_float(__builtin_amdgcn_readlane(r ? cy[1] : cy[0], l));
+        c.z = __int_as_float(__builtin_amdgcn_readlane(r ? cm[1] : cm[0], l));
+        c.w = __int_as_float(__builtin_amdgcn_readlane(r ? cw[1] : cw[0], l));
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(r ? mlo[b][1] : mlo[b][0], l);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(r ? mhi[b][1] : mhi[b][0], l);
+            m[b] = ((uint64_t)hi << 32) | lo;
+        }
     }
 };
 
@@ -88,83 +95,81 @@ __device__ __forceinline__ float inv_dist(float rx, float ry, float eps2) {
 }
 
 // Depth-first traversal of one wavefront from (node, mask, c) with `sp`
-// entries already on `st`.  visit(c, leaf, node, act, acc, open) is called
-// by the whole wavefront for every visited node (act: this lane evaluates it;
-// the visitor sets acc/open per lane).  Stops when the stack is empty.
-// A full stack drops both children like barnes_hut.wgsl:100-105 and raises
-// *overflow.  A wavefront visits a node at most once, so a valid tree ends
-// within 2 * leaf0 + 1 iterations; the bound only guards against a corrupt
-// tree and reports like an overflow.
-template <bool kPrefetch, typename Visit>
+// entries already on `st`.  Each lane holds kB bodies (slot b: my_leaf[b],
+// lane mask mask[b]); the wavefront's bodies share one traversal.
+// visit(c, leaf, node, b, act, acc, open) is called by the whole wavefront
+// for every visited node and slot (act: the slot's body evaluates it; the
+// visitor sets acc/open).  Stops when the stack is empty.  A full stack
+// drops both children like barnes_hut.wgsl:100-105, raises *overflow and
+// calls drop(b, lane_dropped) per slot.  A wavefront visits a node at most
+// once, so a tree from the build kernels ends within 2 * leaf0 + 1
+// iterations.  There is no iteration counter: its compare-and-exit costs
+// 12-19% of the walk time at 800k bodies.
+template <int kB, typename Visit, typename Drop>
 __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child, uint32_t leaf0,
-                                          uint32_t my_leaf, int lane, uint32_t node,
-                                          uint64_t mask, float4 c, RegStack &st, int sp,
-                                          int stack_cap, uint32_t *overflow, Visit &&visit) {
+                                          const uint32_t (&my_leaf)[kB], int lane, uint32_t node,
+                                          uint64_t (&mask)[kB], float4 c, RegStack<kB> &st,
+                                          int sp, int stack_cap, uint32_t *overflow,
+                                          Visit &&visit, Drop &&drop) {
     const_as<uint32_t> *child_s = (const_as<uint32_t> *)child;
     const_as<float> *pcmf_s = (const_as<float> *)pcm;
-    for (uint32_t budget = 2u * leaf0 + 2u;; --budget) {
-        if (budget == 0u) {
-            if (lane == 0) atomicOr(overflow, 1u);
-            break;
-        }
+    for (;;) {
         const bool leaf = node >= leaf0;
-        // Both children's records come in one scalar load (used if a lane
-        // opens the node).  With kPrefetch the load is issued before the test
-        // for every internal node so its latency overlaps the lane arithmetic;
-        // a leaf reads pair 0, which is allocated for every n.
-        const uint32_t pn = (kPrefetch && leaf) ? 0u : node;
-        float4 cl, cr;
-        uint32_t chl = 0, chr = 0;
-        if constexpr (kPrefetch) {
-            cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
-                             pcmf_s[8 * pn + 3]);
-            cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5], pcmf_s[8 * pn + 6],
-                             pcmf_s[8 * pn + 7]);
-            chl = child_s[2 * pn];
-            chr = child_s[2 * pn + 1];
+        uint64_t om[kB];
+        uint64_t any = 0;
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            const bool act =
+                __builtin_amdgcn_inverse_ballot_w64(mask[b]) && !(leaf && node == my_leaf[b]);
+            bool acc, open;
+            visit(c, leaf, node, b, act, acc, open);
+            om[b] = __ballot(open);
+            any |= om[b];
         }
-        const bool act = __builtin_amdgcn_inverse_ballot_w64(mask) && !(leaf && node == my_leaf);
-        bool acc, open;
-        visit(c, leaf, node, act, acc, open);
-        if constexpr (kPrefetch) {
-            // keeps the loads above the test: without a use here the compiler
-            // sinks them into the branch below, after the vote
-            asm volatile("" ::"s"(cl.x), "s"(cl.w), "s"(cr.x), "s"(cr.w), "s"(chl), "s"(chr));
-        }
-        const uint64_t om = __ballot(open);
-        if (om) {
-            if constexpr (!kPrefetch) {
-                cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
-                                 pcmf_s[8 * pn + 3]);
-                cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5], pcmf_s[8 * pn + 6],
-                                 pcmf_s[8 * pn + 7]);
-                chl = child_s[2 * pn];
-                chr = child_s[2 * pn + 1];
-            }
-            const bool in = __builtin_amdgcn_inverse_ballot_w64(om);
+        if (any) {
+            // both children's records in one scalar load
+            const uint32_t pn = node;
+            const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
+                                          pcmf_s[8 * pn + 3]);
+            const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5],
+                                          pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
+            const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
             if (chr >= leaf0) {
                 // A leaf child has no subtree: its lanes take it now, at the
                 // same place in the order.
-                bool a2, o2;
-                visit(cr, true, chr, in && chr != my_leaf, a2, o2);
+#pragma unroll
+                for (int b = 0; b < kB; ++b) {
+                    bool a2, o2;
+                    visit(cr, true, chr, b,
+                          __builtin_amdgcn_inverse_ballot_w64(om[b]) && chr != my_leaf[b], a2, o2);
+                }
                 if (chl < leaf0) {
                     node = chl;
-                    mask = om;
+#pragma unroll
+                    for (int b = 0; b < kB; ++b) mask[b] = om[b];
                     c = cl;
                     continue;
                 }
-                visit(cl, true, chl, in && chl != my_leaf, a2, o2);
+#pragma unroll
+                for (int b = 0; b < kB; ++b) {
+                    bool a2, o2;
+                    visit(cl, true, chl, b,
+                          __builtin_amdgcn_inverse_ballot_w64(om[b]) && chl != my_leaf[b], a2, o2);
+                }
                 // both children done: fall through to the pop
             } else if (sp < stack_cap) {
                 // push left, go right (the reference pops the right child first)
                 st.put(sp >> 6, sp & 63, chl, om, cl);
                 ++sp;
                 node = chr;
-                mask = om;
+#pragma unroll
+                for (int b = 0; b < kB; ++b) mask[b] = om[b];
                 c = cr;
                 continue;
-            } else if (lane == 0) {
-                atomicOr(overflow, 1u);
+            } else {
+                if (lane == 0) atomicOr(overflow, 1u);
+#pragma unroll
+                for (int b = 0; b < kB; ++b) drop(b, __builtin_amdgcn_inverse_ballot_w64(om[b]));
             }
         }
         if (sp == 0) break;

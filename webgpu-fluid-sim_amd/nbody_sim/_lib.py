@@ -84,6 +84,7 @@ class NbodyTuning(ctypes.Structure):
         ("walk_stack", ctypes.c_int32),
         ("walk_variant", ctypes.c_int32),
         ("xcd_order", ctypes.c_int32),
+        ("lane_bodies", ctypes.c_int32),
     ]
 
 

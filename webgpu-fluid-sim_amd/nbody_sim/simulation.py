@@ -117,11 +117,13 @@ class _Handle:
         self._check(_lib.lib().nbody_get_tuning(self.handle, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in _lib.NbodyTuning._fields_}
 
-    def setTuning(self, walk_block=None, walk_stack=None, walk_variant=None, xcd_order=None):
+    def setTuning(self, walk_block=None, walk_stack=None, walk_variant=None, xcd_order=None,
+                  lane_bodies=None):
         """Launch tunables (the results are the same for every setting); None keeps."""
         cur = self.getTuning()
         for k, v in (("walk_block", walk_block), ("walk_stack", walk_stack),
-                     ("walk_variant", walk_variant), ("xcd_order", xcd_order)):
+                     ("walk_variant", walk_variant), ("xcd_order", xcd_order),
+                     ("lane_bodies", lane_bodies)):
             if v is not None:
                 cur[k] = int(v)
         t = _lib.NbodyTuning(**cur)
