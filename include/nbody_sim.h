@@ -150,9 +150,9 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *                holds at most 128)
  *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default)
  *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
- *                of Morton-ordered bodies (default); 0: hardware order
+ *                of Morton-ordered bodies; 0: hardware order (default)
  *   lane_bodies  bodies per lane of the VGPR-stack walk and the adjoint walk:
- *                1, 2 (default) or 4; a wavefront walks 64 * lane_bodies
+ *                1 (default), 2 or 4; a wavefront walks 64 * lane_bodies
  *                Morton-consecutive bodies together (the LDS walk ignores it,
  *                walking one body per lane) */
 typedef struct nbody_tuning {

@@ -81,8 +81,8 @@ struct nbody_sim {
     int32_t debug = 0;
     int walk_block = 256, walk_stack = 128;
     int walk_variant = 1;  // 0: LDS-stack walk, 1: register-stack walk
-    int lane_bodies = 2;   // bodies per lane in the register-stack walks (1, 2, 4)
-    bool xcd_remap = true;
+    int lane_bodies = 1;   // bodies per lane in the register-stack walks (1, 2, 4)
+    bool xcd_remap = false;
 
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;

@@ -50,19 +50,18 @@ template <bool kExact, int kB>
 __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, int stack_cap) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = a.n;
-    const uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (64u * kB) + lane;
+    const uint32_t first = uniform_u32(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (64u * kB));
     const uint32_t leaf0 = n - 1u;
 
-    uint32_t t[kB], my_leaf[kB], bi[kB];
+    uint32_t t[kB], bi[kB];
     bool valid[kB];
     float px[kB], py[kB], gx[kB], gy[kB], hx_sum[kB], hy_sum[kB];
     uint64_t mask[kB];
     uint64_t live = 0;
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
-        t[b] = base + 64u * b;
+        t[b] = first + 64u * b + lane;
         valid[b] = t[b] < n;
-        my_leaf[b] = leaf0 + t[b];
         px[b] = py[b] = gx[b] = gy[b] = hx_sum[b] = hy_sum[b] = 0.0f;
         bi[b] = 0;
         if (valid[b]) {
@@ -81,36 +80,38 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
 
     const float G = a.grav, theta = a.theta, eps2 = a.eps * a.eps;
     RegStack<kB> st;
+    float rx[kB], ry[kB], inv[kB];
     walk_tree<kB>(
-        a.pcm, a.child, leaf0, my_leaf, lane, 0u, mask, a.root[0], st, 0, stack_cap, a.overflow,
-        [&](const float4 &c, bool leaf, uint32_t node, int b, bool act, bool &acc, bool &open) {
-            const float rx = c.x - px[b], ry = c.y - py[b];
-            const float inv = inv_dist<kExact>(rx, ry, eps2);
-            const bool near = !(c.w * inv < theta);
-            acc = act && (leaf || !near);
-            open = act && !leaf && near;
-            const float inv2 = inv * inv;
-            const float inv3 = inv2 * inv;
-            const float rg = rx * gx[b] + ry * gy[b];
+        a.pcm, a.child, leaf0, leaf0 + first, lane, 0u, mask, a.root[0], st, 0, stack_cap,
+        a.overflow,
+        [&](const float4 &c, int b) {
+            rx[b] = c.x - px[b];
+            ry[b] = c.y - py[b];
+            inv[b] = inv_dist<kExact>(rx[b], ry[b], eps2);
+            return c.w * inv[b] < theta;
+        },
+        [&](const float4 &c, uint32_t node, int b, uint64_t acc_mask) {
+            const bool acc = __builtin_amdgcn_inverse_ballot_w64(acc_mask);
+            const float inv2 = inv[b] * inv[b];
+            const float inv3 = inv2 * inv[b];
+            const float rg = rx[b] * gx[b] + ry[b] * gy[b];
             const float gm = G * c.z;
             const float k3 = 3.0f * inv2 * rg;
-            const float hx = acc ? gm * inv3 * (gx[b] - k3 * rx) : 0.0f;
-            const float hy = acc ? gm * inv3 * (gy[b] - k3 * ry) : 0.0f;
+            const float hx = acc ? gm * inv3 * (gx[b] - k3 * rx[b]) : 0.0f;
+            const float hy = acc ? gm * inv3 * (gy[b] - k3 * ry[b]) : 0.0f;
             const float dm = acc ? G * inv3 * rg : 0.0f;
             hx_sum[b] += hx;
             hy_sum[b] += hy;
             // node cotangent: one atomic per wavefront, slot and component
-            if (__ballot(acc)) {
-                const float sx = wave_sum(hx), sy = wave_sum(hy), sm = wave_sum(dm);
-                if (lane == 0) {
-                    float4 *cot = a.cot + node;
-                    atomicAdd(&cot->x, sx);
-                    atomicAdd(&cot->y, sy);
-                    atomicAdd(&cot->z, sm);
-                }
+            const float sx = wave_sum(hx), sy = wave_sum(hy), sm = wave_sum(dm);
+            if (lane == 0) {
+                float4 *cot = a.cot + node;
+                atomicAdd(&cot->x, sx);
+                atomicAdd(&cot->y, sy);
+                atomicAdd(&cot->z, sm);
             }
         },
-        [](int, bool) {});
+        [](int, uint64_t, uint64_t, uint64_t) {});
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
         if (!valid[b]) continue;

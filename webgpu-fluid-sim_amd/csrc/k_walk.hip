@@ -30,45 +30,83 @@ namespace nbs {
 
 namespace {
 
-// One interaction test of body (px, py, m1) against node record c.
-// Returns true when the lane must open the node; otherwise adds the
-// node's pull to (ax, ay) (barnes_hut.wgsl:78-97).
+// Softening^2 of the reference's per-pair mode: eps = multiplier *
+// sqrt(G * max(m1, m2) * dt) (barnes_hut.wgsl:84-86).
+__device__ __forceinline__ float soft_eps2(float m1, float m2, const StepParams &p) {
+    const float m_eff = fmaxf(m1, m2);
+    const float eps = p.eps_mult * sqrtf(p.grav * m_eff * p.dt);
+    return eps * eps;
+}
+
+// Adds the pull of node record c on a body at offset (rx, ry) from it,
+// inv = 1/sqrt(|r|^2 + eps^2).  Exact mode: the reference's expression
+// a + (G m) r inv^3 with every operation rounded as written
+// (barnes_hut.wgsl:91-94).  Fast mode: a + m r inv^3 with fused
+// multiply-adds; the caller applies G once to the sum (accel_out).
+template <bool kExact>
+__device__ __forceinline__ void add_pull(const float4 &c, float rx, float ry, float inv, float G,
+                                         float &ax, float &ay) {
+    if constexpr (kExact) {
+        const float inv3 = inv * inv * inv;
+        const float gm = G * c.z;
+        ax = ax + gm * rx * inv3;
+        ay = ay + gm * ry * inv3;
+    } else {
+        const float f = c.z * (inv * inv * inv);
+        ax = __builtin_fmaf(f, rx, ax);
+        ay = __builtin_fmaf(f, ry, ay);
+    }
+}
+
+// The body's acceleration from the walk's sum (see add_pull).
+template <bool kExact>
+__device__ __forceinline__ float accel_out(float sum, float G) {
+    if constexpr (kExact)
+        return sum;
+    else
+        return G * sum;
+}
+
+// Kick and drift of the body at sorted position t with acceleration
+// (ax, ay) (barnes_hut.wgsl:110, 120), written by body index.
+template <bool kExact>
+__device__ __forceinline__ void kick_drift(const WalkArgs &a, uint32_t t, float px, float py,
+                                           float ax, float ay) {
+    const uint32_t bi = a.idx[t];
+    const float dt = a.p.dt;
+    float2 v = a.vel[bi];
+    float2 x;
+    if constexpr (kExact) {
+        v.x = v.x + ax * dt;  // barnes_hut.wgsl:110
+        v.y = v.y + ay * dt;
+        x.x = px + v.x * dt;  // barnes_hut.wgsl:120
+        x.y = py + v.y * dt;
+    } else {
+        v.x = __builtin_fmaf(ax, dt, v.x);
+        v.y = __builtin_fmaf(ay, dt, v.y);
+        x.x = __builtin_fmaf(v.x, dt, px);
+        x.y = __builtin_fmaf(v.y, dt, py);
+    }
+    a.vel[bi] = v;
+    a.pos[bi] = x;
+    if (a.accel) a.accel[bi] = make_float2(ax, ay);
+}
+
+// One interaction of body (px, py, m1) with node record c.  Returns true
+// when the body must open the node; otherwise adds the node's pull to
+// (ax, ay) (barnes_hut.wgsl:78-97).
 template <bool kExact, bool kRefSoft>
 __device__ __forceinline__ bool interact(const float4 c, bool leaf, float px, float py, float m1,
                                          const StepParams &p, float eps2c, float &ax, float &ay) {
     const float rx = c.x - px;
     const float ry = c.y - py;
-    float eps2;
-    if constexpr (kRefSoft) {
-        const float m_eff = fmaxf(m1, c.z);
-        const float eps = p.eps_mult * sqrtf(p.grav * m_eff * p.dt);
-        eps2 = eps * eps;
-    } else {
-        eps2 = eps2c;
+    const float eps2 = kRefSoft ? soft_eps2(m1, c.z, p) : eps2c;
+    const float inv = inv_dist<kExact>(rx, ry, eps2);
+    if (leaf || c.w * inv < p.theta) {
+        add_pull<kExact>(c, rx, ry, inv, p.grav, ax, ay);
+        return false;
     }
-    if constexpr (kExact) {
-        // dot(r, r), + eps^2, 1/sqrt: each one correctly rounded binary32 op.
-        const float d2 = rx * rx + ry * ry;
-        const float inv = 1.0f / sqrtf(d2 + eps2);
-        if (leaf || c.w * inv < p.theta) {
-            const float inv3 = inv * inv * inv;
-            const float gm = p.grav * c.z;
-            ax = ax + gm * rx * inv3;
-            ay = ay + gm * ry * inv3;
-            return false;
-        }
-        return true;
-    } else {
-        const float s = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2));
-        const float inv = __builtin_amdgcn_rsqf(s);
-        if (leaf || c.w * inv < p.theta) {
-            const float f = p.grav * c.z * (inv * inv * inv);
-            ax = __builtin_fmaf(f, rx, ax);
-            ay = __builtin_fmaf(f, ry, ay);
-            return false;
-        }
-        return true;
-    }
+    return true;
 }
 
 // Record of node `node` for the walk start: the root record, or the child
@@ -215,26 +253,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
         c = s_data[sp];
     }
 
-    if (valid) {
-        const uint32_t bi = a.idx[t];
-        const float dt = p.dt;
-        float2 v = a.vel[bi];
-        float2 x;
-        if constexpr (kExact) {
-            v.x = v.x + ax * dt;  // barnes_hut.wgsl:110
-            v.y = v.y + ay * dt;
-            x.x = px + v.x * dt;  // barnes_hut.wgsl:120
-            x.y = py + v.y * dt;
-        } else {
-            v.x = __builtin_fmaf(ax, dt, v.x);
-            v.y = __builtin_fmaf(ay, dt, v.y);
-            x.x = __builtin_fmaf(v.x, dt, px);
-            x.y = __builtin_fmaf(v.y, dt, py);
-        }
-        a.vel[bi] = v;
-        a.pos[bi] = x;
-        if (a.accel) a.accel[bi] = make_float2(ax, ay);
-    }
+    if (valid)
+        kick_drift<kExact>(a, t, px, py, accel_out<kExact>(ax, p.grav),
+                           accel_out<kExact>(ay, p.grav));
     if constexpr (kStats) {
         // wave sums, one atomic per wave and counter
         uint32_t s0 = n_acc, s1 = n_open, s2 = n_drop;
@@ -251,43 +272,6 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
     }
 }
 
-// Test of one lane against node record c, computed for every lane (the
-// caller selects with `act`), so the wavefront runs it without branches.
-template <bool kExact, bool kRefSoft>
-__device__ __forceinline__ void lane_test(const float4 c, bool leaf, bool act, float px, float py,
-                                          float m1, const StepParams &p, float eps2c, float &ax,
-                                          float &ay, bool &acc, bool &open) {
-    const float rx = c.x - px;
-    const float ry = c.y - py;
-    float eps2;
-    if constexpr (kRefSoft) {
-        const float m_eff = fmaxf(m1, c.z);
-        const float eps = p.eps_mult * sqrtf(p.grav * m_eff * p.dt);
-        eps2 = eps * eps;
-    } else {
-        eps2 = eps2c;
-    }
-    float inv, nx, ny;
-    if constexpr (kExact) {
-        const float d2 = rx * rx + ry * ry;
-        inv = 1.0f / sqrtf(d2 + eps2);
-        const float inv3 = inv * inv * inv;
-        const float gm = p.grav * c.z;
-        nx = ax + gm * rx * inv3;
-        ny = ay + gm * ry * inv3;
-    } else {
-        inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2)));
-        const float f = p.grav * c.z * (inv * inv * inv);
-        nx = __builtin_fmaf(f, rx, ax);
-        ny = __builtin_fmaf(f, ry, ay);
-    }
-    const bool near = !(c.w * inv < p.theta);
-    acc = act && (leaf || !near);
-    open = act && !leaf && near;
-    ax = acc ? nx : ax;
-    ay = acc ? ny : ay;
-}
-
 // Register-stack walk: each lane holds kB bodies, slot b of wavefront w
 // being sorted position w * 64 * kB + 64 * b + lane, so the wavefront's
 // 64 * kB Morton-consecutive bodies share one traversal (walk_common.h).
@@ -298,19 +282,19 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     uint32_t block = blockIdx.x;
     if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
     const uint32_t n = a.n;
-    const uint32_t base = ((block * blockDim.x + threadIdx.x) >> 6) * (64u * kB) + lane;
+    const uint32_t first = uniform_u32(((block * blockDim.x + threadIdx.x) >> 6) * (64u * kB));
     const uint32_t leaf0 = n - 1u;
+    const uint32_t first_leaf = leaf0 + first;  // leaf of slot 0, lane 0
 
-    uint32_t t[kB], my_leaf[kB];
+    uint32_t t[kB];
     bool valid[kB];
     float px[kB], py[kB], m1[kB], ax[kB], ay[kB];
     uint64_t mask[kB];
     uint64_t live = 0;
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
-        t[b] = base + 64u * b;  // sorted position
+        t[b] = first + 64u * b + lane;  // sorted position
         valid[b] = t[b] < n;
-        my_leaf[b] = leaf0 + t[b];
         px[b] = py[b] = m1[b] = ax[b] = ay[b] = 0.0f;
         if (valid[b]) {
             const float4 me = a.leaf[t[b]];  // written by the fill: {pos, mass, 0}
@@ -325,7 +309,6 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
 
     const StepParams p = a.p;
     const float eps2c = p.eps * p.eps;
-    uint32_t n_acc = 0, n_open = 0, n_drop = 0;
     RegStack<kB> st;
     int sp = 0;
     uint32_t node = 0u;
@@ -372,56 +355,46 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
         c = a.root[0];
     }
 
+    // per-slot geometry of the node being visited, kept by probe for pull
+    float rx[kB], ry[kB], inv[kB];
+    uint64_t n_acc = 0, n_open = 0, n_drop = 0;
     walk_tree<kB>(
-        a.pcm, a.child, leaf0, my_leaf, lane, node, mask, c, st, sp, stack_cap, a.overflow,
-        [&](const float4 &cc, bool leaf, uint32_t, int b, bool act, bool &acc, bool &open) {
-            lane_test<kExact, kRefSoft>(cc, leaf, act, px[b], py[b], m1[b], p, eps2c, ax[b], ay[b],
-                                        acc, open);
-            if constexpr (kStats) {
-                n_acc += acc ? 1u : 0u;
-                n_open += open ? 1u : 0u;
-            }
+        a.pcm, a.child, leaf0, first_leaf, lane, node, mask, c, st, sp, stack_cap, a.overflow,
+        [&](const float4 &cc, int b) {
+            rx[b] = cc.x - px[b];
+            ry[b] = cc.y - py[b];
+            const float eps2 = kRefSoft ? soft_eps2(m1[b], cc.z, p) : eps2c;
+            inv[b] = inv_dist<kExact>(rx[b], ry[b], eps2);
+            return cc.w * inv[b] < p.theta;
         },
-        [&](int, bool dropped) {
+        [&](const float4 &cc, uint32_t, int b, uint64_t acc) {
+            float nx = ax[b], ny = ay[b];
+            add_pull<kExact>(cc, rx[b], ry[b], inv[b], p.grav, nx, ny);
+            const bool in = __builtin_amdgcn_inverse_ballot_w64(acc);
+            ax[b] = in ? nx : ax[b];
+            ay[b] = in ? ny : ay[b];
+        },
+        [&](int, uint64_t acc, uint64_t open, uint64_t dropped) {
             if constexpr (kStats) {
-                n_open -= dropped ? 1u : 0u;
-                n_drop += dropped ? 1u : 0u;
+                n_acc += __builtin_popcountll(acc);
+                n_open += __builtin_popcountll(open);
+                n_open -= __builtin_popcountll(dropped);
+                n_drop += __builtin_popcountll(dropped);
             }
         });
 
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
-        if (!valid[b]) continue;
-        const uint32_t bi = a.idx[t[b]];
-        const float dt = p.dt;
-        float2 v = a.vel[bi];
-        float2 x;
-        if constexpr (kExact) {
-            v.x = v.x + ax[b] * dt;  // barnes_hut.wgsl:110
-            v.y = v.y + ay[b] * dt;
-            x.x = px[b] + v.x * dt;  // barnes_hut.wgsl:120
-            x.y = py[b] + v.y * dt;
-        } else {
-            v.x = __builtin_fmaf(ax[b], dt, v.x);
-            v.y = __builtin_fmaf(ay[b], dt, v.y);
-            x.x = __builtin_fmaf(v.x, dt, px[b]);
-            x.y = __builtin_fmaf(v.y, dt, py[b]);
-        }
-        a.vel[bi] = v;
-        a.pos[bi] = x;
-        if (a.accel) a.accel[bi] = make_float2(ax[b], ay[b]);
+        if (valid[b])
+            kick_drift<kExact>(a, t[b], px[b], py[b], accel_out<kExact>(ax[b], p.grav),
+                               accel_out<kExact>(ay[b], p.grav));
     }
     if constexpr (kStats) {
-        uint32_t s0 = n_acc, s1 = n_open, s2 = n_drop;
-        for (int o = 32; o > 0; o >>= 1) {
-            s0 += __shfl_xor(s0, o);
-            s1 += __shfl_xor(s1, o);
-            s2 += __shfl_xor(s2, o);
-        }
+        // the counters are wave-uniform: one atomic per wave and counter
         if (lane == 0) {
-            atomicAdd(&a.stats[0], (unsigned long long)s0);
-            atomicAdd(&a.stats[1], (unsigned long long)s1);
-            if (s2) atomicAdd(&a.stats[2], (unsigned long long)s2);
+            atomicAdd(&a.stats[0], (unsigned long long)n_acc);
+            atomicAdd(&a.stats[1], (unsigned long long)n_open);
+            if (n_drop) atomicAdd(&a.stats[2], (unsigned long long)n_drop);
         }
     }
 }

@@ -81,9 +81,9 @@ struct RegStack {
     }
 };
 
-// Opening test of one lane: lengths, distances and 1/sqrt exactly as the
+// Distance term of one lane: lengths, distances and 1/sqrt exactly as the
 // forward walk computes them (barnes_hut.wgsl:81-94), so every walk built on
-// it takes the same decisions.  Returns inv = 1/sqrt(|r|^2 + eps^2).
+// it takes the same opening decisions.  Returns inv = 1/sqrt(|r|^2 + eps^2).
 template <bool kExact>
 __device__ __forceinline__ float inv_dist(float rx, float ry, float eps2) {
     if constexpr (kExact) {
@@ -94,23 +94,34 @@ __device__ __forceinline__ float inv_dist(float rx, float ry, float eps2) {
     }
 }
 
+// Lane mask `m` without the lane whose body owns leaf `leaf_node`, in a slot
+// whose lane 0 owns leaf `slot_leaf` (all wave-uniform: scalar ALU only).
+__device__ __forceinline__ uint64_t drop_self(uint64_t m, uint32_t leaf_node, uint32_t slot_leaf) {
+    const uint32_t r = leaf_node - slot_leaf;
+    return r < 64u ? (m & ~(1ull << r)) : m;
+}
+
 // Depth-first traversal of one wavefront from (node, mask, c) with `sp`
-// entries already on `st`.  Each lane holds kB bodies (slot b: my_leaf[b],
-// lane mask mask[b]); the wavefront's bodies share one traversal.
-// visit(c, leaf, node, b, act, acc, open) is called by the whole wavefront
-// for every visited node and slot (act: the slot's body evaluates it; the
-// visitor sets acc/open).  Stops when the stack is empty.  A full stack
-// drops both children like barnes_hut.wgsl:100-105, raises *overflow and
-// calls drop(b, lane_dropped) per slot.  A wavefront visits a node at most
-// once, so a tree from the build kernels ends within 2 * leaf0 + 1
-// iterations.  There is no iteration counter: its compare-and-exit costs
-// 12-19% of the walk time at 800k bodies.
-template <int kB, typename Visit, typename Drop>
+// entries already on `st`.  Each lane holds kB bodies; slot b's lane l owns
+// leaf first_leaf + 64 * b + l and mask[b] names the slot's bodies that
+// visit the node.  Per visited node and slot:
+//   probe(c, b) -> bool: the lane's opening test passes (c.w * inv < theta);
+//                  it also keeps what pull needs (called by the whole wave);
+//   pull(c, node, b, acc): the bodies in lane mask `acc` (non-empty) accept;
+//   tally(b, acc, open, dropped): lane masks for statistics.
+// A leaf is accepted by every body in the mask but its own; an internal
+// node by the bodies whose test passes, the others open it and form the
+// masks of its children.  All mask logic is scalar.  A full stack drops
+// both children like barnes_hut.wgsl:100-105 and raises *overflow.  A
+// wavefront visits a node at most once, so a tree from the build kernels
+// ends within 2 * leaf0 + 1 iterations.  There is no iteration counter: its
+// compare-and-exit costs 12-19% of the walk time at 800k bodies.
+template <int kB, typename Probe, typename Pull, typename Tally>
 __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child, uint32_t leaf0,
-                                          const uint32_t (&my_leaf)[kB], int lane, uint32_t node,
+                                          uint32_t first_leaf, int lane, uint32_t node,
                                           uint64_t (&mask)[kB], float4 c, RegStack<kB> &st,
                                           int sp, int stack_cap, uint32_t *overflow,
-                                          Visit &&visit, Drop &&drop) {
+                                          Probe &&probe, Pull &&pull, Tally &&tally) {
     const_as<uint32_t> *child_s = (const_as<uint32_t> *)child;
     const_as<float> *pcmf_s = (const_as<float> *)pcm;
     for (;;) {
@@ -119,12 +130,20 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
         uint64_t any = 0;
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
-            const bool act =
-                __builtin_amdgcn_inverse_ballot_w64(mask[b]) && !(leaf && node == my_leaf[b]);
-            bool acc, open;
-            visit(c, leaf, node, b, act, acc, open);
-            om[b] = __ballot(open);
-            any |= om[b];
+            const bool far = probe(c, b);
+            uint64_t acc, open;
+            if (leaf) {
+                acc = drop_self(mask[b], node, first_leaf + 64u * b);
+                open = 0;
+            } else {
+                const uint64_t fm = __builtin_amdgcn_ballot_w64(far);
+                acc = mask[b] & fm;
+                open = mask[b] & ~fm;
+            }
+            if (acc) pull(c, node, b, acc);
+            tally(b, acc, open, 0ull);
+            om[b] = open;
+            any |= open;
         }
         if (any) {
             // both children's records in one scalar load
@@ -135,13 +154,16 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
                                           pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
             const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
             if (chr >= leaf0) {
-                // A leaf child has no subtree: its lanes take it now, at the
+                // A leaf child has no subtree: its bodies take it now, at the
                 // same place in the order.
 #pragma unroll
                 for (int b = 0; b < kB; ++b) {
-                    bool a2, o2;
-                    visit(cr, true, chr, b,
-                          __builtin_amdgcn_inverse_ballot_w64(om[b]) && chr != my_leaf[b], a2, o2);
+                    const uint64_t acc = drop_self(om[b], chr, first_leaf + 64u * b);
+                    if (acc) {
+                        probe(cr, b);
+                        pull(cr, chr, b, acc);
+                    }
+                    tally(b, acc, 0ull, 0ull);
                 }
                 if (chl < leaf0) {
                     node = chl;
@@ -152,9 +174,12 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
                 }
 #pragma unroll
                 for (int b = 0; b < kB; ++b) {
-                    bool a2, o2;
-                    visit(cl, true, chl, b,
-                          __builtin_amdgcn_inverse_ballot_w64(om[b]) && chl != my_leaf[b], a2, o2);
+                    const uint64_t acc = drop_self(om[b], chl, first_leaf + 64u * b);
+                    if (acc) {
+                        probe(cl, b);
+                        pull(cl, chl, b, acc);
+                    }
+                    tally(b, acc, 0ull, 0ull);
                 }
                 // both children done: fall through to the pop
             } else if (sp < stack_cap) {
@@ -169,7 +194,7 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
             } else {
                 if (lane == 0) atomicOr(overflow, 1u);
 #pragma unroll
-                for (int b = 0; b < kB; ++b) drop(b, __builtin_amdgcn_inverse_ballot_w64(om[b]));
+                for (int b = 0; b < kB; ++b) tally(b, 0ull, 0ull, om[b]);
             }
         }
         if (sp == 0) break;
