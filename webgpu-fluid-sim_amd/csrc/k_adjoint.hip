@@ -50,10 +50,10 @@ template <bool kExact, int kB>
 __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, int stack_cap) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = a.n;
-    const uint32_t first = uniform_u32(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (64u * kB));
+    const uint32_t first = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (64u * kB);
     const uint32_t leaf0 = n - 1u;
 
-    uint32_t t[kB], bi[kB];
+    uint32_t t[kB], my_leaf[kB], bi[kB];
     bool valid[kB];
     float px[kB], py[kB], gx[kB], gy[kB], hx_sum[kB], hy_sum[kB];
     uint64_t mask[kB];
@@ -62,6 +62,7 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
     for (int b = 0; b < kB; ++b) {
         t[b] = first + 64u * b + lane;
         valid[b] = t[b] < n;
+        my_leaf[b] = leaf0 + t[b];
         px[b] = py[b] = gx[b] = gy[b] = hx_sum[b] = hy_sum[b] = 0.0f;
         bi[b] = 0;
         if (valid[b]) {
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
     RegStack<kB> st;
     float rx[kB], ry[kB], inv[kB];
     walk_tree<kB>(
-        a.pcm, a.child, leaf0, leaf0 + first, lane, 0u, mask, a.root[0], st, 0, stack_cap,
+        a.pcm, a.child, leaf0, my_leaf, 0u, mask, a.root[0], st, 0, stack_cap,
         a.overflow,
         [&](const float4 &c, int b) {
             rx[b] = c.x - px[b];

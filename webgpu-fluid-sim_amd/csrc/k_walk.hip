@@ -282,11 +282,10 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     uint32_t block = blockIdx.x;
     if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
     const uint32_t n = a.n;
-    const uint32_t first = uniform_u32(((block * blockDim.x + threadIdx.x) >> 6) * (64u * kB));
+    const uint32_t first = ((block * blockDim.x + threadIdx.x) >> 6) * (64u * kB);
     const uint32_t leaf0 = n - 1u;
-    const uint32_t first_leaf = leaf0 + first;  // leaf of slot 0, lane 0
 
-    uint32_t t[kB];
+    uint32_t t[kB], my_leaf[kB];
     bool valid[kB];
     float px[kB], py[kB], m1[kB], ax[kB], ay[kB];
     uint64_t mask[kB];
@@ -295,6 +294,7 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     for (int b = 0; b < kB; ++b) {
         t[b] = first + 64u * b + lane;  // sorted position
         valid[b] = t[b] < n;
+        my_leaf[b] = leaf0 + t[b];
         px[b] = py[b] = m1[b] = ax[b] = ay[b] = 0.0f;
         if (valid[b]) {
             const float4 me = a.leaf[t[b]];  // written by the fill: {pos, mass, 0}
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     float rx[kB], ry[kB], inv[kB];
     uint64_t n_acc = 0, n_open = 0, n_drop = 0;
     walk_tree<kB>(
-        a.pcm, a.child, leaf0, first_leaf, lane, node, mask, c, st, sp, stack_cap, a.overflow,
+        a.pcm, a.child, leaf0, my_leaf, node, mask, c, st, sp, stack_cap, a.overflow,
         [&](const float4 &cc, int b) {
             rx[b] = cc.x - px[b];
             ry[b] = cc.y - py[b];

@@ -64,20 +64,27 @@ struct RegStack {
         else
             put_row<1>(l, nd, m, c);
     }
-    __device__ __forceinline__ void get(int row, int l, uint32_t &nd, uint64_t (&m)[kB],
-                                        float4 &c) const {
-        const bool r = row != 0;
-        nd = (uint32_t)__builtin_amdgcn_readlane(r ? node[1] : node[0], l);
-        c.x = __int_as_float(__builtin_amdgcn_readlane(r ? cx[1] : cx[0], l));
-        c.y = __int_as_float(__builtin_amdgcn_readlane(r ? cy[1] : cy[0], l));
-        c.z = __int_as_float(__builtin_amdgcn_readlane(r ? cm[1] : cm[0], l));
-        c.w = __int_as_float(__builtin_amdgcn_readlane(r ? cw[1] : cw[0], l));
+    template <int r>
+    __device__ __forceinline__ void get_row(int l, uint32_t &nd, uint64_t (&m)[kB],
+                                            float4 &c) const {
+        nd = (uint32_t)__builtin_amdgcn_readlane(node[r], l);
+        c.x = __int_as_float(__builtin_amdgcn_readlane(cx[r], l));
+        c.y = __int_as_float(__builtin_amdgcn_readlane(cy[r], l));
+        c.z = __int_as_float(__builtin_amdgcn_readlane(cm[r], l));
+        c.w = __int_as_float(__builtin_amdgcn_readlane(cw[r], l));
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(r ? mlo[b][1] : mlo[b][0], l);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(r ? mhi[b][1] : mhi[b][0], l);
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(mlo[b][r], l);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(mhi[b][r], l);
             m[b] = ((uint64_t)hi << 32) | lo;
         }
+    }
+    __device__ __forceinline__ void get(int row, int l, uint32_t &nd, uint64_t (&m)[kB],
+                                        float4 &c) const {
+        if (row == 0)
+            get_row<0>(l, nd, m, c);
+        else
+            get_row<1>(l, nd, m, c);
     }
 };
 
@@ -94,36 +101,37 @@ __device__ __forceinline__ float inv_dist(float rx, float ry, float eps2) {
     }
 }
 
-// Lane mask `m` without the lane whose body owns leaf `leaf_node`, in a slot
-// whose lane 0 owns leaf `slot_leaf` (all wave-uniform: scalar ALU only).
-__device__ __forceinline__ uint64_t drop_self(uint64_t m, uint32_t leaf_node, uint32_t slot_leaf) {
-    const uint32_t r = leaf_node - slot_leaf;
-    return r < 64u ? (m & ~(1ull << r)) : m;
-}
-
 // Depth-first traversal of one wavefront from (node, mask, c) with `sp`
-// entries already on `st`.  Each lane holds kB bodies; slot b's lane l owns
-// leaf first_leaf + 64 * b + l and mask[b] names the slot's bodies that
-// visit the node.  Per visited node and slot:
+// entries already on `st`.  Each lane holds kB bodies; slot b's body in this
+// lane owns leaf my_leaf[b] and mask[b] names the slot's bodies that visit
+// the node.  Per visited node and slot:
 //   probe(c, b) -> bool: the lane's opening test passes (c.w * inv < theta);
 //                  it also keeps what pull needs (called by the whole wave);
 //   pull(c, node, b, acc): the bodies in lane mask `acc` (non-empty) accept;
 //   tally(b, acc, open, dropped): lane masks for statistics.
 // A leaf is accepted by every body in the mask but its own; an internal
 // node by the bodies whose test passes, the others open it and form the
-// masks of its children.  All mask logic is scalar.  A full stack drops
-// both children like barnes_hut.wgsl:100-105 and raises *overflow.  A
-// wavefront visits a node at most once, so a tree from the build kernels
-// ends within 2 * leaf0 + 1 iterations.  There is no iteration counter: its
-// compare-and-exit costs 12-19% of the walk time at 800k bodies.
+// masks of its children.  Leaf children are taken at once (they have no
+// subtree); of two internal children the right one is walked first and the
+// left one waits on the stack, as the reference's stack pops them
+// (barnes_hut.wgsl:100-105).  The loop has one back edge and one exit so the
+// control flow stays scalar-cheap.  A full stack drops both children like
+// the reference and raises *overflow.  A wavefront visits a node at most
+// once, so a tree from the build kernels ends within 2 * leaf0 + 1
+// iterations.  There is no iteration counter: its compare-and-exit costs
+// 12-19% of the walk time at 800k bodies.
 template <int kB, typename Probe, typename Pull, typename Tally>
 __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child, uint32_t leaf0,
-                                          uint32_t first_leaf, int lane, uint32_t node,
+                                          const uint32_t (&my_leaf)[kB], uint32_t node,
                                           uint64_t (&mask)[kB], float4 c, RegStack<kB> &st,
                                           int sp, int stack_cap, uint32_t *overflow,
                                           Probe &&probe, Pull &&pull, Tally &&tally) {
+    const_as<float> *pcm_s = (const_as<float> *)pcm;
     const_as<uint32_t> *child_s = (const_as<uint32_t> *)child;
-    const_as<float> *pcmf_s = (const_as<float> *)pcm;
+    // The bodies of `m` that accept leaf `leaf_node` (all but its owner).
+    auto leaf_acc = [&](uint64_t m, uint32_t leaf_node, int b) {
+        return m & __builtin_amdgcn_ballot_w64(my_leaf[b] != leaf_node);
+    };
     for (;;) {
         const bool leaf = node >= leaf0;
         uint64_t om[kB];
@@ -133,7 +141,7 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
             const bool far = probe(c, b);
             uint64_t acc, open;
             if (leaf) {
-                acc = drop_self(mask[b], node, first_leaf + 64u * b);
+                acc = leaf_acc(mask[b], node, b);
                 open = 0;
             } else {
                 const uint64_t fm = __builtin_amdgcn_ballot_w64(far);
@@ -145,61 +153,69 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
             om[b] = open;
             any |= open;
         }
+        bool go = false;
         if (any) {
-            // both children's records in one scalar load
-            const uint32_t pn = node;
-            const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
-                                          pcmf_s[8 * pn + 3]);
-            const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5],
-                                          pcmf_s[8 * pn + 6], pcmf_s[8 * pn + 7]);
-            const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
-            if (chr >= leaf0) {
-                // A leaf child has no subtree: its bodies take it now, at the
-                // same place in the order.
+            // both children's records and indices, 64-bit offsets (n < 2^30)
+            // (HIP vector types cannot be loaded through the constant
+            // address space: the records are read as dwords)
+            const size_t pn = node;
+            const_as<float> *rec = pcm_s + 8 * pn;
+            const float4 cl = make_float4(rec[0], rec[1], rec[2], rec[3]);
+            const float4 cr = make_float4(rec[4], rec[5], rec[6], rec[7]);
+            const uint2 ch = make_uint2(child_s[2 * pn], child_s[2 * pn + 1]);
+            const bool rleaf = ch.y >= leaf0, lleaf = ch.x >= leaf0;
+            // the child walked next: right if internal, else left; chosen
+            // here, beside the compare, so the choice stays a scalar select
+            const uint32_t next = rleaf ? ch.x : ch.y;
+            const float4 next_c = rleaf ? cl : cr;
+            if (rleaf) {
 #pragma unroll
                 for (int b = 0; b < kB; ++b) {
-                    const uint64_t acc = drop_self(om[b], chr, first_leaf + 64u * b);
+                    const uint64_t acc = leaf_acc(om[b], ch.y, b);
                     if (acc) {
                         probe(cr, b);
-                        pull(cr, chr, b, acc);
+                        pull(cr, ch.y, b, acc);
                     }
                     tally(b, acc, 0ull, 0ull);
                 }
-                if (chl < leaf0) {
-                    node = chl;
+                if (lleaf) {
 #pragma unroll
-                    for (int b = 0; b < kB; ++b) mask[b] = om[b];
-                    c = cl;
-                    continue;
-                }
-#pragma unroll
-                for (int b = 0; b < kB; ++b) {
-                    const uint64_t acc = drop_self(om[b], chl, first_leaf + 64u * b);
-                    if (acc) {
-                        probe(cl, b);
-                        pull(cl, chl, b, acc);
+                    for (int b = 0; b < kB; ++b) {
+                        const uint64_t acc = leaf_acc(om[b], ch.x, b);
+                        if (acc) {
+                            probe(cl, b);
+                            pull(cl, ch.x, b, acc);
+                        }
+                        tally(b, acc, 0ull, 0ull);
                     }
-                    tally(b, acc, 0ull, 0ull);
                 }
-                // both children done: fall through to the pop
-            } else if (sp < stack_cap) {
-                // push left, go right (the reference pops the right child first)
-                st.put(sp >> 6, sp & 63, chl, om, cl);
-                ++sp;
-                node = chr;
+            }
+            go = !(rleaf && lleaf);
+            if (!rleaf) {
+                if (sp < stack_cap) {
+                    st.put(sp >> 6, sp & 63, ch.x, om, cl);
+                    ++sp;
+                } else {
+                    // every lane stores the same flag: no divergent branch
+                    // in the loop (it would make the node selects per-lane)
+                    *overflow = 1u;
+#pragma unroll
+                    for (int b = 0; b < kB; ++b) tally(b, 0ull, 0ull, om[b]);
+                    go = false;
+                }
+            }
+            if (go) {
+                node = next;
+                c = next_c;
 #pragma unroll
                 for (int b = 0; b < kB; ++b) mask[b] = om[b];
-                c = cr;
-                continue;
-            } else {
-                if (lane == 0) atomicOr(overflow, 1u);
-#pragma unroll
-                for (int b = 0; b < kB; ++b) tally(b, 0ull, 0ull, om[b]);
             }
         }
-        if (sp == 0) break;
-        --sp;
-        st.get(sp >> 6, sp & 63, node, mask, c);
+        if (!go) {
+            if (sp == 0) break;
+            --sp;
+            st.get(sp >> 6, sp & 63, node, mask, c);
+        }
     }
 }
 
