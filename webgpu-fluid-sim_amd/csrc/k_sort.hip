@@ -8,9 +8,10 @@
 // Design (Adinets & Merrill, "Onesweep", 2022), written for wave64:
 //   1. one histogram kernel counts every pass's 8-bit digits at once
 //      (LDS histograms, one global atomic per bin per workgroup);
-//   2. one small kernel turns the counts into exclusive digit offsets;
-//   3. per pass, one kernel: each workgroup takes the next tile id from an
-//      atomic counter (so it only waits on tiles that already started),
+//   2. per pass, one kernel: each workgroup scans the pass's digit counts
+//      into exclusive digit offsets (256 entries, cheaper than a launch),
+//      takes the next tile id from an atomic counter (so it only waits on
+//      tiles that already started),
 //      ranks its 4096 keys stably with 64-lane ballot matching, publishes
 //      its per-digit counts, resolves its global offsets by decoupled
 //      look-back, and scatters through an LDS staging tile so that stores
@@ -26,6 +27,7 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;
+static_assert(kThreads == 256, "one thread per 8-bit digit");
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
@@ -69,20 +71,10 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(const K *__restrict__ ke
         if (sh[i]) atomicAdd(&hist[i], sh[i]);
 }
 
-// One block: per pass, counts -> exclusive offsets, in place.
-__global__ __launch_bounds__(kThreads) void offsets_kernel(uint32_t *__restrict__ hist, int passes) {
-    __shared__ uint32_t sh4[kWaves];
-    for (int p = 0; p < passes; ++p) {
-        uint32_t v = hist[p * 256 + threadIdx.x];
-        uint32_t e = block_exclusive_scan(v, sh4);
-        hist[p * 256 + threadIdx.x] = e;
-    }
-}
-
 template <typename K>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
-    uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_base,
+    uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_count,
     uint32_t *lookback, uint32_t *counter) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wcnt[kWaves * 256];  // wave digit counts, then wave offsets
@@ -95,7 +87,8 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     for (int i = tid; i < kWaves * 256; i += kThreads) s_wcnt[i] = 0;
-    __syncthreads();
+    // thread d: where digit d starts in the output (exclusive digit scan)
+    const uint32_t digit_base = block_exclusive_scan(digit_count[tid], s_sh4);
     const uint32_t tile = s_tile;
     const int64_t base = (int64_t)tile * kTile;
 
@@ -172,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         __hip_atomic_store(&mine[d], kFlagPrefix | (prefix + total), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    s_gstart[d] = digit_base[d] + prefix;
+    s_gstart[d] = digit_base + prefix;
     __syncthreads();
 
     // Scatter into the LDS tile in digit order, then stream out.
@@ -210,7 +203,6 @@ int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t 
     launch_clear_u32(sc.hist, sc.words, s);
     int64_t hblocks = tiles < 1024 ? tiles : 1024;
     hist_kernel<K><<<(unsigned)hblocks, kThreads, 0, s>>>(keys_a, n, passes, sc.hist);
-    offsets_kernel<<<1, kThreads, 0, s>>>(sc.hist, passes);
     K *ki = keys_a, *ko = keys_b;
     uint32_t *vi = vals_a, *vo = vals_b;
     for (int p = 0; p < passes; ++p) {
