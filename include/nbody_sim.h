@@ -146,8 +146,8 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
 
 /* Launch tunables.  Every setting computes the same result bit for bit.
  *   walk_block   walk workgroup size (64..1024, multiple of 64)
- *   walk_stack   walk stack entries per wavefront (8..256; the register stack
- *                holds at most 128)
+ *   walk_stack   walk stack entries per wavefront (8..256; the register-stack
+ *                walks use one or two rows of 64: [64, 128])
  *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default)
  *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
  *                of Morton-ordered bodies; 0: hardware order (default)

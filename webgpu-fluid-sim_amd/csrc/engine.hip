@@ -409,6 +409,9 @@ int debug_node_buffer(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
         const uint32_t pw = parent[(size_t)k];
         out[k] = pair[2 * (size_t)(pw & kParentMask) + (pw >> 31)];
     }
+    // leaves carry kLeafLength on the device; the reference's leaf length is 0
+    if (which == NBODY_BUF_NODE_CM)
+        for (int64_t k = n - 1; k < nodes; ++k) out[k].w = 0.0f;
     return NBODY_OK;
 }
 

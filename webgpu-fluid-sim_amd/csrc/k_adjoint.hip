@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
             rx[b] = c.x - px[b];
             ry[b] = c.y - py[b];
             inv[b] = inv_dist<kExact>(rx[b], ry[b], eps2);
-            return c.w * inv[b] < theta;
+            return passes(c.w, inv[b], theta);
         },
         [&](const float4 &c, uint32_t node, int b, uint64_t acc_mask) {
             const bool acc = __builtin_amdgcn_inverse_ballot_w64(acc_mask);
@@ -180,7 +180,7 @@ void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, fl
 void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, int lane_bodies,
                          hipStream_t s) {
     if (a.n == 0) return;
-    const int cap = stack < 128 ? stack : 128;
+    const int cap = stack < 64 ? 64 : (stack > 128 ? 128 : stack);
     if (lane_bodies == 2) {
         if (exact) launch_adjoint_walk_t<true, 2>(a, cap, s);
         else launch_adjoint_walk_t<false, 2>(a, cap, s);

@@ -270,7 +270,8 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
 
     // Write the block's completed nodes where their readers look for them.
     if (t < (uint64_t)n) {
-        const float4 c = s_cm_leaf[j];
+        float4 c = s_cm_leaf[j];
+        c.w = kLeafLength;
         if (n == 1)
             publish(tb, 0u, 0u, c, make_float4(c.x, c.y, c.x, c.y));
         else

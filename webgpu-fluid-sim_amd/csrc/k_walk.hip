@@ -102,7 +102,7 @@ __device__ __forceinline__ bool interact(const float4 c, bool leaf, float px, fl
     const float ry = c.y - py;
     const float eps2 = kRefSoft ? soft_eps2(m1, c.z, p) : eps2c;
     const float inv = inv_dist<kExact>(rx, ry, eps2);
-    if (leaf || c.w * inv < p.theta) {
+    if (leaf || passes(c.w, inv, p.theta)) {
         add_pull<kExact>(c, rx, ry, inv, p.grav, ax, ay);
         return false;
     }
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
             ry[b] = cc.y - py[b];
             const float eps2 = kRefSoft ? soft_eps2(m1[b], cc.z, p) : eps2c;
             inv[b] = inv_dist<kExact>(rx[b], ry[b], eps2);
-            return cc.w * inv[b] < p.theta;
+            return passes(cc.w, inv[b], p.theta);
         },
         [&](const float4 &cc, uint32_t, int b, uint64_t acc) {
             float nx = ax[b], ny = ay[b];
@@ -399,6 +399,9 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     }
 }
 
+// Register-stack capacity for a requested stack size: one or two rows of 64.
+int rs_stack_cap(int stack) { return stack < 64 ? 64 : (stack > 128 ? 128 : stack); }
+
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, int kB>
 void launch_rs(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s) {
     const uint32_t per_block = (uint32_t)block * kB;
@@ -409,7 +412,7 @@ void launch_rs(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s)
         grid = per * 8u;  // blocks past the bodies exit at once
     }
     walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats, kB>
-        <<<grid, block, 0, s>>>(a, stack < 128 ? stack : 128, per);
+        <<<grid, block, 0, s>>>(a, rs_stack_cap(stack), per);
 }
 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>

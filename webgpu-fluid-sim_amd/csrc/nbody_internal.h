@@ -58,6 +58,11 @@ int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32
 // therefore fetches both children's data with one 32-byte load.  The root,
 // which is nobody's child, has its own record.
 constexpr uint32_t kRightChild = 0x80000000u;  // parent word: index | side << 31
+// Length field of a leaf's node record.  Leaves are accepted whatever the
+// opening test says (barnes_hut.wgsl:89); with -inf as their length, the
+// walks' test passes for every leaf, so they need no leaf check.  The
+// reference's leaf length is 0 (the host reports it so).
+constexpr float kLeafLength = -__builtin_huge_valf();
 constexpr uint32_t kParentMask = 0x7FFFFFFFu;
 struct TreeBufs {
     uint2 *child;        // [n-1] {left, right}

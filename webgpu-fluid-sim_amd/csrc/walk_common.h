@@ -101,25 +101,34 @@ __device__ __forceinline__ float inv_dist(float rx, float ry, float eps2) {
     }
 }
 
+// The reference's opening test, length * inv < theta (barnes_hut.wgsl:89),
+// written as !(length * inv >= theta): the same for every number, and it
+// passes when the product is not a number (a leaf, length kLeafLength, with
+// inv == 0), so every leaf passes and none is ever opened.
+__device__ __forceinline__ bool passes(float length, float inv, float theta) {
+    return !(length * inv >= theta);
+}
+
 // Depth-first traversal of one wavefront from (node, mask, c) with `sp`
 // entries already on `st`.  Each lane holds kB bodies; slot b's body in this
 // lane owns leaf my_leaf[b] and mask[b] names the slot's bodies that visit
 // the node.  Per visited node and slot:
-//   probe(c, b) -> bool: the lane's opening test passes (c.w * inv < theta);
-//                  it also keeps what pull needs (called by the whole wave);
+//   probe(c, b) -> bool: passes(c.w, inv, theta) for the lane's body; it
+//                  also keeps what pull needs (called by the whole wave);
 //   pull(c, node, b, acc): the bodies in lane mask `acc` (non-empty) accept;
 //   tally(b, acc, open, dropped): lane masks for statistics.
-// A leaf is accepted by every body in the mask but its own; an internal
-// node by the bodies whose test passes, the others open it and form the
-// masks of its children.  Leaf children are taken at once (they have no
-// subtree); of two internal children the right one is walked first and the
-// left one waits on the stack, as the reference's stack pops them
-// (barnes_hut.wgsl:100-105).  The loop has one back edge and one exit so the
-// control flow stays scalar-cheap.  A full stack drops both children like
-// the reference and raises *overflow.  A wavefront visits a node at most
-// once, so a tree from the build kernels ends within 2 * leaf0 + 1
-// iterations.  There is no iteration counter: its compare-and-exit costs
-// 12-19% of the walk time at 800k bodies.
+// The bodies that pass accept the node (a leaf always passes; its owner is
+// left out), the others open it and form the mask of its children.  Leaf
+// children are taken at once (they have no subtree); of two internal
+// children the right one is walked first and the left one waits on the
+// stack, as the reference's stack pops them (barnes_hut.wgsl:100-105).  All
+// mask logic is scalar.  The stack holds stack_cap entries, at least 64 (the
+// tree depth, and so the stack depth, stays below 64 for 32-bit keys); past
+// stack_cap the left child is dropped and *overflow raised (the reference
+// drops both children of a full stack; either result is flagged as wrong).
+// A wavefront visits a node at most once, so a tree from the build kernels
+// ends within 2 * leaf0 + 1 iterations.  There is no iteration counter: its
+// compare-and-exit costs 12-19% of the walk time at 800k bodies.
 template <int kB, typename Probe, typename Pull, typename Tally>
 __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child, uint32_t leaf0,
                                           const uint32_t (&my_leaf)[kB], uint32_t node,
@@ -133,27 +142,21 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
         return m & __builtin_amdgcn_ballot_w64(my_leaf[b] != leaf_node);
     };
     for (;;) {
-        const bool leaf = node >= leaf0;
         uint64_t om[kB];
         uint64_t any = 0;
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
-            const bool far = probe(c, b);
-            uint64_t acc, open;
-            if (leaf) {
-                acc = leaf_acc(mask[b], node, b);
-                open = 0;
-            } else {
-                const uint64_t fm = __builtin_amdgcn_ballot_w64(far);
-                acc = mask[b] & fm;
-                open = mask[b] & ~fm;
-            }
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(probe(c, b));
+            const uint64_t acc = mask[b] & fm & __builtin_amdgcn_ballot_w64(my_leaf[b] != node);
+            const uint64_t open = mask[b] & ~fm;
             if (acc) pull(c, node, b, acc);
             tally(b, acc, open, 0ull);
             om[b] = open;
             any |= open;
         }
-        bool go = false;
+        // 1 when the walk goes on into a child; an opaque scalar integer
+        // (a boolean crossing these blocks would become a lane mask)
+        uint32_t go = 0u;
         if (any) {
             // both children's records and indices, 64-bit offsets (n < 2^30)
             // (HIP vector types cannot be loaded through the constant
@@ -190,10 +193,14 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
                     }
                 }
             }
-            go = !(rleaf && lleaf);
+            go = uniform_u32((rleaf && lleaf) ? 0u : 1u);
             if (!rleaf) {
-                if (sp < stack_cap) {
-                    st.put(sp >> 6, sp & 63, ch.x, om, cl);
+                // push left, go right (the reference pops the right child first)
+                if (sp < 64) {
+                    st.template put_row<0>(sp, ch.x, om, cl);
+                    ++sp;
+                } else if (sp < stack_cap) {
+                    st.template put_row<1>(sp & 63, ch.x, om, cl);
                     ++sp;
                 } else {
                     // every lane stores the same flag: no divergent branch
@@ -201,7 +208,6 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
                     *overflow = 1u;
 #pragma unroll
                     for (int b = 0; b < kB; ++b) tally(b, 0ull, 0ull, om[b]);
-                    go = false;
                 }
             }
             if (go) {
