@@ -182,7 +182,7 @@ int allocate(nbody_sim *s, int64_t cap) {
             s->vals[b] = a.take<uint32_t>(cap);
         }
         s->tree.leaf = a.take<float4>(cap);
-        s->tree.pcm = a.take<float4>(2 * cap);
+        s->tree.pcm = a.take<float4>(kPairStride * cap);
         s->tree.paabb = a.take<float4>(2 * cap);
         s->tree.root = a.take<float4>(2);
         s->tree.child = a.take<uint2>(cap);
@@ -398,16 +398,19 @@ int debug_node_buffer(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
         std::memcpy(dst, parent.data(), 4 * (size_t)nodes);
         return NBODY_OK;
     }
-    std::vector<float4> pair((size_t)std::max<int64_t>(2 * (n - 1), 1)), root(2);
-    const float4 *src = which == NBODY_BUF_NODE_CM ? s->tree.pcm : s->tree.paabb;
+    const bool cm = which == NBODY_BUF_NODE_CM;
+    const size_t per = cm ? kPairStride : 2;  // float4s per pair
+    std::vector<float4> pair(per * (size_t)std::max<int64_t>(n - 1, 1)), root(2);
+    const float4 *src = cm ? s->tree.pcm : s->tree.paabb;
     if (n > 1)
-        HIP_TRY(s, hipMemcpy(pair.data(), src, 32 * (size_t)(n - 1), hipMemcpyDeviceToHost));
+        HIP_TRY(s, hipMemcpy(pair.data(), src, 16 * per * (size_t)(n - 1),
+                             hipMemcpyDeviceToHost));
     HIP_TRY(s, hipMemcpy(root.data(), s->tree.root, 32, hipMemcpyDeviceToHost));
     float4 *out = static_cast<float4 *>(dst);
     out[0] = which == NBODY_BUF_NODE_CM ? root[0] : root[1];
     for (int64_t k = 1; k < nodes; ++k) {
         const uint32_t pw = parent[(size_t)k];
-        out[k] = pair[2 * (size_t)(pw & kParentMask) + (pw >> 31)];
+        out[k] = pair[cm ? pcm_slot(pw) : paabb_slot(pw)];
     }
     // leaves carry kLeafLength on the device; the reference's leaf length is 0
     if (which == NBODY_BUF_NODE_CM)

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
     RegStack<kB> st;
     float rx[kB], ry[kB], inv[kB];
     walk_tree<kB>(
-        a.pcm, a.child, leaf0, my_leaf, 0u, mask, a.root[0], st, 0, stack_cap,
+        a.pcm, leaf0, my_leaf, 0u, mask, a.root[0], st, 0, stack_cap,
         a.overflow,
         [&](const float4 &c, int b) {
             rx[b] = c.x - px[b];
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
 __device__ __forceinline__ float4 node_data(const AdjointArgs &a, uint32_t node) {
     if (node == 0u) return a.root[0];
     const uint32_t pw = a.parent[node];
-    return a.pcm[2u * (pw & kParentMask) + (pw >> 31)];
+    return a.pcm[pcm_slot(pw)];
 }
 
 // One thread per sorted leaf: sum lc_n / M_n and lM_n - c_n . lc_n / M_n over

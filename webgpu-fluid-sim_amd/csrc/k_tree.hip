@@ -106,6 +106,8 @@ __global__ __launch_bounds__(256) void build_u32_kernel(const uint32_t *__restri
     DeltaU32 delta{keys, n};
     const Split sp = karras_node(delta, i, n);
     tb.child[i] = make_uint2(sp.lc, sp.rc);
+    // the walk reads the indices from the pair record
+    *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
     tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
     tb.parent[sp.lc] = (uint32_t)i;  // lbvh.wgsl:155-161
     tb.parent[sp.rc] = (uint32_t)i | kRightChild;
@@ -123,6 +125,8 @@ __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restri
     DeltaU64 delta{keys, sim_off, n};
     const Split sp = karras_node(delta, i, n);
     tb.child[i] = make_uint2(sp.lc, sp.rc);
+    // the walk reads the indices from the pair record
+    *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
     tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
     tb.parent[sp.lc] = (uint32_t)i;
     tb.parent[sp.rc] = (uint32_t)i | kRightChild;
@@ -164,7 +168,8 @@ __device__ __forceinline__ void merge_values(const float4 &L, const float4 &LB, 
 // Node p from its children's records in p's child-pair slots.
 __device__ __forceinline__ void merge_children(const TreeBufs &tb, uint32_t p, float4 &C,
                                                float4 &B) {
-    merge_values(tb.pcm[2 * p], tb.paabb[2 * p], tb.pcm[2 * p + 1], tb.paabb[2 * p + 1], C, B);
+    merge_values(tb.pcm[kPairStride * (size_t)p], tb.paabb[2 * (size_t)p],
+                 tb.pcm[kPairStride * (size_t)p + 1], tb.paabb[2 * (size_t)p + 1], C, B);
 }
 
 // Store a completed node's data where its readers look for it: the child
@@ -176,9 +181,8 @@ __device__ __forceinline__ void publish(const TreeBufs &tb, uint32_t node, uint3
         tb.root[1] = B;
         return;
     }
-    const uint32_t slot = 2u * (pw & kParentMask) + (pw >> 31);
-    tb.pcm[slot] = C;
-    tb.paabb[slot] = B;
+    tb.pcm[pcm_slot(pw)] = C;
+    tb.paabb[paabb_slot(pw)] = B;
 }
 
 constexpr int kSuperBlocks = 32;                     // phase-1 blocks per phase-2 workgroup

@@ -114,7 +114,7 @@ __device__ __forceinline__ bool interact(const float4 c, bool leaf, float px, fl
 __device__ __forceinline__ float4 node_record(const WalkArgs &a, uint32_t node) {
     if (node == 0u) return a.root[0];
     const uint32_t pw = a.parent[node];
-    return a.pcm[2u * (pw & kParentMask) + (pw >> 31)];
+    return a.pcm[pcm_slot(pw)];
 }
 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
     const bool valid = t < n;
     const uint32_t leaf0 = n - 1u;
     const uint32_t my_leaf = leaf0 + t;
-    const_as<uint32_t> *child_s = (const_as<uint32_t> *)a.child;
     const_as<float> *pcmf_s = (const_as<float> *)a.pcm;
+    const_as<uint32_t> *pcmu_s = (const_as<uint32_t> *)a.pcm;
 
     float px = 0.0f, py = 0.0f, m1 = 0.0f;
     if (valid) {
@@ -207,12 +207,12 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
         const bool leaf = node >= leaf0;
         // Speculative scalar loads of the children (used if the node opens);
         // a leaf reads pair 0, which always exists when n >= 2.
-        const uint32_t pn = leaf ? 0u : node;
-        const float4 cl = make_float4(pcmf_s[8 * pn], pcmf_s[8 * pn + 1], pcmf_s[8 * pn + 2],
-                                      pcmf_s[8 * pn + 3]);
-        const float4 cr = make_float4(pcmf_s[8 * pn + 4], pcmf_s[8 * pn + 5], pcmf_s[8 * pn + 6],
-                                      pcmf_s[8 * pn + 7]);
-        const uint32_t chl = child_s[2 * pn], chr = child_s[2 * pn + 1];
+        const size_t pn = leaf ? 0u : node;
+        const_as<float> *rec = pcmf_s + 4 * kPairStride * pn;
+        const float4 cl = make_float4(rec[0], rec[1], rec[2], rec[3]);
+        const float4 cr = make_float4(rec[4], rec[5], rec[6], rec[7]);
+        const uint32_t chl = pcmu_s[4 * kPairStride * pn + 8];
+        const uint32_t chr = pcmu_s[4 * kPairStride * pn + 9];
         bool open = false;
         if (((mask >> lane) & 1u) && !(leaf && node == my_leaf)) {
             open = interact<kExact, kRefSoft>(c, leaf, px, py, m1, p, eps2c, ax, ay);
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     float rx[kB], ry[kB], inv[kB];
     uint64_t n_acc = 0, n_open = 0, n_drop = 0;
     walk_tree<kB>(
-        a.pcm, a.child, leaf0, my_leaf, node, mask, c, st, sp, stack_cap, a.overflow,
+        a.pcm, leaf0, my_leaf, node, mask, c, st, sp, stack_cap, a.overflow,
         [&](const float4 &cc, int b) {
             rx[b] = cc.x - px[b];
             ry[b] = cc.y - py[b];

@@ -52,11 +52,13 @@ int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32
                     int64_t n, int bits, const SortScratch &sc, hipStream_t s);
 
 // ---------------------------------------------------------------- tree
-// Node data is stored in child pairs: each internal node p holds the
-// {com.x, com.y, mass, length} records of its two children in pcm[2p] (left)
-// and pcm[2p+1] (right), and their AABBs in paabb likewise.  Opening a node
-// therefore fetches both children's data with one 32-byte load.  The root,
-// which is nobody's child, has its own record.
+// Node data is stored in child pairs: each internal node p has a 64-byte
+// pair record pcm[4p .. 4p+3]: the {com.x, com.y, mass, length} records of
+// its left and right children, then the two child indices {left, right}
+// (bits of pcm[4p+2].x, .y; the rest is padding).  The children's AABBs sit
+// in paabb[2p], paabb[2p+1].  Opening a node therefore fetches everything
+// the walk needs from one aligned 64-byte line.  The root, which is
+// nobody's child, has its own record.
 constexpr uint32_t kRightChild = 0x80000000u;  // parent word: index | side << 31
 // Length field of a leaf's node record.  Leaves are accepted whatever the
 // opening test says (barnes_hut.wgsl:89); with -inf as their length, the
@@ -64,13 +66,21 @@ constexpr uint32_t kRightChild = 0x80000000u;  // parent word: index | side << 3
 // reference's leaf length is 0 (the host reports it so).
 constexpr float kLeafLength = -__builtin_huge_valf();
 constexpr uint32_t kParentMask = 0x7FFFFFFFu;
+constexpr int kPairStride = 4;  // float4s per pair record
+// Slot of a node's record in pcm / in paabb, from its parent word.
+__host__ __device__ inline size_t pcm_slot(uint32_t pw) {
+    return (size_t)kPairStride * (pw & kParentMask) + (pw >> 31);
+}
+__host__ __device__ inline size_t paabb_slot(uint32_t pw) {
+    return 2 * (size_t)(pw & kParentMask) + (pw >> 31);
+}
 struct TreeBufs {
     uint2 *child;        // [n-1] {left, right}
     uint2 *range;        // [n-1] {first, last} sorted leaf of each internal node
     uint32_t *parent;    // [2n-1] parent index | kRightChild for right children
     uint32_t *status;    // [n-1] arrival counters of the fill
     float4 *leaf;        // [n] sorted bodies {x, y, mass, 0}
-    float4 *pcm;         // [2(n-1)] children's {com.x, com.y, mass, length}
+    float4 *pcm;         // [4(n-1)] pair records (children's {com, mass, length}, indices)
     float4 *paabb;       // [2(n-1)] children's {min.x, min.y, max.x, max.y}
     float4 *root;        // [2] root {cm, aabb}
     uint32_t *frontier;  // fill phase 1 -> 2 hand-off, bucketed by super-block
@@ -88,7 +98,7 @@ int64_t fill_count_words(int64_t cap);
 // ---------------------------------------------------------------- walk
 struct WalkArgs {
     const float4 *leaf;      // [n] sorted bodies {x, y, mass, 0}
-    const float4 *pcm;       // [2(n-1)] child-pair records
+    const float4 *pcm;       // [4(n-1)] pair records
     const float4 *root;      // root record
     const uint32_t *parent;  // parent words (ensemble sub-tree roots)
     const uint2 *child;      // [n-1]

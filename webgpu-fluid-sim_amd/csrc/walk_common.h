@@ -130,13 +130,13 @@ __device__ __forceinline__ bool passes(float length, float inv, float theta) {
 // ends within 2 * leaf0 + 1 iterations.  There is no iteration counter: its
 // compare-and-exit costs 12-19% of the walk time at 800k bodies.
 template <int kB, typename Probe, typename Pull, typename Tally>
-__device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child, uint32_t leaf0,
+__device__ __forceinline__ void walk_tree(const float4 *pcm, uint32_t leaf0,
                                           const uint32_t (&my_leaf)[kB], uint32_t node,
                                           uint64_t (&mask)[kB], float4 c, RegStack<kB> &st,
                                           int sp, int stack_cap, uint32_t *overflow,
                                           Probe &&probe, Pull &&pull, Tally &&tally) {
     const_as<float> *pcm_s = (const_as<float> *)pcm;
-    const_as<uint32_t> *child_s = (const_as<uint32_t> *)child;
+    const_as<uint32_t> *pcmu_s = (const_as<uint32_t> *)pcm;
     // The bodies of `m` that accept leaf `leaf_node` (all but its owner).
     auto leaf_acc = [&](uint64_t m, uint32_t leaf_node, int b) {
         return m & __builtin_amdgcn_ballot_w64(my_leaf[b] != leaf_node);
@@ -158,14 +158,15 @@ __device__ __forceinline__ void walk_tree(const float4 *pcm, const uint2 *child,
         // (a boolean crossing these blocks would become a lane mask)
         uint32_t go = 0u;
         if (any) {
-            // both children's records and indices, 64-bit offsets (n < 2^30)
-            // (HIP vector types cannot be loaded through the constant
-            // address space: the records are read as dwords)
-            const size_t pn = node;
-            const_as<float> *rec = pcm_s + 8 * pn;
+            // the pair record: both children's records and indices, one
+            // 64-byte line, 64-bit offsets (n < 2^30) (HIP vector types
+            // cannot be loaded through the constant address space: the
+            // record is read as dwords)
+            const size_t rb = 4 * kPairStride * (size_t)node;
+            const_as<float> *rec = pcm_s + rb;
             const float4 cl = make_float4(rec[0], rec[1], rec[2], rec[3]);
             const float4 cr = make_float4(rec[4], rec[5], rec[6], rec[7]);
-            const uint2 ch = make_uint2(child_s[2 * pn], child_s[2 * pn + 1]);
+            const uint2 ch = make_uint2(pcmu_s[rb + 8], pcmu_s[rb + 9]);
             const bool rleaf = ch.y >= leaf0, lleaf = ch.x >= leaf0;
             // the child walked next: right if internal, else left; chosen
             // here, beside the compare, so the choice stays a scalar select
