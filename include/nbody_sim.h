@@ -118,9 +118,13 @@ int nbody_set_stream(nbody_sim *sim, void *hip_stream);
 int nbody_use_graph(nbody_sim *sim, int32_t enable);
 
 /* Debug flags for later steps: NBODY_DEBUG_ACCEL records each body's
- * acceleration; NBODY_DEBUG_STATS counts accepted and opened nodes. */
+ * acceleration; NBODY_DEBUG_STATS counts accepted and opened nodes;
+ * NBODY_DEBUG_FILL_CLIMB makes the tree fill complete its upper levels by
+ * climbing through global arrival counters (the path it takes on its own
+ * only when a level's node list is too long for workgroup memory). */
 #define NBODY_DEBUG_ACCEL 1
 #define NBODY_DEBUG_STATS 2
+#define NBODY_DEBUG_FILL_CLIMB 4
 int nbody_set_debug(nbody_sim *sim, int32_t flags);
 
 /* Intermediate buffers of the last step (reference buffers in comments). */

@@ -20,12 +20,13 @@ def params(theta=0.5, dt=1e-4, eps=1e-3, G=1.0, eps_mult=1.0, soft="constant", m
     return cfg, P
 
 
-def gpu_step(cfg, mass, pos, vel, nsteps=1, math="exact", graph=False, stats=False):
+def gpu_step(cfg, mass, pos, vel, nsteps=1, math="exact", graph=False, stats=False,
+             fill_climb=False):
     """Run nsteps on the GPU; return the debug buffers of the last step + state."""
     sim = nbody_sim.Simulation(cfg.with_(mathMode=math), device=0)
     try:
         sim.setBodies(mass, pos, vel)
-        sim.setDebug(accel=True, stats=stats)
+        sim.setDebug(accel=True, stats=stats, fill_climb=fill_climb)
         if graph:
             sim.useGraph(True)
         sim.step(nsteps)

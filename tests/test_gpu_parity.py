@@ -128,6 +128,21 @@ def test_small_sizes(n):
     assert np.array_equal(g["accel"], r["accel"])
 
 
+@pytest.mark.parametrize("climb", [False, True])
+def test_fill_upper_levels(climb):
+    """Five super-blocks of 32768 leaves: the fill's upper levels, resolved
+    from the build's node lists in workgroup memory (default) and by the
+    global-counter climb, equal the oracle bit for bit."""
+    mass, pos, vel = nbody_sim.generate("exp-disk", 150000, seed=7, dispersion=0.05,
+                                        mass_sigma=0.5)
+    cfg, P = params(theta=0.6, dt=1e-4, eps=1e-3)
+    g = gpu_step(cfg, mass, pos, vel, nsteps=2, fill_climb=climb)
+    r = oracle_step(P, mass, pos, vel, nsteps=2)
+    _compare_tree(g, r)
+    assert np.array_equal(g["accel"], r["accel"])
+    assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
+
+
 def test_coincident_bodies():
     n = 500
     pos = np.zeros((n, 2), np.float32)

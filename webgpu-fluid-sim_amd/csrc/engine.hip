@@ -190,8 +190,9 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->tree.parent = a.take<uint32_t>(nodes);
         s->tree.status = a.take<uint32_t>(cap);
         s->tree.frontier = a.take<uint32_t>(fill_frontier_words(cap));
-        s->tree.top = a.take<uint32_t>(cap);
         s->tree.counts = a.take<uint32_t>(fill_count_words(cap));
+        s->tree.lists = a.take<uint32_t>(fill_list_words(cap));
+        s->tree.slot = a.take<uint32_t>(cap);
         s->sort_words = a.take<uint32_t>(sort_words);
         s->bbox_partial = a.take<float4>((int64_t)s->max_chunks * s->num_sims);
         s->bbox = a.take<float4>(s->num_sims);
@@ -200,7 +201,8 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->sim_root = a.take<uint32_t>(s->num_sims);
         s->accel = a.take<float2>(cap);
         s->stats = a.take<unsigned long long>(4);
-        s->overflow = a.take<uint32_t>(1);
+        s->overflow = a.take<uint32_t>(2);  // [0] walk stack overflow, [1] tree fill fault
+        s->tree.fault = s->overflow + 1;
         if (pass == 0) {
             a.size = a.used;
             void *p = nullptr;
@@ -217,7 +219,7 @@ int allocate(nbody_sim *s, int64_t cap) {
     sort_bind_scratch(s->sort, s->sort_words, cap, s->passes);
     s->cap = cap;
     HIP_TRY(s, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->stream));
-    HIP_TRY(s, hipMemsetAsync(s->overflow, 0, sizeof(uint32_t), s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->overflow, 0, 2 * sizeof(uint32_t), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->tree.counts, 0, 4 * (size_t)fill_count_words(cap), s->stream));
     return NBODY_OK;
 }
@@ -261,7 +263,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
         launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, st);
     mark(4);
     // phase 4: bottom-up fill (lbvh.wgsl:168-233)
-    launch_fill(idx, s->mass, s->pos, n, s->tree, st);
+    launch_fill(idx, s->mass, s->pos, n, s->tree, (s->debug & NBODY_DEBUG_FILL_CLIMB) != 0, st);
     mark(5);
     if (tree_only) return NBODY_OK;
     // phase 5: walk + kick + drift (barnes_hut.wgsl:48-121)
@@ -584,10 +586,12 @@ int nbody_synchronize(nbody_sim *s) {
     if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
     HIP_TRY(s, hipSetDevice(s->device));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
-    uint32_t of = 0;
-    HIP_TRY(s, hipMemcpy(&of, s->overflow, sizeof(of), hipMemcpyDeviceToHost));
-    if (of) {
-        HIP_TRY(s, hipMemset(s->overflow, 0, sizeof(uint32_t)));
+    uint32_t of[2] = {0, 0};
+    HIP_TRY(s, hipMemcpy(of, s->overflow, sizeof(of), hipMemcpyDeviceToHost));
+    if (of[0] || of[1]) {
+        HIP_TRY(s, hipMemset(s->overflow, 0, sizeof(of)));
+        if (of[1])
+            return fail(s, NBODY_ERR_STATE, "tree fill did not complete: malformed tree");
         return fail(s, NBODY_ERR_STACK_OVERFLOW,
                     "tree walk stack overflow: nodes were dropped (raise walk_stack)");
     }
@@ -612,7 +616,7 @@ int nbody_use_graph(nbody_sim *s, int32_t enable) {
 
 int nbody_set_debug(nbody_sim *s, int32_t flags) {
     if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
-    if (flags & ~(NBODY_DEBUG_ACCEL | NBODY_DEBUG_STATS))
+    if (flags & ~(NBODY_DEBUG_ACCEL | NBODY_DEBUG_STATS | NBODY_DEBUG_FILL_CLIMB))
         return fail(s, NBODY_ERR_INVALID_ARG, "unknown debug flag");
     s->debug = flags;
     s->graph_dirty = true;

@@ -24,8 +24,10 @@
 //       its bucket through the nodes inside the super-block (global arrival
 //       counters, workgroup scope); nodes whose parent spans super-blocks go
 //       to the top list.
-//     phase 3 (fill_top_kernel): one workgroup finishes the top of the tree.
-//   Kernel boundaries order the phases.
+//     phase 3 (the last fill_mid workgroup to finish): climbs the top list
+//       to the root.
+//   A kernel boundary orders phases 1 and 2; one device-scope fence per
+//   phase-2 workgroup orders phases 2 and 3.
 //   The merge formula does not depend on which child arrives last, so the
 //   result is deterministic and equals a sequential post-order fill.  The
 //   reference's random leaf rotation (`offset`, lbvh.wgsl:171) is therefore
@@ -70,6 +72,30 @@ struct Split {
     int64_t lo, hi, gamma;
     uint32_t lc, rc;
 };
+
+// Fill hierarchy: phase-1 blocks of 1024 sorted leaves, 32 blocks to a
+// super-block.  A node belongs to the phase whose unit holds its whole leaf
+// range: a block (phase 1), a super-block (phase 2), else the top (phase 3).
+constexpr int kBlockShift = 10;
+constexpr int kSuperShift = 15;
+constexpr int kFillBlock = 1 << kBlockShift;
+constexpr int kSuper = 1 << kSuperShift;                         // leaves per super-block
+// The fill's upper levels: the phase-2 workgroup of super-block s finds
+// its phase-2 nodes (and the top nodes) by their leaf ranges among node
+// indices [s * kSuper, (s+1) * kSuper): a node's index is an end of its
+// range, so a node whose range lies in super-block s has its index there.
+constexpr uint32_t kLdsList = 1024;  // longest list the fill resolves in LDS
+constexpr uint32_t kScan = kSuper / 1024;  // node indices per thread of the scan
+__host__ __device__ inline uint32_t list3_cap(uint32_t num_super) { return num_super * 96u + 64u; }
+__host__ __device__ inline uint32_t num_supers(int64_t n) {
+    return (uint32_t)((n + kSuper - 1) / kSuper);
+}
+// Counter words (tb.counts): [0, S) phase-1 frontier buckets, [S] phase-2
+// done count, [S+1] top list length; S = number of super-blocks.  The top
+// list (every node whose range spans super-blocks, with tb.slot) is
+// tb.lists[0, list3_cap(S)).
+
+__device__ __forceinline__ bool spans(uint2 r, int shift) { return (r.x >> shift) != (r.y >> shift); }
 
 template <typename Delta>
 __device__ __forceinline__ Split karras_node(const Delta &delta, int64_t i, int64_t n) {
@@ -145,7 +171,6 @@ __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restri
     }
 }
 
-constexpr int kFillBlock = 1024;
 
 // lbvh.wgsl:205-228: a node from its two complete children.
 __device__ __forceinline__ void merge_values(const float4 &L, const float4 &LB, const float4 &R,
@@ -185,8 +210,6 @@ __device__ __forceinline__ void publish(const TreeBufs &tb, uint32_t node, uint3
     tb.paabb[paabb_slot(pw)] = B;
 }
 
-constexpr int kSuperBlocks = 32;                     // phase-1 blocks per phase-2 workgroup
-constexpr int kSuper = kFillBlock * kSuperBlocks;   // leaves per super-block
 
 __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *__restrict__ idx,
                                                                 const float *__restrict__ mass,
@@ -284,52 +307,246 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
     if (t < nint && s_done[j]) publish(tb, t, t ? s_par_int[j] : 0u, s_cm_int[j], s_bb_int[j]);
 }
 
-// Climb from `cur` (its data already published) while the parents stay in
-// [lo, hi]; a node whose parent leaves that range goes to out[atomic(*out_n)].
+// Climb from `cur` (its data already published) while the parents' leaf
+// ranges stay in [lo, hi] (global arrival counters, workgroup scope).  A
+// parent's range and parent word come from the build (an earlier kernel),
+// so both are read together, before the arrival counter: a level costs three
+// dependent trips to memory (those reads, the counter, the children's data).
 __device__ __forceinline__ void climb_wg(const TreeBufs &tb, uint32_t nint, uint32_t cur,
-                                         uint32_t lo, uint32_t hi, uint32_t *out,
-                                         uint32_t *out_n) {
+                                         uint32_t lo, uint32_t hi) {
+    uint32_t pw = tb.parent[cur];
     for (int level = 0; level < 128; ++level) {
-        const uint32_t par = tb.parent[cur] & kParentMask;
+        const uint32_t par = pw & kParentMask;
         if (par >= nint) break;
         const uint2 r = tb.range[par];
-        if (r.x < lo || r.y > hi) {
-            out[atomicAdd(out_n, 1u)] = cur;
-            break;
-        }
+        const uint32_t ppw = par ? tb.parent[par] : 0u;
+        if (r.x < lo || r.y > hi) break;
         const uint32_t prev = __hip_atomic_fetch_add(&tb.status[par], 1u, __ATOMIC_ACQ_REL,
                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
         if (prev == 0u) break;
         float4 C, B;
         merge_children(tb, par, C, B);
+        publish(tb, par, ppw, C, B);
         cur = par;
-        publish(tb, cur, cur ? tb.parent[cur] : 0u, C, B);
+        pw = ppw;
         if (cur == 0u) break;
     }
 }
 
-// Phase 2: workgroup sb owns super-block sb's leaves.
-__global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBufs tb,
-                                                              uint32_t num_super) {
-    const uint32_t sb = blockIdx.x;
-    const uint32_t nint = (uint32_t)(n - 1);
-    const uint32_t lo = sb * (uint32_t)kSuper;
-    const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
-    const uint32_t count = tb.counts[sb];
-    for (uint32_t k = threadIdx.x; k < count; k += kFillBlock)
-        climb_wg(tb, nint, tb.frontier[(size_t)sb * kSuper + k], lo, hi, tb.top,
-                 &tb.counts[num_super]);
+// Exclusive prefix sum over the kFillBlock threads of a workgroup; *total
+// receives the sum.  sh: kFillBlock / 64 + 1 words of LDS.
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *sh, uint32_t *total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < kFillBlock / 64; ++w) {
+        const uint32_t t = sh[w];
+        before += w < wid ? t : 0u;
+        all += t;
+    }
+    *total = all;
+    return before + x - v;
 }
 
-// Phase 3: one workgroup finishes the tree, then clears the hand-off counts.
-__global__ __launch_bounds__(kFillBlock) void fill_top_kernel(int64_t n, TreeBufs tb,
-                                                              uint32_t num_super) {
+// One upper-level node list in LDS: entry e is node node[e] with parent
+// word pw[e], its parent's entry par[e] (kNone: the parent is not in the
+// list), the number of its children still pending, and its children's
+// {cm, aabb} records (left, right).
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+struct ListLds {
+    uint32_t node[kLdsList], pw[kLdsList], par[kLdsList], pend[kLdsList];
+    float4 in[kLdsList][4];
+};
+
+// Index of `x` in the sorted node[0, m), or kNone.
+__device__ __forceinline__ uint32_t find_node(const ListLds &ls, uint32_t m, uint32_t x) {
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ls.node[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < m && ls.node[lo] == x ? lo : kNone;
+}
+
+// Resolve a list set up in ls (m <= kLdsList entries, thread e owns entry
+// e).  An entry whose children are complete merges them (lbvh.wgsl:205-228),
+// hands the node to its parent entry through LDS (a workgroup-scope release
+// of the pending count) and publishes it.  No barriers: a level costs one
+// LDS hand-off.  The polls are bounded so that a malformed tree ends the
+// kernel (reported through *fault) instead of hanging it.
+__device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, uint32_t m) {
+    const uint32_t e = threadIdx.x;
+    bool done = e >= m;
+    for (uint32_t it = 0; it < (1u << 16); ++it) {
+        if (!done && __hip_atomic_load(&ls.pend[e], __ATOMIC_ACQUIRE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) {
+            float4 C, B;
+            merge_values(ls.in[e][0], ls.in[e][1], ls.in[e][2], ls.in[e][3], C, B);
+            const uint32_t pw = ls.pw[e];
+            const uint32_t q = ls.par[e];
+            if (q != kNone) {
+                const uint32_t side = pw >> 31;
+                ls.in[q][2 * side] = C;
+                ls.in[q][2 * side + 1] = B;
+                __hip_atomic_fetch_sub(&ls.pend[q], 1u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            publish(tb, ls.node[e], pw, C, B);
+            done = true;
+        }
+        if (__all(done)) return;
+    }
+    *tb.fault = 1u;
+}
+
+// Phases 2 and 3.  Workgroup sb scans the leaf ranges of node indices
+// [sb * kSuper, (sb+1) * kSuper): its phase-2 nodes (range inside the
+// super-block, across a block boundary) become an LDS list in index order,
+// its top nodes (range across super-blocks) go to the global top list.  The
+// phase-2 list is resolved in LDS (or, longer than lds_limit, by climbing
+// from the phase-1 frontier).  The workgroup that finishes last (a
+// device-scope fence orders each workgroup's writes before its count)
+// resolves the top list the same way (or climbs from the top nodes'
+// other children) and clears the counters.
+__global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBufs tb,
+                                                              uint32_t num_super,
+                                                              uint32_t lds_limit) {
+    __shared__ ListLds ls;
+    __shared__ uint32_t sh[kFillBlock / 64 + 1];
+    __shared__ uint32_t s_base, s_last;
+    const uint32_t t = threadIdx.x;
+    const uint32_t sb = blockIdx.x;
     const uint32_t nint = (uint32_t)(n - 1);
-    const uint32_t count = tb.counts[num_super];
-    for (uint32_t k = threadIdx.x; k < count; k += kFillBlock)
-        climb_wg(tb, nint, tb.top[k], 0u, nint, tb.top, &tb.counts[num_super]);
+    uint32_t *top_list = tb.lists;
+    uint32_t *top_count = tb.counts + num_super + 1;
+    const uint32_t top_cap = list3_cap(num_super);
+
+    // scan kScan consecutive node indices per thread
+    const uint32_t i0 = sb * (uint32_t)kSuper + t * kScan;
+    uint32_t bits2 = 0, bits3 = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScan; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < nint) {
+            const uint2 r = tb.range[i];
+            if (spans(r, kBlockShift)) {
+                if (spans(r, kSuperShift))
+                    bits3 |= 1u << k;
+                else
+                    bits2 |= 1u << k;
+            }
+        }
+    }
+    uint32_t m2, n3;
+    const uint32_t off2 = block_scan(__popc(bits2), sh, &m2);
+    const uint32_t off3 = block_scan(__popc(bits3), sh, &n3);
+    if (t == 0) s_base = n3 ? atomicAdd(top_count, n3) : 0u;
+    const bool lds2 = m2 <= lds_limit;
+    if (lds2) {
+        uint32_t o = off2;
+        for (uint32_t b = bits2; b; b &= b - 1) ls.node[o++] = i0 + __builtin_ctz(b);
+    }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k <= num_super; k += kFillBlock) tb.counts[k] = 0u;
+    {
+        uint32_t o = s_base + off3;
+        for (uint32_t b = bits3; b; b &= b - 1, ++o) {
+            const uint32_t i = i0 + __builtin_ctz(b);
+            if (o < top_cap) top_list[o] = i;
+            tb.slot[i] = o;
+        }
+    }
+
+    if (lds2) {
+        if (t < m2) {
+            const uint32_t p = ls.node[t];
+            const uint32_t pw = tb.parent[p];
+            const uint2 ch = tb.child[p];
+            const uint32_t par = pw & kParentMask;
+            // a parent spanning super-blocks is not in the list (not found)
+            ls.par[t] = par < nint ? find_node(ls, m2, par) : kNone;
+            uint32_t pend = 0;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t c = side ? ch.y : ch.x;
+                if (c < nint && find_node(ls, m2, c) != kNone) {
+                    ++pend;
+                } else {
+                    ls.in[t][2 * side] = tb.pcm[kPairStride * (size_t)p + side];
+                    ls.in[t][2 * side + 1] = tb.paabb[2 * (size_t)p + side];
+                }
+            }
+            ls.pw[t] = pw;
+            ls.pend[t] = pend;
+        }
+        __syncthreads();
+        resolve_list(tb, ls, m2);
+    } else {
+        const uint32_t lo = sb * (uint32_t)kSuper;
+        const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
+        const uint32_t count = tb.counts[sb];
+        for (uint32_t k = t; k < count; k += kFillBlock)
+            climb_wg(tb, nint, tb.frontier[(size_t)sb * kSuper + k], lo, hi);
+    }
+
+    __syncthreads();
+    if (t == 0) {
+        __threadfence();
+        s_last = atomicAdd(&tb.counts[num_super], 1u) == num_super - 1u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    const uint32_t m3 =
+        __hip_atomic_load(top_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (m3 <= lds_limit) {
+        if (t < m3) {
+            const uint32_t p = top_list[t];
+            const uint32_t pw = tb.parent[p];
+            const uint2 ch = tb.child[p];
+            const uint32_t par = pw & kParentMask;
+            // every ancestor of a top node is a top node
+            ls.par[t] = par < nint ? tb.slot[par] : kNone;
+            ls.node[t] = p;
+            uint32_t pend = 0;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t c = side ? ch.y : ch.x;
+                if (c < nint && spans(tb.range[c], kSuperShift)) {
+                    ++pend;
+                } else {
+                    ls.in[t][2 * side] = tb.pcm[kPairStride * (size_t)p + side];
+                    ls.in[t][2 * side + 1] = tb.paabb[2 * (size_t)p + side];
+                }
+            }
+            ls.pw[t] = pw;
+            ls.pend[t] = pend;
+        }
+        __syncthreads();
+        resolve_list(tb, ls, m3);
+    } else {
+        // climb from every child of a top node that is not a top node
+        const uint32_t cnt = m3 < top_cap ? m3 : top_cap;
+        for (uint32_t k = t; k < cnt; k += kFillBlock) {
+            const uint2 ch = tb.child[top_list[k]];
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t c = side ? ch.y : ch.x;
+                if (!(c < nint && spans(tb.range[c], kSuperShift))) climb_wg(tb, nint, c, 0u, nint);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = t; k < num_super + 2; k += kFillBlock) tb.counts[k] = 0u;
 }
 
 }  // namespace
@@ -347,18 +564,19 @@ void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, c
 }
 
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
-                 const TreeBufs &t, hipStream_t s) {
+                 const TreeBufs &t, bool climb, hipStream_t s) {
     if (n < 1) return;
     const unsigned blocks = (unsigned)((n + kFillBlock - 1) / kFillBlock);
     fill_local_kernel<<<blocks, kFillBlock, 0, s>>>(idx, mass, pos, n, t);
     if (n > kFillBlock) {
-        const uint32_t num_super = (uint32_t)((n + kSuper - 1) / kSuper);
-        fill_mid_kernel<<<num_super, kFillBlock, 0, s>>>(n, t, num_super);
-        fill_top_kernel<<<1, kFillBlock, 0, s>>>(n, t, num_super);
+        const uint32_t num_super = num_supers(n);
+        fill_mid_kernel<<<num_super, kFillBlock, 0, s>>>(n, t, num_super,
+                                                         climb ? 0u : kLdsList);
     }
 }
 
 int64_t fill_frontier_words(int64_t cap) { return ((cap + kSuper - 1) / kSuper) * kSuper; }
-int64_t fill_count_words(int64_t cap) { return (cap + kSuper - 1) / kSuper + 1; }
+int64_t fill_count_words(int64_t cap) { return (int64_t)num_supers(cap) + 2; }
+int64_t fill_list_words(int64_t cap) { return list3_cap(num_supers(cap)); }
 
 }  // namespace nbs

@@ -84,16 +84,19 @@ struct TreeBufs {
     float4 *paabb;       // [2(n-1)] children's {min.x, min.y, max.x, max.y}
     float4 *root;        // [2] root {cm, aabb}
     uint32_t *frontier;  // fill phase 1 -> 2 hand-off, bucketed by super-block
-    uint32_t *top;       // fill phase 2 -> 3 hand-off
-    uint32_t *counts;    // [super-blocks + 1] bucket sizes, then the top list's
+    uint32_t *counts;    // frontier, done and top-list counters (fill_count_words)
+    uint32_t *lists;     // the fill's top node list (fill_list_words)
+    uint32_t *slot;      // [n-1] position of a top node in the top list
+    uint32_t *fault;     // set when the fill cannot complete (malformed tree)
 };
 void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, hipStream_t s);
 void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
                       uint32_t *sim_root, hipStream_t s);
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
-                 const TreeBufs &t, hipStream_t s);
-int64_t fill_frontier_words(int64_t cap);  // sizes of frontier / top / counts for `cap` bodies
+                 const TreeBufs &t, bool climb, hipStream_t s);
+int64_t fill_frontier_words(int64_t cap);  // sizes of frontier / counts / list for `cap` bodies
 int64_t fill_count_words(int64_t cap);
+int64_t fill_list_words(int64_t cap);
 
 // ---------------------------------------------------------------- walk
 struct WalkArgs {

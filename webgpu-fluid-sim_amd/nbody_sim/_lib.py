@@ -29,6 +29,7 @@ MATH_EXACT = 1
 
 DEBUG_ACCEL = 1
 DEBUG_STATS = 2
+DEBUG_FILL_CLIMB = 4
 
 BUF_KEYS = 1
 BUF_BODY_INDICES = 2

@@ -108,8 +108,11 @@ class _Handle:
     def useGraph(self, enable: bool = True):
         self._check(_lib.lib().nbody_use_graph(self.handle, int(bool(enable))))
 
-    def setDebug(self, accel: bool = False, stats: bool = False):
-        flags = (_lib.DEBUG_ACCEL if accel else 0) | (_lib.DEBUG_STATS if stats else 0)
+    def setDebug(self, accel: bool = False, stats: bool = False, fill_climb: bool = False):
+        """Record accelerations / walk statistics; fill_climb forces the tree
+        fill's global-counter climb for its upper levels (a test hook)."""
+        flags = ((_lib.DEBUG_ACCEL if accel else 0) | (_lib.DEBUG_STATS if stats else 0)
+                 | (_lib.DEBUG_FILL_CLIMB if fill_climb else 0))
         self._check(_lib.lib().nbody_set_debug(self.handle, flags))
 
     def getTuning(self) -> dict:
