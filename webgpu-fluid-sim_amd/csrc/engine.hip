@@ -201,8 +201,7 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->sim_root = a.take<uint32_t>(s->num_sims);
         s->accel = a.take<float2>(cap);
         s->stats = a.take<unsigned long long>(4);
-        s->overflow = a.take<uint32_t>(2);  // [0] walk stack overflow, [1] tree fill fault
-        s->tree.fault = s->overflow + 1;
+        s->overflow = a.take<uint32_t>(1);
         if (pass == 0) {
             a.size = a.used;
             void *p = nullptr;
@@ -219,7 +218,7 @@ int allocate(nbody_sim *s, int64_t cap) {
     sort_bind_scratch(s->sort, s->sort_words, cap, s->passes);
     s->cap = cap;
     HIP_TRY(s, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->stream));
-    HIP_TRY(s, hipMemsetAsync(s->overflow, 0, 2 * sizeof(uint32_t), s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->overflow, 0, sizeof(uint32_t), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->tree.counts, 0, 4 * (size_t)fill_count_words(cap), s->stream));
     return NBODY_OK;
 }
@@ -586,12 +585,10 @@ int nbody_synchronize(nbody_sim *s) {
     if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
     HIP_TRY(s, hipSetDevice(s->device));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
-    uint32_t of[2] = {0, 0};
-    HIP_TRY(s, hipMemcpy(of, s->overflow, sizeof(of), hipMemcpyDeviceToHost));
-    if (of[0] || of[1]) {
-        HIP_TRY(s, hipMemset(s->overflow, 0, sizeof(of)));
-        if (of[1])
-            return fail(s, NBODY_ERR_STATE, "tree fill did not complete: malformed tree");
+    uint32_t of = 0;
+    HIP_TRY(s, hipMemcpy(&of, s->overflow, sizeof(of), hipMemcpyDeviceToHost));
+    if (of) {
+        HIP_TRY(s, hipMemset(s->overflow, 0, sizeof(uint32_t)));
         return fail(s, NBODY_ERR_STACK_OVERFLOW,
                     "tree walk stack overflow: nodes were dropped (raise walk_stack)");
     }

@@ -378,35 +378,31 @@ __device__ __forceinline__ uint32_t find_node(const ListLds &ls, uint32_t m, uin
     return lo < m && ls.node[lo] == x ? lo : kNone;
 }
 
-// Resolve a list set up in ls (m <= kLdsList entries, thread e owns entry
-// e).  An entry whose children are complete merges them (lbvh.wgsl:205-228),
-// hands the node to its parent entry through LDS (a workgroup-scope release
-// of the pending count) and publishes it.  No barriers: a level costs one
-// LDS hand-off.  The polls are bounded so that a malformed tree ends the
-// kernel (reported through *fault) instead of hanging it.
+// Resolve a list set up in ls (m <= kLdsList entries).  The entries with
+// no pending child start a climb in LDS: merge the children
+// (lbvh.wgsl:205-228), publish the node, hand it to the parent entry's
+// input and count the parent's arrival (workgroup-scope acquire/release);
+// the child that arrives last climbs on, the other stops.  No barriers and
+// no waiting: a level costs a merge and one LDS atomic.
 __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, uint32_t m) {
     const uint32_t e = threadIdx.x;
-    bool done = e >= m;
-    for (uint32_t it = 0; it < (1u << 16); ++it) {
-        if (!done && __hip_atomic_load(&ls.pend[e], __ATOMIC_ACQUIRE,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) {
-            float4 C, B;
-            merge_values(ls.in[e][0], ls.in[e][1], ls.in[e][2], ls.in[e][3], C, B);
-            const uint32_t pw = ls.pw[e];
-            const uint32_t q = ls.par[e];
-            if (q != kNone) {
-                const uint32_t side = pw >> 31;
-                ls.in[q][2 * side] = C;
-                ls.in[q][2 * side + 1] = B;
-                __hip_atomic_fetch_sub(&ls.pend[q], 1u, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-            publish(tb, ls.node[e], pw, C, B);
-            done = true;
-        }
-        if (__all(done)) return;
+    if (e >= m || ls.pend[e] != 0u) return;
+    uint32_t cur = e;
+    for (int level = 0; level < 128; ++level) {
+        float4 C, B;
+        merge_values(ls.in[cur][0], ls.in[cur][1], ls.in[cur][2], ls.in[cur][3], C, B);
+        const uint32_t pw = ls.pw[cur];
+        const uint32_t q = ls.par[cur];
+        publish(tb, ls.node[cur], pw, C, B);
+        if (q == kNone) break;
+        const uint32_t side = pw >> 31;
+        ls.in[q][2 * side] = C;
+        ls.in[q][2 * side + 1] = B;
+        if (__hip_atomic_fetch_sub(&ls.pend[q], 1u, __ATOMIC_ACQ_REL,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP) != 1u)
+            break;  // the sibling is still pending: it climbs on
+        cur = q;
     }
-    *tb.fault = 1u;
 }
 
 // Phases 2 and 3.  Workgroup sb scans the leaf ranges of node indices
@@ -431,20 +427,20 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     uint32_t *top_count = tb.counts + num_super + 1;
     const uint32_t top_cap = list3_cap(num_super);
 
-    // scan kScan consecutive node indices per thread
+    // scan kScan consecutive node indices per thread: every load is issued
+    // before any is used (indices clamped into the array)
     const uint32_t i0 = sb * (uint32_t)kSuper + t * kScan;
+    uint2 r[kScan];
+#pragma unroll
+    for (uint32_t k = 0; k < kScan; ++k) r[k] = tb.range[min(i0 + k, nint - 1u)];
     uint32_t bits2 = 0, bits3 = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScan; ++k) {
-        const uint32_t i = i0 + k;
-        if (i < nint) {
-            const uint2 r = tb.range[i];
-            if (spans(r, kBlockShift)) {
-                if (spans(r, kSuperShift))
-                    bits3 |= 1u << k;
-                else
-                    bits2 |= 1u << k;
-            }
+        if (i0 + k < nint && spans(r[k], kBlockShift)) {
+            if (spans(r[k], kSuperShift))
+                bits3 |= 1u << k;
+            else
+                bits2 |= 1u << k;
         }
     }
     uint32_t m2, n3;

@@ -87,7 +87,6 @@ struct TreeBufs {
     uint32_t *counts;    // frontier, done and top-list counters (fill_count_words)
     uint32_t *lists;     // the fill's top node list (fill_list_words)
     uint32_t *slot;      // [n-1] position of a top node in the top list
-    uint32_t *fault;     // set when the fill cannot complete (malformed tree)
 };
 void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, hipStream_t s);
 void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
