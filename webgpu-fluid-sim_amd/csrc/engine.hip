@@ -9,10 +9,10 @@
 //   mass f32[C], pos float2[C], vel float2[C]      -- by body index
 //   keys[2] u32|u64[C], vals[2] u32[C]             -- sort ping-pong
 //   leaf float4[C] {x, y, mass, 0} in sorted order   -- the walk's own bodies
-//   pcm float4[2C] children's {com, mass, length} per internal node (walk)
+//   pcm float4[4C] pair records: children's {com, mass, length} + indices (walk)
 //   paabb float4[2C] children's AABBs per internal node (fill)
 //   root float4[2], child uint2[C], range uint2[C], parent u32[2C-1], status u32[C]
-//   fill hand-off lists u32[2C] + counts, sort scratch, bbox partials, debug accel float2[C],
+//   fill counters + top list + slots u32[C], sort scratch, bbox partials, debug accel float2[C],
 //   counters
 #include <hip/hip_runtime.h>
 
@@ -189,7 +189,6 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->tree.range = a.take<uint2>(cap);
         s->tree.parent = a.take<uint32_t>(nodes);
         s->tree.status = a.take<uint32_t>(cap);
-        s->tree.frontier = a.take<uint32_t>(fill_frontier_words(cap));
         s->tree.counts = a.take<uint32_t>(fill_count_words(cap));
         s->tree.lists = a.take<uint32_t>(fill_list_words(cap));
         s->tree.slot = a.take<uint32_t>(cap);

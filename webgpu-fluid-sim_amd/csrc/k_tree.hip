@@ -12,20 +12,19 @@
 //
 //   Agent-scope acquire/release removes the race but, on a multi-XCD MI355X,
 //   writes back and invalidates the XCD's L2 at every counter update; done by
-//   every thread at every level it costs milliseconds.  The climb is split in
-//   three phases so that every counter is ordered at workgroup scope:
+//   every thread at every level it costs milliseconds.  The fill is split in
+//   phases whose ordering is at workgroup scope:
 //     phase 1 (fill_local_kernel): workgroup b owns sorted leaves
-//       [b*1024, b*1024+1023].  It loads its slice of the child, range and
-//       parent arrays into LDS, completes every node whose leaf range lies in
-//       the block with LDS-resident node data and LDS arrival counters, then
-//       writes all completed nodes out cooperatively.  A node whose parent
-//       spans blocks goes to its super-block's (32 blocks) frontier bucket.
-//     phase 2 (fill_mid_kernel): one workgroup per super-block climbs from
-//       its bucket through the nodes inside the super-block (global arrival
-//       counters, workgroup scope); nodes whose parent spans super-blocks go
-//       to the top list.
-//     phase 3 (the last fill_mid workgroup to finish): climbs the top list
-//       to the root.
+//       [b*1024, b*1024+1023].  It loads its slice of the child and parent
+//       arrays into LDS and completes every node whose leaf range lies in
+//       the block (the build flags their children's parent words with
+//       kInBlock) with LDS-resident node data and LDS arrival counters, then
+//       writes all completed nodes out cooperatively.
+//     phase 2 (fill_mid_kernel): one workgroup per super-block of 32 blocks
+//       finds, among its node indices, the nodes whose range lies in the
+//       super-block across a block boundary, and completes them by LDS
+//       climbs; it also lists the top nodes (range across super-blocks).
+//     phase 3 (the last fill_mid workgroup to finish): completes the top.
 //   A kernel boundary orders phases 1 and 2; one device-scope fence per
 //   phase-2 workgroup orders phases 2 and 3.
 //   The merge formula does not depend on which child arrives last, so the
@@ -90,12 +89,16 @@ __host__ __device__ inline uint32_t list3_cap(uint32_t num_super) { return num_s
 __host__ __device__ inline uint32_t num_supers(int64_t n) {
     return (uint32_t)((n + kSuper - 1) / kSuper);
 }
-// Counter words (tb.counts): [0, S) phase-1 frontier buckets, [S] phase-2
-// done count, [S+1] top list length; S = number of super-blocks.  The top
-// list (every node whose range spans super-blocks, with tb.slot) is
-// tb.lists[0, list3_cap(S)).
+// Counter words (tb.counts): [0] phase-2 done count, [1] top list length.
+// The top list (every node whose range spans super-blocks; tb.slot holds
+// each one's position) is tb.lists[0, list3_cap(S)), S = super-blocks.
 
 __device__ __forceinline__ bool spans(uint2 r, int shift) { return (r.x >> shift) != (r.y >> shift); }
+
+// kInBlock when the node's leaf range lies in one phase-1 block.
+__device__ __forceinline__ uint32_t in_block(const Split &sp) {
+    return ((uint64_t)sp.lo >> kBlockShift) == ((uint64_t)sp.hi >> kBlockShift) ? kInBlock : 0u;
+}
 
 template <typename Delta>
 __device__ __forceinline__ Split karras_node(const Delta &delta, int64_t i, int64_t n) {
@@ -135,8 +138,10 @@ __global__ __launch_bounds__(256) void build_u32_kernel(const uint32_t *__restri
     // the walk reads the indices from the pair record
     *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
     tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
-    tb.parent[sp.lc] = (uint32_t)i;  // lbvh.wgsl:155-161
-    tb.parent[sp.rc] = (uint32_t)i | kRightChild;
+    // lbvh.wgsl:155-161, with the fill's in-block flag
+    const uint32_t pw = (uint32_t)i | in_block(sp);
+    tb.parent[sp.lc] = pw;
+    tb.parent[sp.rc] = pw | kRightChild;
     if (i == 0) tb.parent[0] = kNoNode;
     tb.status[i] = 0u;  // lbvh.wgsl:164
 }
@@ -154,8 +159,9 @@ __global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restri
     // the walk reads the indices from the pair record
     *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
     tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
-    tb.parent[sp.lc] = (uint32_t)i;
-    tb.parent[sp.rc] = (uint32_t)i | kRightChild;
+    const uint32_t pw = (uint32_t)i | in_block(sp);
+    tb.parent[sp.lc] = pw;
+    tb.parent[sp.rc] = pw | kRightChild;
     tb.status[i] = 0u;
     // A node spanning two sims sits above the sim level; a child whose
     // range lies inside one sim is that sim's root.
@@ -215,56 +221,43 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
                                                                 const float *__restrict__ mass,
                                                                 const float2 *__restrict__ pos,
                                                                 int64_t n, TreeBufs tb) {
-    __shared__ uint2 s_child[kFillBlock];     // internal node base+j
-    __shared__ uint2 s_range[kFillBlock];
+    __shared__ uint2 s_child[kFillBlock];  // internal node base+j
     __shared__ uint32_t s_par_int[kFillBlock];
-    __shared__ uint32_t s_par_leaf[kFillBlock];  // leaf of sorted position base+j
     __shared__ float4 s_cm_int[kFillBlock];
     __shared__ float4 s_bb_int[kFillBlock];
     __shared__ float4 s_cm_leaf[kFillBlock];
-    __shared__ uint32_t s_arrive[kFillBlock];
-    __shared__ uint32_t s_done[kFillBlock];
+    __shared__ uint32_t s_arrive[kFillBlock];  // 2: the node is complete
 
     const uint32_t j = threadIdx.x;
     const uint32_t base = blockIdx.x * kFillBlock;
     const uint32_t nint = (uint32_t)(n - 1);  // internal nodes
-    const uint32_t last = base + kFillBlock - 1 < (uint32_t)(n - 1) ? base + kFillBlock - 1
-                                                                   : (uint32_t)(n - 1);
     const uint32_t t = base + j;
     s_arrive[j] = 0u;
-    s_done[j] = 0u;
     if (t < nint) {
         s_child[j] = tb.child[t];
-        s_range[j] = tb.range[t];
         s_par_int[j] = tb.parent[t];
     }
-    float4 C, B;
+    uint32_t pw_leaf = kNoNode;
     if (t < (uint64_t)n) {
-        s_par_leaf[j] = n > 1 ? tb.parent[nint + t] : kNoNode;
+        pw_leaf = n > 1 ? tb.parent[nint + t] : kNoNode;
         const uint32_t b = idx[t];
         const float2 p = pos[b];
         // lbvh.wgsl:179-187
-        C = make_float4(p.x, p.y, mass[b], 0.0f);
-        B = make_float4(p.x, p.y, p.x, p.y);
+        const float4 C = make_float4(p.x, p.y, mass[b], 0.0f);
         s_cm_leaf[j] = C;
         tb.leaf[t] = C;
     }
     __syncthreads();
 
-    // lbvh.wgsl:191-232, restricted to the nodes inside this block.
+    // lbvh.wgsl:191-232, restricted to the nodes inside this block: a node
+    // whose parent word lacks kInBlock is a top of the block's forest.
     if (t < (uint64_t)n && n > 1) {
-        uint32_t cur = nint + t;
-        uint32_t pw = s_par_leaf[j];
+        uint32_t pw = pw_leaf;
         for (int level = 0; level < 128; ++level) {
+            if (!(pw & kInBlock)) break;
             const uint32_t par = pw & kParentMask;
-            if (par >= nint) break;  // corrupt parent word
             const uint32_t lp = par - base;
-            if (par < base || par > last || s_range[lp].x < base || s_range[lp].y > last) {
-                const uint32_t sb = base / kSuper;
-                const uint32_t slot = atomicAdd(&tb.counts[sb], 1u);
-                tb.frontier[(size_t)sb * kSuper + slot] = cur;
-                break;
-            }
+            if (lp >= (uint32_t)kFillBlock) break;  // corrupt parent word
             const uint32_t prev = __hip_atomic_fetch_add(&s_arrive[lp], 1u, __ATOMIC_ACQ_REL,
                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             if (prev == 0u) break;
@@ -284,12 +277,11 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
                 R = s_cm_int[ch.y - base];
                 RB = s_bb_int[ch.y - base];
             }
+            float4 C, B;
             merge_values(L, LB, R, RB, C, B);
             s_cm_int[lp] = C;
             s_bb_int[lp] = B;
-            s_done[lp] = 1u;
-            cur = par;
-            if (cur == 0u) break;
+            if (par == 0u) break;
             pw = s_par_int[lp];
         }
     }
@@ -302,9 +294,10 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
         if (n == 1)
             publish(tb, 0u, 0u, c, make_float4(c.x, c.y, c.x, c.y));
         else
-            publish(tb, nint + t, s_par_leaf[j], c, make_float4(c.x, c.y, c.x, c.y));
+            publish(tb, nint + t, pw_leaf, c, make_float4(c.x, c.y, c.x, c.y));
     }
-    if (t < nint && s_done[j]) publish(tb, t, t ? s_par_int[j] : 0u, s_cm_int[j], s_bb_int[j]);
+    if (t < nint && s_arrive[j] == 2u)
+        publish(tb, t, t ? s_par_int[j] : 0u, s_cm_int[j], s_bb_int[j]);
 }
 
 // Climb from `cur` (its data already published) while the parents' leaf
@@ -410,7 +403,7 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
 // super-block, across a block boundary) become an LDS list in index order,
 // its top nodes (range across super-blocks) go to the global top list.  The
 // phase-2 list is resolved in LDS (or, longer than lds_limit, by climbing
-// from the phase-1 frontier).  The workgroup that finishes last (a
+// from the phase-2 nodes' other children).  The workgroup that finishes last (a
 // device-scope fence orders each workgroup's writes before its count)
 // resolves the top list the same way (or climbs from the top nodes'
 // other children) and clears the counters.
@@ -424,7 +417,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     const uint32_t sb = blockIdx.x;
     const uint32_t nint = (uint32_t)(n - 1);
     uint32_t *top_list = tb.lists;
-    uint32_t *top_count = tb.counts + num_super + 1;
+    uint32_t *top_count = tb.counts + 1;
     const uint32_t top_cap = list3_cap(num_super);
 
     // scan kScan consecutive node indices per thread: every load is issued
@@ -487,17 +480,23 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
         __syncthreads();
         resolve_list(tb, ls, m2);
     } else {
+        // climb from every child of a phase-2 node that is not one itself
         const uint32_t lo = sb * (uint32_t)kSuper;
         const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
-        const uint32_t count = tb.counts[sb];
-        for (uint32_t k = t; k < count; k += kFillBlock)
-            climb_wg(tb, nint, tb.frontier[(size_t)sb * kSuper + k], lo, hi);
+        for (uint32_t b = bits2; b; b &= b - 1) {
+            const uint2 ch = tb.child[i0 + __builtin_ctz(b)];
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t c = side ? ch.y : ch.x;
+                if (!(c < nint && spans(tb.range[c], kBlockShift))) climb_wg(tb, nint, c, lo, hi);
+            }
+        }
     }
 
     __syncthreads();
     if (t == 0) {
         __threadfence();
-        s_last = atomicAdd(&tb.counts[num_super], 1u) == num_super - 1u;
+        s_last = atomicAdd(&tb.counts[0], 1u) == num_super - 1u;
     }
     __syncthreads();
     if (!s_last) return;
@@ -542,7 +541,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
         }
     }
     __syncthreads();
-    for (uint32_t k = t; k < num_super + 2; k += kFillBlock) tb.counts[k] = 0u;
+    if (t < 2) tb.counts[t] = 0u;
 }
 
 }  // namespace
@@ -571,8 +570,7 @@ void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int6
     }
 }
 
-int64_t fill_frontier_words(int64_t cap) { return ((cap + kSuper - 1) / kSuper) * kSuper; }
-int64_t fill_count_words(int64_t cap) { return (int64_t)num_supers(cap) + 2; }
+int64_t fill_count_words(int64_t) { return 2; }
 int64_t fill_list_words(int64_t cap) { return list3_cap(num_supers(cap)); }
 
 }  // namespace nbs

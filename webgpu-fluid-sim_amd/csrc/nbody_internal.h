@@ -59,13 +59,16 @@ int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32
 // in paabb[2p], paabb[2p+1].  Opening a node therefore fetches everything
 // the walk needs from one aligned 64-byte line.  The root, which is
 // nobody's child, has its own record.
-constexpr uint32_t kRightChild = 0x80000000u;  // parent word: index | side << 31
+constexpr uint32_t kRightChild = 0x80000000u;  // parent word: index | flags | side << 31
 // Length field of a leaf's node record.  Leaves are accepted whatever the
 // opening test says (barnes_hut.wgsl:89); with -inf as their length, the
 // walks' test passes for every leaf, so they need no leaf check.  The
 // reference's leaf length is 0 (the host reports it so).
 constexpr float kLeafLength = -__builtin_huge_valf();
-constexpr uint32_t kParentMask = 0x7FFFFFFFu;
+// Parent word flag: the parent's leaf range lies in one 1024-leaf block
+// (the fill completes it in its first phase).  Parent indices are < 2^30.
+constexpr uint32_t kInBlock = 0x40000000u;
+constexpr uint32_t kParentMask = 0x3FFFFFFFu;
 constexpr int kPairStride = 4;  // float4s per pair record
 // Slot of a node's record in pcm / in paabb, from its parent word.
 __host__ __device__ inline size_t pcm_slot(uint32_t pw) {
@@ -83,8 +86,7 @@ struct TreeBufs {
     float4 *pcm;         // [4(n-1)] pair records (children's {com, mass, length}, indices)
     float4 *paabb;       // [2(n-1)] children's {min.x, min.y, max.x, max.y}
     float4 *root;        // [2] root {cm, aabb}
-    uint32_t *frontier;  // fill phase 1 -> 2 hand-off, bucketed by super-block
-    uint32_t *counts;    // frontier, done and top-list counters (fill_count_words)
+    uint32_t *counts;    // fill counters (fill_count_words)
     uint32_t *lists;     // the fill's top node list (fill_list_words)
     uint32_t *slot;      // [n-1] position of a top node in the top list
 };
@@ -93,8 +95,7 @@ void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, c
                       uint32_t *sim_root, hipStream_t s);
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
                  const TreeBufs &t, bool climb, hipStream_t s);
-int64_t fill_frontier_words(int64_t cap);  // sizes of frontier / counts / list for `cap` bodies
-int64_t fill_count_words(int64_t cap);
+int64_t fill_count_words(int64_t cap);  // sizes of counts / top list for `cap` bodies
 int64_t fill_list_words(int64_t cap);
 
 // ---------------------------------------------------------------- walk
