@@ -412,6 +412,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
                                                               uint32_t lds_limit) {
     __shared__ ListLds ls;
     __shared__ uint32_t sh[kFillBlock / 64 + 1];
+    __shared__ uint32_t s_cnt[kScan * 16];
     __shared__ uint32_t s_base, s_last;
     const uint32_t t = threadIdx.x;
     const uint32_t sb = blockIdx.x;
@@ -420,40 +421,60 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     uint32_t *top_count = tb.counts + 1;
     const uint32_t top_cap = list3_cap(num_super);
 
-    // scan kScan consecutive node indices per thread: every load is issued
-    // before any is used (indices clamped into the array)
-    const uint32_t i0 = sb * (uint32_t)kSuper + t * kScan;
+    // scan: node index base_i + k * kFillBlock + t (coalesced), every load
+    // issued before any is used (indices clamped into the array)
+    const uint32_t lane = t & 63, w = t >> 6;
+    const uint32_t base_i = sb * (uint32_t)kSuper;
     uint2 r[kScan];
 #pragma unroll
-    for (uint32_t k = 0; k < kScan; ++k) r[k] = tb.range[min(i0 + k, nint - 1u)];
+    for (uint32_t k = 0; k < kScan; ++k) r[k] = tb.range[min(base_i + k * kFillBlock + t, nint - 1u)];
     uint32_t bits2 = 0, bits3 = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScan; ++k) {
-        if (i0 + k < nint && spans(r[k], kBlockShift)) {
+        if (base_i + k * kFillBlock + t < nint && spans(r[k], kBlockShift)) {
             if (spans(r[k], kSuperShift))
                 bits3 |= 1u << k;
             else
                 bits2 |= 1u << k;
         }
     }
-    uint32_t m2, n3;
-    const uint32_t off2 = block_scan(__popc(bits2), sh, &m2);
-    const uint32_t off3 = block_scan(__popc(bits3), sh, &n3);
-    if (t == 0) s_base = n3 ? atomicAdd(top_count, n3) : 0u;
-    const bool lds2 = m2 <= lds_limit;
-    if (lds2) {
-        uint32_t o = off2;
-        for (uint32_t b = bits2; b; b &= b - 1) ls.node[o++] = i0 + __builtin_ctz(b);
+    // list positions in index order: counts per (k, wave), phase-2 in the
+    // low and top in the high half-word, one scan
+#pragma unroll
+    for (uint32_t k = 0; k < kScan; ++k) {
+        const uint64_t b2 = __ballot((bits2 >> k) & 1u), b3 = __ballot((bits3 >> k) & 1u);
+        if (lane == 0) s_cnt[k * 16 + w] = __popcll(b2) | (uint32_t)__popcll(b3) << 16;
     }
     __syncthreads();
-    {
-        uint32_t o = s_base + off3;
-        for (uint32_t b = bits3; b; b &= b - 1, ++o) {
-            const uint32_t i = i0 + __builtin_ctz(b);
-            if (o < top_cap) top_list[o] = i;
-            tb.slot[i] = o;
+    const uint32_t v = t < kScan * 16 ? s_cnt[t] : 0u;
+    uint32_t total;
+    const uint32_t excl = block_scan(v, sh, &total);
+    if (t < kScan * 16) s_cnt[t] = excl;
+    const uint32_t m2 = total & 0xFFFFu, n3 = total >> 16;
+    if (t == 0) s_base = n3 ? atomicAdd(top_count, n3) : 0u;
+    const bool lds2 = m2 <= lds_limit;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kScan; ++k) {
+        const uint64_t b2 = __ballot((bits2 >> k) & 1u), b3 = __ballot((bits3 >> k) & 1u);
+        const uint32_t o = s_cnt[k * 16 + w];
+        const uint32_t i = base_i + k * kFillBlock + t;
+        const uint32_t below = __builtin_amdgcn_mbcnt_lo((uint32_t)b2, 0u);
+        if ((bits2 >> k) & 1u) {
+            if (lds2)
+                ls.node[(o & 0xFFFFu) +
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(b2 >> 32), below)] = i;
+        }
+        if ((bits3 >> k) & 1u) {
+            const uint32_t pos =
+                s_base + (o >> 16) +
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(b3 >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)b3, 0u));
+            if (pos < top_cap) top_list[pos] = i;
+            tb.slot[i] = pos;
         }
     }
+    __syncthreads();
 
     if (lds2) {
         if (t < m2) {
@@ -484,7 +505,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
         const uint32_t lo = sb * (uint32_t)kSuper;
         const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
         for (uint32_t b = bits2; b; b &= b - 1) {
-            const uint2 ch = tb.child[i0 + __builtin_ctz(b)];
+            const uint2 ch = tb.child[base_i + __builtin_ctz(b) * kFillBlock + t];
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
                 const uint32_t c = side ? ch.y : ch.x;
