@@ -36,9 +36,24 @@ DT = 5e-5
 EPS = 1e-3
 SEED = 1
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
+PMC_PROFILE = os.path.join(ROOT, "profiles", "step_800k_pmc.md")
 PEAK_HBM_GBS = 8000.0
 FLOPS_ACCEPT = 18          # per accepted interaction (constant softening, rsqrt = 1)
 FLOPS_OPEN = 9             # per opened node
+
+
+def walk_traffic():
+    """HBM bytes per walk launch at this workload, from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration written there); (None, None) if the summary is absent."""
+    try:
+        with open(PMC_PROFILE) as f:
+            for line in f:
+                if line.startswith("walk_rs_kernel<false, false, false, false, 1> HBM bytes per launch:"):
+                    return float(line.rsplit(":", 1)[1]), os.path.relpath(PMC_PROFILE, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def log(msg):
@@ -236,6 +251,7 @@ def main():
     achieved = flops / (walk_ms * 1e-3) / 1e12
     log(f"phases ms/step: {phase_avg}; walk flops/launch {flops:.3e}")
 
+    traffic, traffic_src = walk_traffic()
     out = {
         "metric": METRIC,
         "value": value,
@@ -255,13 +271,14 @@ def main():
                    "math_mode": "fast", "graph": True},
         "phases_ms": {k: round(v, 5) for k, v in phase_avg.items()},
         "roofline": {
-            "kernel": "walk_kernel (tree walk + kick + drift)",
+            "kernel": "walk_rs_kernel (tree walk + kick + drift)",
             "bound": "valu-fp32",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "flops_per_launch": flops,
             "walk_ms": walk_ms,
             "interactions_per_step": st["accepted"] / args.profile_steps,
