@@ -148,17 +148,24 @@ int nbody_walk_stats(nbody_sim *sim, uint64_t *out3, int32_t reset);
 int nbody_set_profiling(nbody_sim *sim, int32_t enable);
 int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t reset);
 
-/* Launch tunables.  Every setting computes the same result bit for bit.
+/* Launch tunables.  Block size, stack size, XCD order and bodies per lane
+ * never change a result.  Variants 0 and 1 add every body's terms in the
+ * reference's order and agree bit for bit; variant 2 takes the same
+ * decisions and adds the same terms in its own fixed order (fast mode only;
+ * in the exact mode it runs variant 1).
  *   walk_block   walk workgroup size (64..1024, multiple of 64)
  *   walk_stack   walk stack entries per wavefront (8..256; the register-stack
- *                walks use one or two rows of 64: [64, 128])
- *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes (default)
+ *                walks use one or two rows of 64: [64, 128]; the pair walk
+ *                always has 63)
+ *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes; 2: pair walk, both
+ *                children of an opened node tested per iteration, stack of
+ *                (node, mask) in VGPR lanes (default)
  *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
  *                of Morton-ordered bodies; 0: hardware order (default)
  *   lane_bodies  bodies per lane of the VGPR-stack walk and the adjoint walk:
  *                1 (default), 2 or 4; a wavefront walks 64 * lane_bodies
- *                Morton-consecutive bodies together (the LDS walk ignores it,
- *                walking one body per lane) */
+ *                Morton-consecutive bodies together (the LDS and pair walks
+ *                ignore it, walking one body per lane) */
 typedef struct nbody_tuning {
     int32_t walk_block;
     int32_t walk_stack;

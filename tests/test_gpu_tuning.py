@@ -1,24 +1,26 @@
-"""Launch tunables (walk variant, workgroup size, stack depth, XCD order,
-bodies per lane) never change results, and the autotuner picks a
-configuration that runs."""
+"""Launch tunables (workgroup size, stack depth, XCD order, bodies per lane)
+never change results; the walks that keep the reference's term order
+(variants 0 and 1) agree bit for bit, the pair walk (variant 2, fast mode)
+takes the same decisions and sums the same terms in its own fixed order; and
+the autotuner picks a configuration that runs."""
 import numpy as np
 import pytest
 
 import nbody_sim
-from helpers import params
+from helpers import params, rel_err
 from nbody_sim.autotune import autotune
 
 pytestmark = pytest.mark.gpu
 
 
-def run(cfg, bodies, tuning, steps=3, ensemble_sizes=None):
+def run(cfg, bodies, tuning, steps=3, ensemble_sizes=None, stats=True):
     if ensemble_sizes is None:
         sim = nbody_sim.Simulation(cfg)
     else:
         sim = nbody_sim.Ensemble(cfg, ensemble_sizes)
     sim.setBodies(*bodies)
     sim.setTuning(*tuning)
-    sim.setDebug(accel=True, stats=True)
+    sim.setDebug(accel=True, stats=stats)
     sim.step(steps)
     sim.synchronize()
     acc = sim.debugBuffer("accel")
@@ -41,6 +43,54 @@ def test_walk_variants_identical(math):
         assert np.array_equal(got[0], ref[0]), tuning
         assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), tuning
         assert got[3]["accepted"] == ref[3]["accepted"] and got[3]["opened"] == ref[3]["opened"]
+
+
+def test_pair_walk_same_decisions_own_order():
+    """Fast mode: the pair walk (variant 2) takes every decision of the
+    reference-order walk (identical counters) and its sums agree to
+    rounding; its own order is fixed, so every launch setting and leaving
+    the own leaf in as an exact zero (no counters) give identical bits."""
+    bodies = nbody_sim.generate("exp-disk", 30000, seed=12, dispersion=0.05, mass_sigma=0.5)
+    cfg, _ = params(theta=0.7, dt=5e-5)
+    cfg = cfg.with_(mathMode="fast")
+    ref = run(cfg, bodies, (256, 128, 1, 0, 1), steps=1)
+    pair = run(cfg, bodies, (256, 128, 2, 0, 1), steps=1)
+    assert pair[3]["accepted"] == ref[3]["accepted"] and pair[3]["opened"] == ref[3]["opened"]
+    assert pair[3]["overflowed"] == 0
+    e = rel_err(pair[0], ref[0])
+    print(f"pair vs reference order: median rel {np.median(e):.2e}, max {e.max():.2e}")
+    # the same terms in another order: rounding-level differences only
+    assert np.median(e) < 1e-6 and np.mean(e <= 1e-5) >= 0.999
+    for tuning in [(64, 64, 2, 1, 1), (1024, 128, 2, 0, 4), (512, 64, 2, 1, 2)]:
+        got = run(cfg, bodies, tuning, steps=1)
+        assert np.array_equal(got[0], pair[0]), tuning
+        assert np.array_equal(got[1], pair[1]) and np.array_equal(got[2], pair[2]), tuning
+    # without counters the own leaf is accepted as an exact zero term
+    quiet = run(cfg, bodies, (256, 128, 2, 0, 1), steps=1, stats=False)
+    assert np.array_equal(quiet[0], pair[0]) and np.array_equal(quiet[1], pair[1])
+    # the exact mode keeps the reference order: variant 2 runs the register-stack walk
+    ex1 = run(cfg.with_(mathMode="exact"), bodies, (256, 128, 1, 0, 1), steps=1)
+    ex2 = run(cfg.with_(mathMode="exact"), bodies, (256, 128, 2, 0, 1), steps=1)
+    assert np.array_equal(ex1[0], ex2[0]) and np.array_equal(ex1[1], ex2[1])
+
+
+def test_pair_walk_reference_softening_and_ensemble():
+    """Variant 2 in the reference softening mode (own-leaf compare kept) and
+    over an ensemble: the same decisions as the reference-order walk."""
+    bodies = nbody_sim.generate("two-clusters", 5000, seed=3)
+    cfg, _ = params(theta=0.6, dt=0.1 / 60, soft="reference", morton="reference")
+    a = run(cfg, bodies, (256, 128, 1, 0, 1))
+    b = run(cfg, bodies, (256, 128, 2, 0, 1))
+    assert a[3]["accepted"] == b[3]["accepted"] and a[3]["opened"] == b[3]["opened"]
+    assert np.median(rel_err(b[0], a[0])) < 1e-6
+    sizes = [700, 1, 2048, 333, 64]
+    parts = [nbody_sim.generate("plummer", n, seed=i) for i, n in enumerate(sizes)]
+    bodies = [np.concatenate([p[i] for p in parts]) for i in range(3)]
+    cfg, _ = params(theta=0.6)
+    a = run(cfg, bodies, (256, 128, 1, 0, 1), ensemble_sizes=sizes)
+    b = run(cfg, bodies, (128, 128, 2, 1, 1), ensemble_sizes=sizes)
+    assert a[3]["accepted"] == b[3]["accepted"] and a[3]["opened"] == b[3]["opened"]
+    assert np.median(rel_err(b[0], a[0])) < 1e-6
 
 
 def test_register_stack_walk_ensemble_and_reference_modes():

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -55,6 +56,7 @@ struct nbody_sim {
     int num_sims = 1;
     std::vector<int64_t> sim_off_host;  // [num_sims + 1]
     int64_t cap = 0, n = 0;
+    float mass_absmax = 0.0f;  // largest |mass| uploaded (NaN if any is not a number)
     int passes = 4, key_bits = 32;
 
     hipStream_t own_stream = nullptr;
@@ -80,7 +82,7 @@ struct nbody_sim {
 
     int32_t debug = 0;
     int walk_block = 256, walk_stack = 128;
-    int walk_variant = 1;  // 0: LDS-stack walk, 1: register-stack walk
+    int walk_variant = 2;  // 0: LDS-stack walk, 1: register-stack walk, 2: pair walk (fast mode)
     int lane_bodies = 1;   // bodies per lane in the register-stack walks (1, 2, 4)
     bool xcd_remap = false;
 
@@ -222,6 +224,18 @@ int allocate(nbody_sim *s, int64_t cap) {
     return NBODY_OK;
 }
 
+// True when a body's own leaf adds an exact zero to its fast-mode sum: the
+// offset is exactly zero, and with constant softening 1/sqrt(eps^2) and
+// |m| * eps^-3 stay finite (far from overflow), so the term m * inv^3 * 0 is
+// a zero, never a NaN.
+bool self_term_zero(const nbody_sim *s) {
+    if (s->cfg.softening_mode != NBODY_SOFTEN_CONSTANT) return false;
+    const double eps2 = (double)s->cfg.epsilon * (double)s->cfg.epsilon;
+    const double m = (double)s->mass_absmax;
+    if (!(eps2 >= 1e-30) || !(m <= 1e30)) return false;  // also false for NaN
+    return m * std::pow(eps2, -1.5) <= 1e30;
+}
+
 // Enqueue one step: simulation.ts:393-421 (tree_only: phases 0-4, which
 // rebuild the tree of the current positions without moving the bodies).
 int enqueue_step(nbody_sim *s, bool tree_only = false) {
@@ -281,6 +295,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     w.overflow = s->overflow;
     w.n = (uint32_t)n;
     w.p = p;
+    w.self_zero = self_term_zero(s) ? 1u : 0u;
     launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
                 s->xcd_remap, s->walk_variant, s->lane_bodies, st);
     mark(6);
@@ -520,6 +535,13 @@ int nbody_set_bodies(nbody_sim *s, int64_t n, const float *mass, const float *po
         s->sim_off_host[0] = 0;
         s->sim_off_host[1] = n;
     }
+    float mmax = 0.0f;
+    for (int64_t i = 0; i < n; ++i) {
+        const float a = std::fabs(mass[i]);
+        if (!(a <= mmax)) mmax = a;  // a NaN sticks
+        if (mmax != mmax) break;
+    }
+    s->mass_absmax = mmax;
     if (n > 0) {
         HIP_TRY(s, hipMemcpyAsync(s->mass, mass, 4 * (size_t)n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(s, hipMemcpyAsync(s->pos, pos_xy, 8 * (size_t)n, hipMemcpyHostToDevice, s->stream));
@@ -702,8 +724,8 @@ int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");
     if (t->walk_stack < 8 || t->walk_stack > 256)
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_stack must be in [8, 256]");
-    if (t->walk_variant < 0 || t->walk_variant > 1)
-        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0 or 1");
+    if (t->walk_variant < 0 || t->walk_variant > 2)
+        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0, 1 or 2");
     if (t->lane_bodies != 1 && t->lane_bodies != 2 && t->lane_bodies != 4)
         return fail(s, NBODY_ERR_INVALID_ARG, "lane_bodies must be 1, 2 or 4");
     s->walk_block = t->walk_block;

@@ -399,8 +399,227 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     }
 }
 
+// Stack of the pair walk: entry k = (node, lane mask) in lane k of three
+// VGPRs (see RegStack).  The node's record is not kept: an entry is a node
+// that some lanes open, and what the walk needs next is its pair record,
+// loaded when the entry is popped.  The stack holds at most one pending left
+// child per level of the current path, so its 63 entries serve any tree of
+// depth up to 64.  Lane 63 stays free: the walk writes the slot above the top
+// on every iteration and advances the top only when it pushes, which keeps
+// the three registers in place (a conditional write makes the compiler copy
+// them on every iteration).
+constexpr int kPairStackCap = 63;
+struct PairStack {
+    int node, mlo, mhi;
+    __device__ __forceinline__ void put(int l, uint32_t nd, uint64_t m) {
+        node = nbs_writelane((int)nd, l, node);
+        mlo = nbs_writelane((int)(uint32_t)m, l, mlo);
+        mhi = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi);
+    }
+    __device__ __forceinline__ void get(int l, uint32_t &nd, uint64_t &m) const {
+        nd = (uint32_t)__builtin_amdgcn_readlane(node, l);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(mlo, l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(mhi, l);
+        m = ((uint64_t)hi << 32) | lo;
+    }
+};
+
+// Pair walk (fast arithmetic, one body per lane).  The loop's unit is an
+// opened node: its pair record holds both children's records and indices,
+// so both children are tested in one iteration, from one scalar load, with
+// two independent dependency chains.  The lanes that open a child form its
+// mask; of two opened children the right one is walked next and the left
+// one waits on the stack, as in the reference's walk (barnes_hut.wgsl:
+// 100-105).  Every lane takes exactly the opening decisions of its private
+// reference walk, with the fast mode's arithmetic (the same probe as the
+// register-stack walk), so the accepted nodes, the visit counts and every
+// term of the sum are the same; the order of the terms differs where a lane
+// accepts a left child whose right sibling it opens (the pair walk adds it
+// before the right subtree, the reference after), which the fast mode's
+// tolerance covers.  The next node's record load is issued before this
+// node's pulls, so the pulls hide part of its latency.
+//
+// kSelf: leave each body's own leaf out with a lane compare.  Without it the
+// own leaf is accepted: its offset is exactly zero, so with a finite pull
+// factor m * inv^3 (the host checks that eps^2 and the largest mass keep it
+// finite) its term is an exact +0 and the sum is unchanged.
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf>
+__global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint32_t xcd_per) {
+    const int lane = threadIdx.x & 63;
+    uint32_t block = blockIdx.x;
+    if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
+    const uint32_t n = a.n;
+    const uint32_t t = block * blockDim.x + threadIdx.x;  // sorted position
+    const bool valid = t < n;
+    const uint32_t leaf0 = n - 1u;
+    const uint32_t my_leaf = leaf0 + t;
+    float px = 0.0f, py = 0.0f, m1 = 0.0f;
+    if (valid) {
+        const float4 me = a.leaf[t];  // written by the fill: {pos, mass, 0}
+        px = me.x;
+        py = me.y;
+        m1 = me.z;
+    }
+    const uint64_t live = __ballot(valid);
+    if (live == 0) return;
+
+    const StepParams p = a.p;
+    const float eps2c = p.eps * p.eps;
+    float ax = 0.0f, ay = 0.0f;
+    uint64_t n_acc = 0, n_open = 0, n_drop = 0;
+    auto probe = [&](const float4 &c, float &rx, float &ry, float &inv) {
+        rx = c.x - px;
+        ry = c.y - py;
+        const float eps2 = kRefSoft ? soft_eps2(m1, c.z, p) : eps2c;
+        inv = inv_dist<false>(rx, ry, eps2);
+        return passes(c.w, inv, p.theta);
+    };
+    auto pull = [&](const float4 &c, float rx, float ry, float inv, uint64_t acc) {
+        float nx = ax, ny = ay;
+        add_pull<false>(c, rx, ry, inv, p.grav, nx, ny);
+        const bool in = __builtin_amdgcn_inverse_ballot_w64(acc);
+        ax = in ? nx : ax;
+        ay = in ? ny : ay;
+    };
+    PairStack st;
+    int sp = 0;
+    // A start node (the root, or a sim's sub-tree root) is tested by its
+    // lanes on its own record; the lanes that open it make a stack entry.
+    auto start = [&](uint32_t r, uint64_t m, const float4 &c) {
+        float rx, ry, inv;
+        const uint64_t pass = __ballot(probe(c, rx, ry, inv));
+        uint64_t acc = m & pass;
+        // a start node that is a leaf is the root of a one-body (sub-)tree
+        if (r >= leaf0) acc &= __ballot(my_leaf != r);
+        const uint64_t open = m & ~pass;
+        if (acc) pull(c, rx, ry, inv, acc);
+        if constexpr (kStats) {
+            n_acc += __builtin_popcountll(acc);
+            n_open += __builtin_popcountll(open);
+        }
+        if (open) {
+            if (sp < kPairStackCap) {
+                st.put(sp, r, open);
+                ++sp;
+            } else {
+                *a.overflow = 1u;
+                if constexpr (kStats) {
+                    n_open -= __builtin_popcountll(open);
+                    n_drop += __builtin_popcountll(open);
+                }
+            }
+        }
+    };
+    if constexpr (kEnsemble) {
+        const uint32_t my_root = valid ? a.sim_root[(uint32_t)(a.keys64[t] >> kSimShift)] : 0u;
+        uint64_t rem = live;
+        while (rem) {
+            const int l0 = __builtin_ctzll(rem);
+            const uint32_t r = uniform_u32((uint32_t)__builtin_amdgcn_readlane((int)my_root, l0));
+            const uint64_t m = __ballot(valid && my_root == r) & rem;
+            rem &= ~m;
+            start(r, m, node_record(a, r));
+        }
+    } else {
+        start(0u, live, a.root[0]);
+    }
+
+    if (sp > 0) {
+        const_as<float> *pcm_s = (const_as<float> *)a.pcm;
+        const_as<uint32_t> *pcmu_s = (const_as<uint32_t> *)a.pcm;
+        uint32_t node;
+        uint64_t mask;
+        --sp;
+        st.get(sp, node, mask);
+        // the pair record of `node` (dwords: HIP vector types cannot be
+        // loaded through the constant address space); 64-bit offsets
+        size_t rb = 4 * kPairStride * (size_t)node;
+        float4 cl = make_float4(pcm_s[rb + 0], pcm_s[rb + 1], pcm_s[rb + 2], pcm_s[rb + 3]);
+        float4 cr = make_float4(pcm_s[rb + 4], pcm_s[rb + 5], pcm_s[rb + 6], pcm_s[rb + 7]);
+        uint2 ch = make_uint2(pcmu_s[rb + 8], pcmu_s[rb + 9]);
+        for (;;) {
+            float rxl, ryl, invl, rxr, ryr, invr;
+            const uint64_t bl = __ballot(probe(cl, rxl, ryl, invl));
+            const uint64_t br = __ballot(probe(cr, rxr, ryr, invr));
+            uint64_t accl = mask & bl, accr = mask & br;
+            if constexpr (kSelf) {
+                accl &= __ballot(my_leaf != ch.x);
+                accr &= __ballot(my_leaf != ch.y);
+            }
+            // leaves always pass, so an opened child is an internal node
+            const uint64_t openl = mask & ~bl, openr = mask & ~br;
+            if constexpr (kStats) {
+                n_acc += __builtin_popcountll(accl) + __builtin_popcountll(accr);
+                n_open += __builtin_popcountll(openl) + __builtin_popcountll(openr);
+            }
+            // the next entry: right child first, the left one waits on the
+            // stack (written to the free slot every time, kept by advancing
+            // the top); with neither child opened, the top of the stack
+            // (0/1 flags as integers from popcounts: booleans here would be
+            // materialised as lane masks and moved through VGPRs)
+            const uint32_t hl = min((uint32_t)__builtin_popcountll(openl), 1u);
+            const uint32_t hr = min((uint32_t)__builtin_popcountll(openr), 1u);
+            const uint32_t push = hl & hr;
+            st.put(sp, ch.x, openl);
+            const uint32_t top = (uint32_t)sp + push;
+            if (top > (uint32_t)kPairStackCap) {
+                *a.overflow = 1u;  // every lane stores the same flag
+                if constexpr (kStats) {
+                    n_open -= __builtin_popcountll(openl);
+                    n_drop += __builtin_popcountll(openl);
+                }
+            }
+            sp = (int)min(top, (uint32_t)kPairStackCap);
+            uint32_t next = hr ? ch.y : ch.x;
+            uint64_t nmask = hr ? openr : openl;
+            if ((openl | openr) == 0 && sp > 0) {
+                --sp;
+                st.get(sp, next, nmask);
+            }
+            // issue the next record's loads before this node's pulls
+            float4 ncl = cl, ncr = cr;
+            uint2 nch = ch;
+            if (nmask) {
+                rb = 4 * kPairStride * (size_t)next;
+                ncl = make_float4(pcm_s[rb + 0], pcm_s[rb + 1], pcm_s[rb + 2], pcm_s[rb + 3]);
+                ncr = make_float4(pcm_s[rb + 4], pcm_s[rb + 5], pcm_s[rb + 6], pcm_s[rb + 7]);
+                nch = make_uint2(pcmu_s[rb + 8], pcmu_s[rb + 9]);
+            }
+            if (accr) pull(cr, rxr, ryr, invr, accr);
+            if (accl) pull(cl, rxl, ryl, invl, accl);
+            if (!nmask) break;
+            mask = nmask;
+            cl = ncl;
+            cr = ncr;
+            ch = nch;
+        }
+    }
+
+    if (valid)
+        kick_drift<false>(a, t, px, py, accel_out<false>(ax, p.grav), accel_out<false>(ay, p.grav));
+    if constexpr (kStats) {
+        // the counters are wave-uniform: one atomic per wave and counter
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], (unsigned long long)n_acc);
+            atomicAdd(&a.stats[1], (unsigned long long)n_open);
+            if (n_drop) atomicAdd(&a.stats[2], (unsigned long long)n_drop);
+        }
+    }
+}
+
 // Register-stack capacity for a requested stack size: one or two rows of 64.
 int rs_stack_cap(int stack) { return stack < 64 ? 64 : (stack > 128 ? 128 : stack); }
+
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf>
+void launch_pair(const WalkArgs &a, int block, bool xcd, hipStream_t s) {
+    uint32_t grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
+    uint32_t per = 0;
+    if (xcd && grid >= 16) {
+        per = (grid + 7u) / 8u;
+        grid = per * 8u;  // blocks past the bodies exit at once
+    }
+    walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf><<<grid, block, 0, s>>>(a, per);
+}
 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, int kB>
 void launch_rs(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s) {
@@ -418,6 +637,24 @@ void launch_rs(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s)
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
 void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, int lane_bodies,
               hipStream_t s) {
+    if constexpr (!kExact) {
+        if (variant == 2) {
+            // the own-leaf compare is needed for the counters and whenever
+            // the own leaf's term might not be an exact zero
+            if constexpr (kStats || kRefSoft) {
+                launch_pair<kRefSoft, kEnsemble, kStats, true>(a, block, xcd, s);
+            } else {
+                if (a.self_zero)
+                    launch_pair<kRefSoft, kEnsemble, kStats, false>(a, block, xcd, s);
+                else
+                    launch_pair<kRefSoft, kEnsemble, kStats, true>(a, block, xcd, s);
+            }
+            return;
+        }
+    }
+    // the pair walk sums in its own order: the exact mode, which keeps the
+    // reference's order, runs the register-stack walk instead
+    if (variant == 2) variant = 1;
     if (variant == 1) {
         if (lane_bodies == 4)
             launch_rs<kExact, kRefSoft, kEnsemble, kStats, 4>(a, block, stack, xcd, s);

@@ -114,6 +114,10 @@ struct WalkArgs {
     unsigned long long *stats;  // optional {accepted, opened, overflowed}
     uint32_t *overflow;         // set to 1 when a full stack drops nodes
     uint32_t n;
+    // 1 when a body's own leaf adds an exact zero to its fast-mode sum:
+    // constant softening with eps^2 and the largest |mass| such that
+    // |m| * eps^-3 stays finite (the offset is exactly zero)
+    uint32_t self_zero;
     StepParams p;
 };
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,

@@ -1,18 +1,19 @@
 """Launch autotuner for the tree walk.
 
-Times each candidate (walk variant: 0 LDS stack, 1 register stack; bodies
-per lane of the register-stack walk; workgroup size; stack entries; XCD
-order) on the handle's own bodies with the engine's HIP-event phase timers,
-restores the bodies after every trial, and applies the fastest.  Every
-candidate computes the same result (the tunables only change how bodies are
-laid out over wavefronts and where the traversal stack lives), so tuning
-never changes the physics.
+Times each candidate (walk variant: 0 LDS stack, 1 register stack, 2 pair
+walk; bodies per lane of the register-stack walk; workgroup size; stack
+entries; XCD order) on the handle's own bodies with the engine's HIP-event
+phase timers, restores the bodies after every trial, and applies the
+fastest.  Every candidate takes the same opening decisions; variants 0 and 1
+add each body's terms in the reference's order and agree bit for bit, the
+pair walk (fast mode) adds the same terms in its own order, so tuning never
+changes the physics beyond rounding.
 """
 from __future__ import annotations
 
 import itertools
 
-DEFAULT_VARIANTS = (1, 0)
+DEFAULT_VARIANTS = (2, 1, 0)
 DEFAULT_LANE_BODIES = (1, 2, 4)
 DEFAULT_XCD = (1, 0)
 DEFAULT_BLOCKS = (64, 128, 256, 512)
@@ -21,7 +22,10 @@ DEFAULT_STACKS = (64, 128)
 
 def _candidates(variants, blocks, stacks, xcd, lane_bodies):
     for v, b, st, x in itertools.product(variants, blocks, stacks, xcd):
-        # the LDS-stack walk has one body per lane
+        # the LDS-stack and pair walks have one body per lane, and the pair
+        # walk's stack is fixed
+        if v == 2 and st != stacks[0]:
+            continue
         for lb in (lane_bodies if v == 1 else (1,)):
             yield (v, b, st, x, lb)
 
