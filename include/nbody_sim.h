@@ -161,7 +161,7 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *                children of an opened node tested per iteration, stack of
  *                (node, mask) in VGPR lanes (default)
  *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
- *                of Morton-ordered bodies; 0: hardware order (default)
+ *                of Morton-ordered bodies (default); 0: hardware order
  *   lane_bodies  bodies per lane of the VGPR-stack walk and the adjoint walk:
  *                1 (default), 2 or 4; a wavefront walks 64 * lane_bodies
  *                Morton-consecutive bodies together (the LDS and pair walks

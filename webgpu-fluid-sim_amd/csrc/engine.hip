@@ -84,7 +84,7 @@ struct nbody_sim {
     int walk_block = 256, walk_stack = 128;
     int walk_variant = 2;  // 0: LDS-stack walk, 1: register-stack walk, 2: pair walk (fast mode)
     int lane_bodies = 1;   // bodies per lane in the register-stack walks (1, 2, 4)
-    bool xcd_remap = false;
+    bool xcd_remap = true;  // each XCD walks one contiguous run of Morton-ordered bodies
 
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;

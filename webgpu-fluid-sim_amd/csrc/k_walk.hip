@@ -402,13 +402,28 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
 // Stack of the pair walk: entry k = (node, lane mask) in lane k of three
 // VGPRs (see RegStack).  The node's record is not kept: an entry is a node
 // that some lanes open, and what the walk needs next is its pair record,
-// loaded when the entry is popped.  The stack holds at most one pending left
-// child per level of the current path, so its 63 entries serve any tree of
-// depth up to 64.  Lane 63 stays free: the walk writes the slot above the top
-// on every iteration and advances the top only when it pushes, which keeps
-// the three registers in place (a conditional write makes the compiler copy
-// them on every iteration).
-constexpr int kPairStackCap = 63;
+// loaded when the entry is popped.  Lane 0 holds a sentinel (node 0, empty
+// mask): popping it ends the walk, so a pop needs no emptiness test and the
+// popped node always has a record to load.  Lane 63 stays free: the walk
+// writes the slot above the top on every iteration and advances the top only
+// when it pushes, which keeps the three registers in place (a conditional
+// write makes the compiler copy them on every iteration).  The stack holds at
+// most one pending left child per level of the current path, so its 62
+// entries serve any tree of depth up to 63 (30-bit keys tie-broken on
+// positions below 2^30 stay below that; deeper trees raise the overflow flag).
+constexpr int kPairStackTop = 63;  // highest top (lanes 0..62 in use)
+
+// 1 when the wave-uniform mask m is not empty, computed on the scalar unit
+// (written as C++, the compiler turns the flag into a vector select and a
+// readfirstlane).  An ALU compare and select only: no memory access.
+__device__ __forceinline__ uint32_t nonzero_flag(uint64_t m) {
+    uint32_t r;
+    asm volatile("s_cmp_lg_u64 %1, 0\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(m) : "scc");
+    return r;
+}
+__device__ __forceinline__ float uniform_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 struct PairStack {
     int node, mlo, mhi;
     __device__ __forceinline__ void put(int l, uint32_t nd, uint64_t m) {
@@ -436,8 +451,14 @@ struct PairStack {
 // term of the sum are the same; the order of the terms differs where a lane
 // accepts a left child whose right sibling it opens (the pair walk adds it
 // before the right subtree, the reference after), which the fast mode's
-// tolerance covers.  The next node's record load is issued before this
-// node's pulls, so the pulls hide part of its latency.
+// tolerance covers.  A lane's order is fixed (a right-first pre-order of its
+// own opened nodes), whatever other lanes share its wavefront.
+//
+// The loop is written for the scalar unit, which bounds it: the next
+// record's load is issued before this node's pulls (their factors are
+// computed first, so the load may overwrite the record's registers), the
+// pulls are branch-free (a lane outside the accept mask keeps its sum), and
+// every decision is an integer or a lane mask, never a branch on a boolean.
 //
 // kSelf: leave each body's own leaf out with a lane compare.  Without it the
 // own leaf is accepted: its offset is exactly zero, so with a finite pull
@@ -474,15 +495,16 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
         inv = inv_dist<false>(rx, ry, eps2);
         return passes(c.w, inv, p.theta);
     };
-    auto pull = [&](const float4 &c, float rx, float ry, float inv, uint64_t acc) {
-        float nx = ax, ny = ay;
-        add_pull<false>(c, rx, ry, inv, p.grav, nx, ny);
+    // a + f r for the lanes of `acc` (f = m inv^3: add_pull's fast form)
+    auto accumulate = [&](float f, float rx, float ry, uint64_t acc) {
+        const float nx = __builtin_fmaf(f, rx, ax), ny = __builtin_fmaf(f, ry, ay);
         const bool in = __builtin_amdgcn_inverse_ballot_w64(acc);
         ax = in ? nx : ax;
         ay = in ? ny : ay;
     };
     PairStack st;
-    int sp = 0;
+    st.put(0, 0u, 0ull);  // the sentinel
+    int sp = 1;
     // A start node (the root, or a sim's sub-tree root) is tested by its
     // lanes on its own record; the lanes that open it make a stack entry.
     auto start = [&](uint32_t r, uint64_t m, const float4 &c) {
@@ -492,13 +514,13 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
         // a start node that is a leaf is the root of a one-body (sub-)tree
         if (r >= leaf0) acc &= __ballot(my_leaf != r);
         const uint64_t open = m & ~pass;
-        if (acc) pull(c, rx, ry, inv, acc);
+        if (acc) accumulate(c.z * (inv * inv * inv), rx, ry, acc);
         if constexpr (kStats) {
             n_acc += __builtin_popcountll(acc);
             n_open += __builtin_popcountll(open);
         }
         if (open) {
-            if (sp < kPairStackCap) {
+            if (sp < kPairStackTop) {
                 st.put(sp, r, open);
                 ++sp;
             } else {
@@ -524,7 +546,8 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
         start(0u, live, a.root[0]);
     }
 
-    if (sp > 0) {
+    uint32_t n_over = 0;  // dropped left children (stack full)
+    if (sp > 1) {
         const_as<float> *pcm_s = (const_as<float> *)a.pcm;
         const_as<uint32_t> *pcmu_s = (const_as<uint32_t> *)a.pcm;
         uint32_t node;
@@ -532,15 +555,22 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
         --sp;
         st.get(sp, node, mask);
         // the pair record of `node` (dwords: HIP vector types cannot be
-        // loaded through the constant address space); 64-bit offsets
+        // loaded through the constant address space); 64-bit offsets.  The
+        // first one passes through readfirstlane: a loop-entry value that is
+        // a load of the same form as the loop's own would let the compiler
+        // merge both loads into one at the loop head, where nothing can hide
+        // its latency.
         size_t rb = 4 * kPairStride * (size_t)node;
-        float4 cl = make_float4(pcm_s[rb + 0], pcm_s[rb + 1], pcm_s[rb + 2], pcm_s[rb + 3]);
-        float4 cr = make_float4(pcm_s[rb + 4], pcm_s[rb + 5], pcm_s[rb + 6], pcm_s[rb + 7]);
-        uint2 ch = make_uint2(pcmu_s[rb + 8], pcmu_s[rb + 9]);
+        float4 cl = make_float4(uniform_f32(pcm_s[rb + 0]), uniform_f32(pcm_s[rb + 1]),
+                                uniform_f32(pcm_s[rb + 2]), uniform_f32(pcm_s[rb + 3]));
+        float4 cr = make_float4(uniform_f32(pcm_s[rb + 4]), uniform_f32(pcm_s[rb + 5]),
+                                uniform_f32(pcm_s[rb + 6]), uniform_f32(pcm_s[rb + 7]));
+        uint2 ch = make_uint2(uniform_u32(pcmu_s[rb + 8]), uniform_u32(pcmu_s[rb + 9]));
         for (;;) {
             float rxl, ryl, invl, rxr, ryr, invr;
             const uint64_t bl = __ballot(probe(cl, rxl, ryl, invl));
             const uint64_t br = __ballot(probe(cr, rxr, ryr, invr));
+            const float fl = cl.z * (invl * invl * invl), fr = cr.z * (invr * invr * invr);
             uint64_t accl = mask & bl, accr = mask & br;
             if constexpr (kSelf) {
                 accl &= __ballot(my_leaf != ch.x);
@@ -552,48 +582,45 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
                 n_acc += __builtin_popcountll(accl) + __builtin_popcountll(accr);
                 n_open += __builtin_popcountll(openl) + __builtin_popcountll(openr);
             }
-            // the next entry: right child first, the left one waits on the
-            // stack (written to the free slot every time, kept by advancing
-            // the top); with neither child opened, the top of the stack
-            // (0/1 flags as integers from popcounts: booleans here would be
-            // materialised as lane masks and moved through VGPRs)
-            const uint32_t hl = min((uint32_t)__builtin_popcountll(openl), 1u);
-            const uint32_t hr = min((uint32_t)__builtin_popcountll(openr), 1u);
-            const uint32_t push = hl & hr;
+            // The next entry: the right child first, the left one waits on
+            // the stack (written to the free slot every time, kept by
+            // advancing the top); with neither child opened, the top entry
+            // (read every time: a branch here would split the loop).  Flags
+            // are 0/1 integers: booleans here would be materialised as lane
+            // masks.
+            const uint32_t hl = nonzero_flag(openl), hr = nonzero_flag(openr);
+            uint32_t tnode;
+            uint64_t tmask;
+            st.get(sp - 1, tnode, tmask);
             st.put(sp, ch.x, openl);
-            const uint32_t top = (uint32_t)sp + push;
-            if (top > (uint32_t)kPairStackCap) {
-                *a.overflow = 1u;  // every lane stores the same flag
-                if constexpr (kStats) {
+            const uint32_t top = (uint32_t)sp + (hl & hr);
+            n_over += top >> 6;  // top == 64: the left child is dropped
+            if constexpr (kStats) {
+                if (top >> 6) {
                     n_open -= __builtin_popcountll(openl);
                     n_drop += __builtin_popcountll(openl);
                 }
             }
-            sp = (int)min(top, (uint32_t)kPairStackCap);
+            const uint32_t any = hl | hr;
             uint32_t next = hr ? ch.y : ch.x;
             uint64_t nmask = hr ? openr : openl;
-            if ((openl | openr) == 0 && sp > 0) {
-                --sp;
-                st.get(sp, next, nmask);
-            }
-            // issue the next record's loads before this node's pulls
-            float4 ncl = cl, ncr = cr;
-            uint2 nch = ch;
-            if (nmask) {
-                rb = 4 * kPairStride * (size_t)next;
-                ncl = make_float4(pcm_s[rb + 0], pcm_s[rb + 1], pcm_s[rb + 2], pcm_s[rb + 3]);
-                ncr = make_float4(pcm_s[rb + 4], pcm_s[rb + 5], pcm_s[rb + 6], pcm_s[rb + 7]);
-                nch = make_uint2(pcmu_s[rb + 8], pcmu_s[rb + 9]);
-            }
-            if (accr) pull(cr, rxr, ryr, invr, accr);
-            if (accl) pull(cl, rxl, ryl, invl, accl);
-            if (!nmask) break;
+            next = any ? next : tnode;
+            nmask = any ? nmask : tmask;  // the sentinel's empty mask ends the walk
+            sp = any ? (int)min(top, (uint32_t)kPairStackTop) : sp - 1;
+            // the next record's loads, issued before this node's pulls
+            rb = 4 * kPairStride * (size_t)next;
+            cl = make_float4(pcm_s[rb + 0], pcm_s[rb + 1], pcm_s[rb + 2], pcm_s[rb + 3]);
+            cr = make_float4(pcm_s[rb + 4], pcm_s[rb + 5], pcm_s[rb + 6], pcm_s[rb + 7]);
+            ch = make_uint2(pcmu_s[rb + 8], pcmu_s[rb + 9]);
+            // keep the pulls behind the loads in the schedule
+            __builtin_amdgcn_sched_barrier(0);
+            accumulate(fr, rxr, ryr, accr);
+            accumulate(fl, rxl, ryl, accl);
             mask = nmask;
-            cl = ncl;
-            cr = ncr;
-            ch = nch;
+            if (mask == 0) break;
         }
     }
+    if (n_over) *a.overflow = 1u;
 
     if (valid)
         kick_drift<false>(a, t, px, py, accel_out<false>(ax, p.grav), accel_out<false>(ay, p.grav));
