@@ -160,18 +160,23 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes; 2: pair walk, both
  *                children of an opened node tested per iteration, stack of
  *                (node, mask) in VGPR lanes (default)
- *   xcd_order    1: deal walk workgroups so each XCD walks one contiguous run
- *                of Morton-ordered bodies (default); 0: hardware order
+ *   xcd_order    order of the walk's workgroups over the 8 XCDs (each has its
+ *                own L2): 0 hardware order (round-robin); 1 each XCD walks one
+ *                contiguous run of Morton-ordered bodies (default); k > 1
+ *                runs of k workgroups dealt to the XCDs in turn
  *   lane_bodies  bodies per lane of the VGPR-stack walk and the adjoint walk:
  *                1 (default), 2 or 4; a wavefront walks 64 * lane_bodies
  *                Morton-consecutive bodies together (the LDS and pair walks
- *                ignore it, walking one body per lane) */
+ *                ignore it, walking one body per lane)
+ *   radix_bits   digit width of the radix sort: 8 (default), 10 or 11 bits
+ *                per pass (30-bit keys: 4, 3 or 3 passes) */
 typedef struct nbody_tuning {
     int32_t walk_block;
     int32_t walk_stack;
     int32_t walk_variant;
     int32_t xcd_order;
     int32_t lane_bodies;
+    int32_t radix_bits;
 } nbody_tuning;
 int nbody_set_tuning(nbody_sim *sim, const nbody_tuning *t);
 int nbody_get_tuning(const nbody_sim *sim, nbody_tuning *out);

@@ -116,10 +116,95 @@ def test_autotune_picks_a_working_configuration():
     sim = nbody_sim.Simulation(cfg)
     sim.setBodies(*bodies)
     best, table = autotune(sim, steps=2, blocks=(128, 256), stacks=(128,), xcd=(1,),
-                           lane_bodies=(1, 2))
-    assert best in table and all(v > 0 for v in table.values())
+                           lane_bodies=(1, 2), radix=(10, 8))
+    assert best[:5] in table and ("radix_bits", best[5]) in table
+    assert all(v > 0 for v in table.values())
+    t = sim.getTuning()
+    assert (t["walk_variant"], t["walk_block"], t["radix_bits"]) == (best[0], best[1], best[5])
     _, p0, _ = sim.getBodies()
     assert np.array_equal(p0, bodies[1])  # state restored
     sim.step(2)
     sim.synchronize()
+    sim.close()
+
+
+@pytest.mark.parametrize("morton", ["bbox30", "reference"])
+def test_radix_widths_identical(morton):
+    """The sort is stable for every digit width (8, 10, 11 bits: 4, 3, 3
+    passes for 30-bit keys, 4, 4, 3 for 32-bit keys): the same keys,
+    permutation and therefore the same steps."""
+    bodies = nbody_sim.generate("exp-disk", 40000, seed=21, dispersion=0.05, mass_sigma=0.5)
+    cfg, _ = params(theta=0.7, dt=5e-5, morton=morton)
+    out = {}
+    for rb in (8, 10, 11):
+        sim = nbody_sim.Simulation(cfg)
+        sim.setBodies(*bodies)
+        sim.setTuning(radix_bits=rb)
+        sim.setDebug(accel=True)
+        sim.step(3)
+        sim.synchronize()
+        out[rb] = [sim.debugBuffer(k) for k in ("keys", "body_indices", "accel")]
+        out[rb] += list(sim.getBodies()[1:])
+        sim.close()
+    for rb in (10, 11):
+        for a, b in zip(out[8], out[rb]):
+            assert np.array_equal(a, b), rb
+
+
+def test_radix_widths_ensemble():
+    """38-bit ensemble keys (sim id above the Morton code): 5, 4 and 4 passes."""
+    sizes = [900, 5000, 1, 3000, 64]
+    parts = [nbody_sim.generate("plummer", n, seed=40 + i) for i, n in enumerate(sizes)]
+    bodies = [np.concatenate([p[i] for p in parts]) for i in range(3)]
+    cfg, _ = params(theta=0.6)
+    ref = None
+    for rb in (8, 10, 11):
+        ens = nbody_sim.Ensemble(cfg, sizes)
+        ens.setBodies(*bodies)
+        ens.setTuning(radix_bits=rb)
+        ens.step(2)
+        ens.synchronize()
+        got = [ens.debugBuffer("keys"), ens.debugBuffer("body_indices"), ens.getBodies()[1]]
+        ens.close()
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b), rb
+
+
+def test_bbox_carried_between_steps():
+    """Plain bbox30 handles take each step's bounding box from the previous
+    walk's accumulators: after eager steps, graph replays, new bodies and an
+    adjoint sweep (which moves the positions outside a step) the box equals
+    the reduction of the current positions."""
+    from oracle import oracle as O
+
+    bodies = nbody_sim.generate("exp-disk", 20000, seed=22, dispersion=0.05, mass_sigma=0.5)
+    cfg, _ = params(theta=0.7, dt=5e-5)
+    sim = nbody_sim.Simulation(cfg)
+
+    def box_of_step_after(setup):
+        setup()
+        _, p, _ = sim.getBodies()
+        sim.step(1)
+        sim.synchronize()
+        return sim.debugBuffer("bbox")[0], O.bbox(p)
+
+    def steps(k, graph):
+        sim.useGraph(graph)
+        sim.step(k)
+        sim.synchronize()
+
+    sim.setBodies(*bodies)
+    for setup in (lambda: None, lambda: steps(4, False), lambda: steps(5, True),
+                  lambda: sim.setBodies(*nbody_sim.generate("uniform-disk", 20000, seed=5)),
+                  lambda: steps(3, True)):
+        got, want = box_of_step_after(setup)
+        assert np.array_equal(got, np.asarray(want, np.float32)), (got, want)
+    sim.useGraph(False)
+    sim.adjointForward(4, 2)
+    sim.adjointBackward(np.ones((20000, 2), np.float32), None)
+    got, want = box_of_step_after(lambda: None)
+    assert np.array_equal(got, np.asarray(want, np.float32))
     sim.close()

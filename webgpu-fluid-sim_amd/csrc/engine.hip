@@ -57,7 +57,8 @@ struct nbody_sim {
     std::vector<int64_t> sim_off_host;  // [num_sims + 1]
     int64_t cap = 0, n = 0;
     float mass_absmax = 0.0f;  // largest |mass| uploaded (NaN if any is not a number)
-    int passes = 4, key_bits = 32;
+    int key_bits = 32;
+    int radix_bits = 8;  // sort digit width (nbody_tuning.radix_bits)
 
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -71,6 +72,12 @@ struct nbody_sim {
     uint32_t *sort_words = nullptr;
     SortScratch sort{};
     float4 *bbox_partial = nullptr, *bbox = nullptr;
+    // Bounding-box accumulators: the walk folds the drifted positions into
+    // them, the next step's encode reads them, the build zeroes them.
+    // bbox_fresh: they hold the box of the current positions (false after
+    // any change of the positions outside a step, or a tree-only step).
+    uint32_t *bbox_slots = nullptr;
+    bool bbox_fresh = false;
     int max_chunks = 1;
     int64_t *sim_off = nullptr;
     uint8_t *body_sim = nullptr;
@@ -84,7 +91,7 @@ struct nbody_sim {
     int walk_block = 256, walk_stack = 128;
     int walk_variant = 2;  // 0: LDS-stack walk, 1: register-stack walk, 2: pair walk (fast mode)
     int lane_bodies = 1;   // bodies per lane in the register-stack walks (1, 2, 4)
-    bool xcd_remap = true;  // each XCD walks one contiguous run of Morton-ordered bodies
+    int xcd_order = 1;  // walk workgroup order over the XCDs (nbody_tuning.xcd_order)
 
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;
@@ -171,7 +178,7 @@ int allocate(nbody_sim *s, int64_t cap) {
     const size_t key_size = s->ensemble ? 8 : 4;
     const int64_t nodes = 2 * cap - 1;
     s->max_chunks = (int)std::max<int64_t>(1, (cap + kBBoxChunk - 1) / kBBoxChunk);
-    const int64_t sort_words = sort_scratch_words(cap, s->passes);
+    const int64_t sort_words = sort_scratch_words(cap);
     // Size pass: lay out with a null base, then allocate the total.
     for (int pass = 0; pass < 2; ++pass) {
         Arena &a = s->arena;
@@ -197,6 +204,7 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->sort_words = a.take<uint32_t>(sort_words);
         s->bbox_partial = a.take<float4>((int64_t)s->max_chunks * s->num_sims);
         s->bbox = a.take<float4>(s->num_sims);
+        s->bbox_slots = a.take<uint32_t>(4 * kBBoxSlots);
         s->sim_off = a.take<int64_t>(s->num_sims + 1);
         s->body_sim = s->ensemble ? a.take<uint8_t>(cap) : nullptr;
         s->sim_root = a.take<uint32_t>(s->num_sims);
@@ -216,7 +224,9 @@ int allocate(nbody_sim *s, int64_t cap) {
             a.base = static_cast<char *>(p);
         }
     }
-    sort_bind_scratch(s->sort, s->sort_words, cap, s->passes);
+    s->bbox_fresh = false;
+    sort_bind_scratch(s->sort, s->sort_words, cap);
+    HIP_TRY(s, hipMemsetAsync(s->sort_words, 0, 4 * (size_t)sort_words, s->stream));
     s->cap = cap;
     HIP_TRY(s, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->overflow, 0, sizeof(uint32_t), s->stream));
@@ -236,6 +246,22 @@ bool self_term_zero(const nbody_sim *s) {
     return m * std::pow(eps2, -1.5) <= 1e30;
 }
 
+// Plain handles with bounding-box keys and two or more bodies carry the box
+// from the walk of one step to the encode of the next (the accumulators);
+// ensembles reduce their per-sim boxes with the bbox kernels every step.
+bool uses_slots(const nbody_sim *s) {
+    return !s->ensemble && s->cfg.morton_mode == NBODY_MORTON_BBOX30 && s->n > 1;
+}
+
+// Set the accumulators to the box of the current positions (the bbox
+// kernels), unless they already hold it.  Not for graph capture.
+void prime_bbox(nbody_sim *s) {
+    if (!uses_slots(s) || s->bbox_fresh) return;
+    launch_bbox(s->pos, s->sim_off, 1, s->max_chunks, s->bbox_partial, s->bbox, s->bbox_slots,
+                s->stream);
+    s->bbox_fresh = true;
+}
+
 // Enqueue one step: simulation.ts:393-421 (tree_only: phases 0-4, which
 // rebuild the tree of the current positions without moving the bodies).
 int enqueue_step(nbody_sim *s, bool tree_only = false) {
@@ -249,35 +275,50 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     };
 
     mark(0);
-    // phase 0: bounding box (bbox30 keys only)
-    if (s->ensemble || s->cfg.morton_mode == NBODY_MORTON_BBOX30)
-        launch_bbox(s->pos, s->sim_off, s->num_sims, s->max_chunks, s->bbox_partial, s->bbox, st);
+    // phase 0: bounding box (bbox30 keys only): the ensembles' per-sim boxes,
+    // or a plain handle's accumulators when the positions changed outside a
+    // step (a graph capture happens after prime_bbox)
+    const bool slots = uses_slots(s);
+    if (slots)
+        prime_bbox(s);
+    else if (s->ensemble || s->cfg.morton_mode == NBODY_MORTON_BBOX30)
+        launch_bbox(s->pos, s->sim_off, s->num_sims, s->max_chunks, s->bbox_partial, s->bbox,
+                    nullptr, st);
     mark(1);
-    // phase 1: Morton encode (lbvh.wgsl:42-69)
-    launch_encode(s->pos, n, s->cfg.morton_mode, s->bbox, s->body_sim,
-                  s->ensemble ? nullptr : (uint32_t *)s->keys[0],
-                  s->ensemble ? (uint64_t *)s->keys[0] : nullptr, s->vals[0], st);
+    // phase 1: Morton encode (lbvh.wgsl:42-69) and the sort's digit counts
+    const int passes = n > 1 ? sort_passes(s->key_bits, s->radix_bits) : 0;
+    launch_encode(s->pos, n, s->cfg.morton_mode, s->bbox, slots ? s->bbox_slots : nullptr,
+                  s->body_sim, s->ensemble ? nullptr : (uint32_t *)s->keys[0],
+                  s->ensemble ? (uint64_t *)s->keys[0] : nullptr, s->vals[0], s->sort.hist,
+                  passes, s->radix_bits, st);
     mark(2);
     // phase 2: stable radix sort (simulation.ts:400)
     if (s->ensemble)
         s->result_buf = launch_sort_u64((uint64_t *)s->keys[0], s->vals[0], (uint64_t *)s->keys[1],
-                                        s->vals[1], n, s->key_bits, s->sort, st);
+                                        s->vals[1], n, s->key_bits, s->radix_bits, s->sort, st);
     else
         s->result_buf = launch_sort_u32((uint32_t *)s->keys[0], s->vals[0], (uint32_t *)s->keys[1],
-                                        s->vals[1], n, s->key_bits, s->sort, st);
+                                        s->vals[1], n, s->key_bits, s->radix_bits, s->sort, st);
     const uint32_t *idx = s->vals[s->result_buf];
     mark(3);
-    // phase 3: Karras build (lbvh.wgsl:85-165)
+    // phase 3: Karras build (lbvh.wgsl:85-165); it also zeroes the sort's
+    // scratch for the next step
+    const int64_t clear = sort_clear_words(n, s->key_bits, s->radix_bits);
     if (s->ensemble)
         launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree,
-                         s->sim_root, st);
+                         s->sim_root, s->sort.hist, clear, nullptr, st);
     else
-        launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, st);
+        launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, s->sort.hist,
+                         clear, slots ? s->bbox_slots : nullptr, st);
     mark(4);
     // phase 4: bottom-up fill (lbvh.wgsl:168-233)
     launch_fill(idx, s->mass, s->pos, n, s->tree, (s->debug & NBODY_DEBUG_FILL_CLIMB) != 0, st);
     mark(5);
-    if (tree_only) return NBODY_OK;
+    if (tree_only) {
+        // the build has emptied the accumulators and no walk refills them
+        s->bbox_fresh = false;
+        return NBODY_OK;
+    }
     // phase 5: walk + kick + drift (barnes_hut.wgsl:48-121)
     WalkArgs w{};
     w.leaf = s->tree.leaf;
@@ -296,8 +337,10 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     w.n = (uint32_t)n;
     w.p = p;
     w.self_zero = self_term_zero(s) ? 1u : 0u;
+    w.bbox_slots = slots ? s->bbox_slots : nullptr;
     launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
-                s->xcd_remap, s->walk_variant, s->lane_bodies, st);
+                s->xcd_order, s->walk_variant, s->lane_bodies, st);
+    s->bbox_fresh = slots;  // the walk has folded the drifted positions in
     mark(6);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(s, e, "kernel launch");
@@ -374,7 +417,6 @@ nbody_sim *create_common(const nbody_config *cfg, int32_t device, int *err, bool
     } else {
         s->key_bits = cfg->morton_mode == NBODY_MORTON_BBOX30 ? 30 : 32;
     }
-    s->passes = (s->key_bits + 7) / 8;
     hipError_t e = hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         set_err(hip_fail(nullptr, e, "hipStreamCreate"));
@@ -507,6 +549,7 @@ int nbody_set_config(nbody_sim *s, const nbody_config *cfg) {
         return fail(s, NBODY_ERR_INVALID_ARG, "unknown morton_mode");
     s->cfg = *cfg;
     s->graph_dirty = true;
+    s->bbox_fresh = false;  // the accumulators are kept only in the bbox30 mode
     return NBODY_OK;
 }
 
@@ -545,6 +588,7 @@ int nbody_set_bodies(nbody_sim *s, int64_t n, const float *mass, const float *po
     if (n > 0) {
         HIP_TRY(s, hipMemcpyAsync(s->mass, mass, 4 * (size_t)n, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(s, hipMemcpyAsync(s->pos, pos_xy, 8 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+        s->bbox_fresh = false;
         HIP_TRY(s, hipMemcpyAsync(s->vel, vel_xy, 8 * (size_t)n, hipMemcpyHostToDevice, s->stream));
     }
     HIP_TRY(s, hipMemcpyAsync(s->sim_off, s->sim_off_host.data(), 8 * (size_t)(s->num_sims + 1),
@@ -584,6 +628,7 @@ int nbody_step(nbody_sim *s, int32_t nsteps) {
     HIP_TRY(s, hipSetDevice(s->device));
     if (s->n <= 0 || nsteps == 0) return NBODY_OK;
     if (s->use_graph) {
+        prime_bbox(s);  // eagerly, before the captured steps that expect it
         if (s->graph_dirty || !s->graph_exec) {
             int rc = capture_graph(s);
             if (rc != NBODY_OK) return rc;
@@ -728,11 +773,16 @@ int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0, 1 or 2");
     if (t->lane_bodies != 1 && t->lane_bodies != 2 && t->lane_bodies != 4)
         return fail(s, NBODY_ERR_INVALID_ARG, "lane_bodies must be 1, 2 or 4");
+    if (t->xcd_order < 0 || t->xcd_order > 65536)
+        return fail(s, NBODY_ERR_INVALID_ARG, "xcd_order must be in [0, 65536]");
+    if (t->radix_bits != 8 && t->radix_bits != 10 && t->radix_bits != 11)
+        return fail(s, NBODY_ERR_INVALID_ARG, "radix_bits must be 8, 10 or 11");
+    s->radix_bits = t->radix_bits;
     s->walk_block = t->walk_block;
     s->lane_bodies = t->lane_bodies;
     s->walk_stack = t->walk_stack;
     s->walk_variant = t->walk_variant;
-    s->xcd_remap = t->xcd_order != 0;
+    s->xcd_order = t->xcd_order;
     s->graph_dirty = true;
     return NBODY_OK;
 }
@@ -742,8 +792,9 @@ int nbody_get_tuning(const nbody_sim *s, nbody_tuning *out) {
     out->walk_block = s->walk_block;
     out->walk_stack = s->walk_stack;
     out->walk_variant = s->walk_variant;
-    out->xcd_order = s->xcd_remap ? 1 : 0;
+    out->xcd_order = s->xcd_order;
     out->lane_bodies = s->lane_bodies;
+    out->radix_bits = s->radix_bits;
     return NBODY_OK;
 }
 
@@ -843,6 +894,7 @@ int nbody_adjoint_backward(nbody_sim *s, const float *gpos_T, const float *gvel_
         // Recompute the segment forward from its checkpoint, keeping x_k.
         const float2 *cp = s->adj_cp + (size_t)seg * 2 * n;
         HIP_TRY(s, hipMemcpyAsync(s->pos, cp, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+        s->bbox_fresh = false;
         HIP_TRY(s, hipMemcpyAsync(s->vel, cp + n, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
         for (int32_t i = 0; i < len; ++i) {
             HIP_TRY(s, hipMemcpyAsync(s->adj_seg + (size_t)i * n, s->pos, 8 * (size_t)n,
@@ -856,6 +908,7 @@ int nbody_adjoint_backward(nbody_sim *s, const float *gpos_T, const float *gvel_
         for (int32_t i = len - 1; i >= 0; --i) {
             HIP_TRY(s, hipMemcpyAsync(s->pos, s->adj_seg + (size_t)i * n, 8 * (size_t)n,
                                       hipMemcpyDeviceToDevice, st));
+            s->bbox_fresh = false;
             int rc = enqueue_step(s, /*tree_only=*/true);
             if (rc != NBODY_OK) return rc;
             a.idx = s->vals[s->result_buf];
@@ -871,6 +924,7 @@ int nbody_adjoint_backward(nbody_sim *s, const float *gpos_T, const float *gvel_
     if (gvel0) HIP_TRY(s, hipMemcpyAsync(gvel0, s->adj_lv, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
     if (gmass) HIP_TRY(s, hipMemcpyAsync(gmass, s->adj_lm, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
     HIP_TRY(s, hipMemcpyAsync(s->pos, s->adj_end, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    s->bbox_fresh = false;
     HIP_TRY(s, hipMemcpyAsync(s->vel, s->adj_end + n, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
     s->graph_dirty = true;
     return nbody_synchronize(s);

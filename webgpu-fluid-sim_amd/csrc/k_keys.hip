@@ -6,8 +6,10 @@
 // the step's own square bounding box (origin at the box minimum, side = the
 // larger extent), which keeps the keys meaningful for unit-scale inputs.
 // Both are HBM-bound streams: 8 B of position in, 8 B of key + index out per
-// body (12 B for the ensemble's 64-bit keys).  The arithmetic is compiled
-// without contraction so every rounding matches the binary32 reference order.
+// body (12 B for the ensemble's 64-bit keys).  The encode also counts the
+// radix sort's digits (every pass) for the keys it writes.  The arithmetic is
+// compiled without contraction so every rounding matches the binary32
+// reference order.
 #include "nbody_internal.h"
 
 namespace nbs {
@@ -63,11 +65,13 @@ __global__ __launch_bounds__(256) void bbox_partial_kernel(const float2 *__restr
     }
 }
 
-// grid = num_sims, block = one wave.
+// grid = num_sims, block = one wave.  slots (one sim): the walk's
+// bounding-box accumulators are set to this box (slot 0) and empty slots.
 __global__ __launch_bounds__(64) void bbox_final_kernel(const int64_t *__restrict__ sim_off,
                                                         int max_chunks,
                                                         const float4 *__restrict__ partial,
-                                                        float4 *__restrict__ bbox) {
+                                                        float4 *__restrict__ bbox,
+                                                        uint32_t *__restrict__ slots) {
     const int s = blockIdx.x;
     const int64_t ns = sim_off[s + 1] - sim_off[s];
     const int nch = (int)((ns + kBBoxChunk - 1) / kBBoxChunk);
@@ -84,6 +88,32 @@ __global__ __launch_bounds__(64) void bbox_final_kernel(const int64_t *__restric
     mxx = wave_max(mxx);
     mxy = wave_max(mxy);
     if (threadIdx.x == 0) bbox[s] = make_float4(mnx, mny, mxx, mxy);
+    if (slots) {
+        // (a box without bodies, +inf/-inf, stays an empty slot)
+        const bool first = threadIdx.x == 0;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (first && mnx <= mxx) {
+            v = make_uint4(~float_order(mnx), ~float_order(mny), float_order(mxx),
+                           float_order(mxy));
+        }
+        reinterpret_cast<uint4 *>(slots)[threadIdx.x] = v;  // kBBoxSlots == 64 lanes
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
+// The bounding box held by the walk's accumulators (all 64 lanes of the
+// wave take part; one slot per lane).
+__device__ __forceinline__ float4 bbox_from_slots(const uint32_t *__restrict__ slots) {
+    const uint4 v = reinterpret_cast<const uint4 *>(slots)[threadIdx.x & 63];
+    const uint32_t mnx = wave_max_u32(v.x), mny = wave_max_u32(v.y);
+    const uint32_t mxx = wave_max_u32(v.z), mxy = wave_max_u32(v.w);
+    // an empty accumulator reads as the identity box (+inf, -inf)
+    return make_float4(mnx ? float_unorder(~mnx) : INFINITY, mny ? float_unorder(~mny) : INFINITY,
+                       mxx ? float_unorder(mxx) : -INFINITY, mxy ? float_unorder(mxy) : -INFINITY);
 }
 
 // Spread the low 16 bits of v to the even bit positions.
@@ -124,25 +154,65 @@ __device__ __forceinline__ uint32_t code_bbox30(float2 p, float4 b) {
     return part1by1(x) | (part1by1(y) << 1);
 }
 
-template <int kMode, bool kEnsemble>
-__global__ __launch_bounds__(256) void encode_kernel(const float2 *__restrict__ pos, int64_t n,
-                                                     const float4 *__restrict__ bbox,
+// Keys of bodies [blockIdx.x * kEncodeTile, +kEncodeTile), 16 per thread
+// (item it of thread t is body base + it * 1024 + t: coalesced), and the
+// digit counts of every sort pass for them: LDS histograms, then one global
+// atomic per non-empty bin per workgroup (large workgroups keep those few).
+// The sort needs these counts before its first pass; counting here saves a
+// kernel and a read of the keys.
+constexpr int kEncodeThreads = 1024;
+constexpr int kEncodeTile = 16 * kEncodeThreads;
+template <int kMode, bool kEnsemble, int RB>
+__global__ __launch_bounds__(kEncodeThreads) void encode_kernel(const float2 *__restrict__ pos, int64_t n,
+                                                     float4 *__restrict__ bbox,
+                                                     const uint32_t *__restrict__ slots,
                                                      const uint8_t *__restrict__ body_sim,
                                                      uint32_t *__restrict__ keys32,
                                                      uint64_t *__restrict__ keys64,
-                                                     uint32_t *__restrict__ idx) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float2 p = pos[i];
-    if constexpr (kEnsemble) {
-        const uint32_t s = body_sim[i];
-        keys64[i] = ((uint64_t)s << kSimShift) | code_bbox30(p, bbox[s]);
-    } else if constexpr (kMode == 0) {
-        keys32[i] = code_reference(p);
-    } else {
-        keys32[i] = code_bbox30(p, bbox[0]);
+                                                     uint32_t *__restrict__ idx,
+                                                     uint32_t *__restrict__ hist, int passes) {
+    constexpr int D = 1 << RB;
+    constexpr uint32_t kMask = (uint32_t)D - 1u;
+    __shared__ uint32_t sh[kSortMaxPasses * D];
+    const int bins = passes * D;
+    for (int i = threadIdx.x; i < bins; i += kEncodeThreads) sh[i] = 0u;
+    float4 box = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (kMode == 1 && !kEnsemble) {
+        if (slots) {
+            box = bbox_from_slots(slots);  // every wave reduces the 64 slots itself
+            if (blockIdx.x == 0 && threadIdx.x == 0) bbox[0] = box;
+        } else {
+            box = bbox[0];
+        }
     }
-    idx[i] = (uint32_t)i;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kEncodeTile;
+#pragma unroll 4
+    for (int it = 0; it < kEncodeTile / kEncodeThreads; ++it) {
+        const int64_t i = base + it * kEncodeThreads + threadIdx.x;
+        if (i >= n) break;
+        const float2 p = pos[i];
+        uint64_t key;
+        if constexpr (kEnsemble) {
+            const uint32_t s = body_sim[i];
+            key = ((uint64_t)s << kSimShift) | code_bbox30(p, bbox[s]);
+            keys64[i] = key;
+        } else if constexpr (kMode == 0) {
+            key = code_reference(p);
+            keys32[i] = (uint32_t)key;
+        } else {
+            key = code_bbox30(p, box);
+            keys32[i] = (uint32_t)key;
+        }
+        idx[i] = (uint32_t)i;
+        for (int q = 0; q < passes; ++q)
+            atomicAdd(&sh[q * D + (uint32_t)((key >> (RB * q)) & kMask)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < bins; i += kEncodeThreads) {
+        const uint32_t c = sh[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
 }
 
 __global__ __launch_bounds__(256) void clear_u32_kernel(uint32_t *__restrict__ p, int64_t words) {
@@ -154,23 +224,45 @@ __global__ __launch_bounds__(256) void clear_u32_kernel(uint32_t *__restrict__ p
 }  // namespace
 
 void launch_bbox(const float2 *pos, const int64_t *sim_off, int num_sims, int max_chunks,
-                 float4 *partial, float4 *bbox, hipStream_t s) {
+                 float4 *partial, float4 *bbox, uint32_t *slots, hipStream_t s) {
     dim3 grid(max_chunks, num_sims);
     bbox_partial_kernel<<<grid, 256, 0, s>>>(pos, sim_off, max_chunks, partial);
-    bbox_final_kernel<<<num_sims, 64, 0, s>>>(sim_off, max_chunks, partial, bbox);
+    bbox_final_kernel<<<num_sims, 64, 0, s>>>(sim_off, max_chunks, partial, bbox,
+                                              num_sims == 1 ? slots : nullptr);
 }
 
-void launch_encode(const float2 *pos, int64_t n, int morton_mode, const float4 *bbox,
-                   const uint8_t *body_sim, uint32_t *keys32, uint64_t *keys64, uint32_t *idx,
+template <int RB>
+void launch_encode_rb(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                      const uint32_t *slots, const uint8_t *body_sim, uint32_t *keys32,
+                      uint64_t *keys64, uint32_t *idx, uint32_t *hist, int passes,
+                      hipStream_t s) {
+    const unsigned grid = (unsigned)((n + kEncodeTile - 1) / kEncodeTile);
+    const int th = kEncodeThreads;
+    if (body_sim)
+        encode_kernel<1, true, RB>
+            <<<grid, th, 0, s>>>(pos, n, bbox, slots, body_sim, keys32, keys64, idx, hist, passes);
+    else if (morton_mode == 0)
+        encode_kernel<0, false, RB>
+            <<<grid, th, 0, s>>>(pos, n, bbox, slots, body_sim, keys32, keys64, idx, hist, passes);
+    else
+        encode_kernel<1, false, RB>
+            <<<grid, th, 0, s>>>(pos, n, bbox, slots, body_sim, keys32, keys64, idx, hist, passes);
+}
+
+void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                   const uint32_t *slots, const uint8_t *body_sim, uint32_t *keys32,
+                   uint64_t *keys64, uint32_t *idx, uint32_t *hist, int passes, int radix_bits,
                    hipStream_t s) {
     if (n <= 0) return;
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    if (body_sim)
-        encode_kernel<1, true><<<grid, 256, 0, s>>>(pos, n, bbox, body_sim, keys32, keys64, idx);
-    else if (morton_mode == 0)
-        encode_kernel<0, false><<<grid, 256, 0, s>>>(pos, n, bbox, body_sim, keys32, keys64, idx);
+    if (radix_bits == 11)
+        launch_encode_rb<11>(pos, n, morton_mode, bbox, slots, body_sim, keys32, keys64, idx,
+                             hist, passes, s);
+    else if (radix_bits == 10)
+        launch_encode_rb<10>(pos, n, morton_mode, bbox, slots, body_sim, keys32, keys64, idx,
+                             hist, passes, s);
     else
-        encode_kernel<1, false><<<grid, 256, 0, s>>>(pos, n, bbox, body_sim, keys32, keys64, idx);
+        launch_encode_rb<8>(pos, n, morton_mode, bbox, slots, body_sim, keys32, keys64, idx,
+                            hist, passes, s);
 }
 
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s) {

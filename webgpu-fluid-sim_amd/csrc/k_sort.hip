@@ -5,17 +5,22 @@
 // body indices as payload.  The indices start as the identity every step
 // (lbvh.wgsl:68), so a stable sort orders equal keys by body index.
 //
-// Design (Adinets & Merrill, "Onesweep", 2022), written for wave64:
-//   1. one histogram kernel counts every pass's 8-bit digits at once
-//      (LDS histograms, one global atomic per bin per workgroup);
-//   2. per pass, one kernel: each workgroup scans the pass's digit counts
-//      into exclusive digit offsets (256 entries, cheaper than a launch),
-//      takes the next tile id from an atomic counter (so it only waits on
-//      tiles that already started),
-//      ranks its 4096 keys stably with 64-lane ballot matching, publishes
-//      its per-digit counts, resolves its global offsets by decoupled
-//      look-back, and scatters through an LDS staging tile so that stores
-//      of one digit run are contiguous.
+// Design (Adinets & Merrill, "Onesweep", 2022), written for wave64, with the
+// digit width RB = 8, 10 or 11 bits a launch parameter (30-bit keys: 4, 3 or
+// 3 passes):
+//   0. the encode kernel counts every pass's digits as it writes the keys
+//      (k_keys.hip), so the sort starts from complete digit histograms;
+//   1. per pass, one kernel: each workgroup scans the pass's 2^RB digit
+//      counts into exclusive digit offsets (cheaper than a launch), takes the
+//      next tile id from an atomic counter (so it only waits on tiles that
+//      already started), ranks its 4096 keys stably with 64-lane ballot
+//      matching (RB ballots per key), publishes its per-digit counts,
+//      resolves its global offsets by decoupled look-back, and scatters
+//      through an LDS staging tile so that stores of one digit run are
+//      contiguous.
+// The histograms, tile counters and look-back words must be zero when a
+// sort starts; the tree build that follows every sort clears them for the
+// next step (sort_clear_words), so no clearing kernel runs per step.
 // Per pass the HBM traffic is one read and one write of keys and values.
 #include "nbody_internal.h"
 
@@ -27,11 +32,11 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;
-static_assert(kThreads == 256, "one thread per 8-bit digit");
+static_assert(kTile == kSortTile, "tile size");
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
-constexpr int kMaxPasses = 8;
+constexpr int kLook = 8;  // predecessors read per look-back round trip
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -55,40 +60,44 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
     return before + x - v;
 }
 
-template <typename K>
-__global__ __launch_bounds__(kThreads) void hist_kernel(const K *__restrict__ keys, int64_t n,
-                                                        int passes, uint32_t *__restrict__ hist) {
-    __shared__ uint32_t sh[kMaxPasses * 256];
-    for (int i = threadIdx.x; i < passes * 256; i += kThreads) sh[i] = 0;
-    __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads) {
-        const K k = keys[i];
-        for (int p = 0; p < passes; ++p) atomicAdd(&sh[p * 256 + (uint32_t)((k >> (8 * p)) & 255u)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < passes * 256; i += kThreads)
-        if (sh[i]) atomicAdd(&hist[i], sh[i]);
-}
-
-template <typename K>
+// One pass over digit (k >> shift) & (2^RB - 1).  Thread t owns digits
+// [t * DPT, t * DPT + DPT) for the scans and the look-back.
+template <typename K, int RB>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_count,
     uint32_t *lookback, uint32_t *counter) {
+    constexpr int D = 1 << RB;
+    constexpr int DPT = D / kThreads;
+    constexpr uint32_t kMask = (uint32_t)D - 1u;
+    static_assert(DPT >= 1 && DPT <= 8, "digits per thread");
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wcnt[kWaves * 256];  // wave digit counts, then wave offsets
-    __shared__ uint32_t s_lstart[256];         // digit start inside the tile
-    __shared__ uint32_t s_gstart[256];         // digit start in the output
+    __shared__ uint32_t s_wcnt[kWaves * D];  // wave digit counts, then wave offsets
+    __shared__ uint32_t s_lstart[D];         // digit start inside the tile
+    __shared__ uint32_t s_gstart[D];         // digit start in the output
     __shared__ uint32_t s_sh4[kWaves];
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
-    for (int i = tid; i < kWaves * 256; i += kThreads) s_wcnt[i] = 0;
-    // thread d: where digit d starts in the output (exclusive digit scan)
-    const uint32_t digit_base = block_exclusive_scan(digit_count[tid], s_sh4);
+    for (int i = tid; i < kWaves * D; i += kThreads) s_wcnt[i] = 0;
+    // where each owned digit starts in the output (exclusive digit scan)
+    uint32_t dbase[DPT];
+    {
+        uint32_t c[DPT], sum = 0;
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            c[j] = digit_count[tid * DPT + j];
+            sum += c[j];
+        }
+        uint32_t run = block_exclusive_scan(sum, s_sh4);  // its barriers publish s_tile
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            dbase[j] = run;
+            run += c[j];
+        }
+    }
     const uint32_t tile = s_tile;
     const int64_t base = (int64_t)tile * kTile;
 
@@ -104,18 +113,18 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         const bool ok = p < n;
         k[it] = ok ? kin[p] : (K)0;
         v[it] = ok ? vin[p] : 0u;
-        dg[it] = ok ? (uint32_t)((k[it] >> shift) & 255u) : 256u;
+        dg[it] = ok ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
     }
 
     // Stable rank inside the wave: peers = lanes holding the same digit.
-    uint32_t *wc = s_wcnt + wid * 256;
+    uint32_t *wc = s_wcnt + wid * D;
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint32_t d = dg[it];
-        const bool ok = d < 256u;
+        const bool ok = d < (uint32_t)D;
         uint64_t peers = __ballot(ok);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bb = __ballot(bit);
             peers &= bit ? bb : ~bb;
@@ -131,49 +140,95 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     }
     __syncthreads();
 
-    // Thread d owns digit d: wave offsets, tile offsets, look-back.
-    const uint32_t d = tid;
-    uint32_t total = 0;
+    // Owned digits: wave offsets, tile offsets, look-back.
+    uint32_t total[DPT], tsum = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        const uint32_t c = s_wcnt[w * 256 + d];
-        s_wcnt[w * 256 + d] = total;
-        total += c;
-    }
-    const uint32_t lstart = block_exclusive_scan(total, s_sh4);
-    s_lstart[d] = lstart;
-
-    uint32_t *mine = lookback + (int64_t)tile * 256;
-    uint32_t prefix = 0;
-    if (tile == 0) {
-        __hip_atomic_store(&mine[d], kFlagPrefix | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(&mine[d], kFlagAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int64_t j = (int64_t)tile - 1;
-        for (;;) {
-            const uint32_t w = __hip_atomic_load(&lookback[j * 256 + d], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t f = w & ~kValMask;
-            if (f == 0u) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            prefix += w & kValMask;
-            if (f == kFlagPrefix) break;
-            --j;
+    for (int j = 0; j < DPT; ++j) {
+        const int d = tid * DPT + j;
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_wcnt[w * D + d];
+            s_wcnt[w * D + d] = t;
+            t += c;
         }
-        __hip_atomic_store(&mine[d], kFlagPrefix | (prefix + total), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        total[j] = t;
+        tsum += t;
     }
-    s_gstart[d] = digit_base + prefix;
+    {
+        uint32_t run = block_exclusive_scan(tsum, s_sh4);
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            s_lstart[tid * DPT + j] = run;
+            run += total[j];
+        }
+    }
+
+    uint32_t *mine = lookback + (int64_t)tile * D + tid * DPT;
+    uint32_t prefix[DPT];
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        prefix[j] = 0;
+        __hip_atomic_store(&mine[j], (tile == 0 ? kFlagPrefix : kFlagAgg) | total[j],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (tile != 0) {
+        // All owned digits look back together, each from its own
+        // predecessor, reading a window of kLook predecessors per round trip
+        // (their aggregates are summed in order up to the first inclusive
+        // prefix; an unpublished word ends the round and is polled again).
+        int64_t pj[DPT];
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) pj[j] = (int64_t)tile - 1;
+        uint32_t pend = (1u << DPT) - 1u;
+        while (pend) {
+            uint32_t w[DPT][kLook];
+#pragma unroll
+            for (int j = 0; j < DPT; ++j)
+#pragma unroll
+                for (int q = 0; q < kLook; ++q)
+                    w[j][q] = ((pend >> j) & 1u) && pj[j] - q >= 0
+                                  ? __hip_atomic_load(&lookback[(pj[j] - q) * D + tid * DPT + j],
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0u;
+            uint32_t waiting = 0;
+#pragma unroll
+            for (int j = 0; j < DPT; ++j) {
+                if (!((pend >> j) & 1u)) continue;
+                int used = 0;
+#pragma unroll
+                for (int q = 0; q < kLook; ++q) {
+                    const uint32_t f = w[j][q] & ~kValMask;
+                    if (f == 0u) {
+                        waiting = 1;
+                        break;
+                    }
+                    prefix[j] += w[j][q] & kValMask;
+                    ++used;
+                    if (f == kFlagPrefix) {
+                        pend &= ~(1u << j);
+                        break;
+                    }
+                }
+                pj[j] -= used;
+            }
+            if (waiting) __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int j = 0; j < DPT; ++j)
+            __hip_atomic_store(&mine[j], kFlagPrefix | (prefix[j] + total[j]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) s_gstart[tid * DPT + j] = dbase[j] + prefix[j];
     __syncthreads();
 
     // Scatter into the LDS tile in digit order, then stream out.
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const uint32_t dd = dg[it];
-        if (dd < 256u) {
-            const uint32_t lp = s_lstart[dd] + s_wcnt[wid * 256 + dd] + rank[it];
+        if (dd < (uint32_t)D) {
+            const uint32_t lp = s_lstart[dd] + s_wcnt[wid * D + dd] + rank[it];
             s_keys[lp] = k[it];
             s_vals[lp] = v[it];
         }
@@ -186,7 +241,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         const int p = it * kThreads + tid;
         if (p < tile_n) {
             const K kk = s_keys[p];
-            const uint32_t dd = (uint32_t)((kk >> shift) & 255u);
+            const uint32_t dd = (uint32_t)((kk >> shift) & kMask);
             const uint32_t dst = s_gstart[dd] + (uint32_t)(p - (int)s_lstart[dd]);
             kout[dst] = kk;
             vout[dst] = s_vals[p];
@@ -194,20 +249,16 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     }
 }
 
-template <typename K>
-int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int bits,
-              const SortScratch &sc, hipStream_t s) {
-    if (n <= 1) return 0;
-    const int passes = (bits + 7) / 8;
+template <typename K, int RB>
+int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int passes,
+            const SortScratch &sc, hipStream_t s) {
+    constexpr int D = 1 << RB;
     const int64_t tiles = sort_tiles(n);
-    launch_clear_u32(sc.hist, sc.words, s);
-    int64_t hblocks = tiles < 1024 ? tiles : 1024;
-    hist_kernel<K><<<(unsigned)hblocks, kThreads, 0, s>>>(keys_a, n, passes, sc.hist);
     K *ki = keys_a, *ko = keys_b;
     uint32_t *vi = vals_a, *vo = vals_b;
     for (int p = 0; p < passes; ++p) {
-        onesweep_kernel<K><<<(unsigned)tiles, kThreads, 0, s>>>(
-            ki, vi, ko, vo, n, 8 * p, sc.hist + p * 256, sc.lookback + (int64_t)p * tiles * 256,
+        onesweep_kernel<K, RB><<<(unsigned)tiles, kThreads, 0, s>>>(
+            ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
             sc.counters + p);
         K *tk = ki;
         ki = ko;
@@ -219,31 +270,51 @@ int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t 
     return passes & 1;
 }
 
+template <typename K>
+int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int key_bits,
+              int radix_bits, const SortScratch &sc, hipStream_t s) {
+    if (n <= 1) return 0;
+    const int passes = sort_passes(key_bits, radix_bits);
+    if (radix_bits == 11) return sort_rb<K, 11>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, s);
+    if (radix_bits == 10) return sort_rb<K, 10>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, s);
+    return sort_rb<K, 8>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, s);
+}
+
 }  // namespace
 
 int64_t sort_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
 
-int64_t sort_scratch_words(int64_t n, int passes) {
-    // hist and counters live in front of the look-back words; the clear
-    // kernel zeroes all three in one launch.
-    return (int64_t)passes * 256 + (int64_t)passes * sort_tiles(n) * 256 + 64;
+int sort_passes(int key_bits, int radix_bits) { return (key_bits + radix_bits - 1) / radix_bits; }
+
+// Histograms (kSortMaxPasses x 2^11 words), 64 counter words, then the
+// look-back words of the widest configuration: 4 passes of 2^11 digits
+// (38-bit ensemble keys at 11 bits) per tile.
+int64_t sort_scratch_words(int64_t cap) {
+    return kSortHistWords + 64 + (int64_t)4 * 2048 * sort_tiles(cap);
 }
 
-void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t n, int passes) {
+int64_t sort_clear_words(int64_t n, int key_bits, int radix_bits) {
+    return kSortHistWords + 64 +
+           (int64_t)sort_passes(key_bits, radix_bits) * sort_tiles(n) * ((int64_t)1 << radix_bits);
+}
+
+void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t cap) {
     sc.hist = base;
-    sc.counters = base + (int64_t)passes * 256;
+    sc.counters = base + kSortHistWords;
     sc.lookback = sc.counters + 64;
-    sc.words = sort_scratch_words(n, passes);
+    sc.words = sort_scratch_words(cap);
 }
 
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
-                    int64_t n, int bits, const SortScratch &sc, hipStream_t s) {
-    return sort_impl<uint32_t>(keys_a, vals_a, keys_b, vals_b, n, bits, sc, s);
+                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
+                    hipStream_t s) {
+    return sort_impl<uint32_t>(keys_a, vals_a, keys_b, vals_b, n, key_bits, radix_bits, sc, s);
 }
 
 int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,
-                    int64_t n, int bits, const SortScratch &sc, hipStream_t s) {
-    return sort_impl<uint64_t>(keys_a, vals_a, keys_b, vals_b, n, bits, sc, s);
+                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
+                    hipStream_t s) {
+    return sort_impl<uint64_t>(keys_a, vals_a, keys_b, vals_b, n, key_bits, radix_bits, sc, s);
 }
 
 }  // namespace nbs

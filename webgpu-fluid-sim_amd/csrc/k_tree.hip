@@ -37,8 +37,37 @@ namespace nbs {
 
 namespace {
 
+// Sorted keys as the build reads them: the workgroup's own keys and a halo
+// of kBuildHalo keys on either side from LDS, farther keys from memory.  The
+// searches of a node probe its neighbourhood first, so almost every probe
+// is an LDS read: only nodes with wide leaf ranges go to memory, for their
+// first (farthest) probes.
+constexpr int kBuildThreads = 256;
+constexpr int kBuildHalo = 1024;
+constexpr int kBuildWindow = kBuildThreads + 2 * kBuildHalo;
+template <typename K>
+struct KeyWindow {
+    const K *__restrict__ g;  // all keys
+    const K *s;               // LDS copy of keys [lo, hi)
+    int64_t lo, hi;
+    __device__ __forceinline__ K operator[](int64_t j) const {
+        return (j >= lo && j < hi) ? s[j - lo] : g[j];
+    }
+};
+// Load the window of the workgroup's nodes [b0, b0 + 256) into s_keys.
+template <typename K>
+__device__ __forceinline__ KeyWindow<K> load_window(const K *__restrict__ keys, int64_t n,
+                                                    K *s_keys) {
+    const int64_t b0 = (int64_t)blockIdx.x * kBuildThreads;
+    const int64_t lo = b0 > kBuildHalo ? b0 - kBuildHalo : 0;
+    const int64_t hi = b0 + kBuildThreads + kBuildHalo < n ? b0 + kBuildThreads + kBuildHalo : n;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kBuildThreads) s_keys[j - lo] = keys[j];
+    __syncthreads();
+    return KeyWindow<K>{keys, s_keys, lo, hi};
+}
+
 struct DeltaU32 {
-    const uint32_t *__restrict__ k;
+    KeyWindow<uint32_t> k;
     int64_t n;
     // lbvh.wgsl:72-83
     __device__ __forceinline__ int operator()(int64_t i, int64_t j) const {
@@ -53,7 +82,7 @@ struct DeltaU32 {
 // are tie-broken on sim-local sorted positions, so every sim's sub-tree has
 // the shape of the tree a plain run of that sim builds.
 struct DeltaU64 {
-    const uint64_t *__restrict__ k;
+    KeyWindow<uint64_t> k;
     const int64_t *__restrict__ sim_off;
     int64_t n;
     __device__ __forceinline__ int operator()(int64_t i, int64_t j) const {
@@ -128,11 +157,27 @@ __device__ __forceinline__ Split karras_node(const Delta &delta, int64_t i, int6
     return r;
 }
 
-__global__ __launch_bounds__(256) void build_u32_kernel(const uint32_t *__restrict__ keys,
-                                                        int64_t n, TreeBufs tb) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+// Zero the sort's scratch for the next step (every sort is followed by a
+// build, which runs after the sort has finished with it), and the walk's
+// bounding-box accumulators (the encode has read them; the walk that
+// follows refills them).
+__device__ __forceinline__ void clear_words(uint32_t *__restrict__ p, int64_t words,
+                                            uint32_t *__restrict__ slots) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+    if (slots && blockIdx.x == 0 && threadIdx.x < kBBoxSlots)
+        reinterpret_cast<uint4 *>(slots)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
+    const uint32_t *__restrict__ keys, int64_t n, TreeBufs tb, uint32_t *__restrict__ clear,
+    int64_t clear_n, uint32_t *__restrict__ slots) {
+    __shared__ uint32_t s_keys[kBuildWindow];
+    clear_words(clear, clear_n, slots);
+    const DeltaU32 delta{load_window(keys, n, s_keys), n};
+    const int64_t i = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
     if (i >= n - 1) return;
-    DeltaU32 delta{keys, n};
     const Split sp = karras_node(delta, i, n);
     tb.child[i] = make_uint2(sp.lc, sp.rc);
     // the walk reads the indices from the pair record
@@ -146,14 +191,19 @@ __global__ __launch_bounds__(256) void build_u32_kernel(const uint32_t *__restri
     tb.status[i] = 0u;  // lbvh.wgsl:164
 }
 
-__global__ __launch_bounds__(256) void build_u64_kernel(const uint64_t *__restrict__ keys,
+__global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t *__restrict__ keys,
                                                         int64_t n,
                                                         const int64_t *__restrict__ sim_off,
                                                         TreeBufs tb,
-                                                        uint32_t *__restrict__ sim_root) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+                                                        uint32_t *__restrict__ sim_root,
+                                                        uint32_t *__restrict__ clear,
+                                                        int64_t clear_n,
+                                                        uint32_t *__restrict__ slots) {
+    __shared__ uint64_t s_keys[kBuildWindow];
+    clear_words(clear, clear_n, slots);
+    const DeltaU64 delta{load_window(keys, n, s_keys), sim_off, n};
+    const int64_t i = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
     if (i >= n - 1) return;
-    DeltaU64 delta{keys, sim_off, n};
     const Split sp = karras_node(delta, i, n);
     tb.child[i] = make_uint2(sp.lc, sp.rc);
     // the walk reads the indices from the pair record
@@ -567,16 +617,19 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
 
 }  // namespace
 
-void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, hipStream_t s) {
+void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, uint32_t *clear,
+                      int64_t clear_n, uint32_t *slots, hipStream_t s) {
     if (n < 2) return;
-    build_u32_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, t);
+    build_u32_kernel<<<(unsigned)((n - 1 + kBuildThreads - 1) / kBuildThreads), kBuildThreads, 0,
+                       s>>>(keys, n, t, clear, clear_n, slots);
 }
 
 void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
-                      uint32_t *sim_root, hipStream_t s) {
+                      uint32_t *sim_root, uint32_t *clear, int64_t clear_n, uint32_t *slots,
+                      hipStream_t s) {
     if (n < 2) return;
-    build_u64_kernel<<<(unsigned)((n - 1 + 255) / 256), 256, 0, s>>>(keys, n, sim_off, t,
-                                                                     sim_root);
+    build_u64_kernel<<<(unsigned)((n - 1 + kBuildThreads - 1) / kBuildThreads), kBuildThreads, 0,
+                       s>>>(keys, n, sim_off, t, sim_root, clear, clear_n, slots);
 }
 
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,

@@ -67,11 +67,51 @@ __device__ __forceinline__ float accel_out(float sum, float G) {
         return G * sum;
 }
 
+// Bounding box of a lane's drifted positions as order-preserving images
+// (float_order; 0 = none, numbers only, like fminf/fmaxf).
+struct BoxAcc {
+    uint32_t nx = 0, ny = 0, xx = 0, xy = 0;
+    __device__ __forceinline__ void add(float2 x) {
+        if (x.x == x.x) {
+            nx = max(nx, ~float_order(x.x));
+            xx = max(xx, float_order(x.x));
+        }
+        if (x.y == x.y) {
+            ny = max(ny, ~float_order(x.y));
+            xy = max(xy, float_order(x.y));
+        }
+    }
+};
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
+// Fold the wave's drifted positions into the bounding-box accumulators the
+// next step's encode reads (one slot per wave index mod kBBoxSlots; an
+// atomic max per non-empty component).  Called by whole wavefronts.
+__device__ __forceinline__ void fold_bbox(uint32_t *slots, BoxAcc b, uint32_t wave) {
+    if (!slots) return;
+    b.nx = wave_max_u32(b.nx);
+    b.ny = wave_max_u32(b.ny);
+    b.xx = wave_max_u32(b.xx);
+    b.xy = wave_max_u32(b.xy);
+    if ((threadIdx.x & 63) == 0) {
+        uint32_t *s = slots + 4 * (wave & (kBBoxSlots - 1));
+        if (b.nx) atomicMax(s + 0, b.nx);
+        if (b.ny) atomicMax(s + 1, b.ny);
+        if (b.xx) atomicMax(s + 2, b.xx);
+        if (b.xy) atomicMax(s + 3, b.xy);
+    }
+}
+
 // Kick and drift of the body at sorted position t with acceleration
-// (ax, ay) (barnes_hut.wgsl:110, 120), written by body index.
+// (ax, ay) (barnes_hut.wgsl:110, 120), written by body index; returns the
+// drifted position.
 template <bool kExact>
-__device__ __forceinline__ void kick_drift(const WalkArgs &a, uint32_t t, float px, float py,
-                                           float ax, float ay) {
+__device__ __forceinline__ float2 kick_drift(const WalkArgs &a, uint32_t t, float px, float py,
+                                             float ax, float ay) {
     const uint32_t bi = a.idx[t];
     const float dt = a.p.dt;
     float2 v = a.vel[bi];
@@ -90,6 +130,7 @@ __device__ __forceinline__ void kick_drift(const WalkArgs &a, uint32_t t, float 
     a.vel[bi] = v;
     a.pos[bi] = x;
     if (a.accel) a.accel[bi] = make_float2(ax, ay);
+    return x;
 }
 
 // One interaction of body (px, py, m1) with node record c.  Returns true
@@ -117,9 +158,33 @@ __device__ __forceinline__ float4 node_record(const WalkArgs &a, uint32_t node) 
     return a.pcm[pcm_slot(pw)];
 }
 
+// XCD-aware workgroup order.  The hardware deals workgroups round-robin over
+// the 8 XCDs (workgroup b runs on XCD b % 8), and each XCD has its own L2.
+// With run > 0, XCD x walks runs of `run` consecutive logical workgroups
+// (Morton-consecutive bodies, so its L2 holds their region's sub-tree), the
+// runs dealt to the XCDs in turn; run = 0 keeps the hardware order.  The
+// grid is a multiple of 8 * run; logical workgroups past the bodies exit.
+__device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t run) {
+    if (!run) return b;
+    const uint32_t x = b & 7u, i = b >> 3;
+    const uint32_t r = i / run;
+    return (r * 8u + x) * run + (i - r * run);
+}
+
+// Grid and run length for `blocks` logical workgroups: xcd_order 0 =
+// hardware order, 1 = one run per XCD, k > 1 = runs of k workgroups.
+void xcd_grid(uint32_t blocks, int xcd_order, uint32_t &grid, uint32_t &run) {
+    grid = blocks;
+    run = 0;
+    if (xcd_order <= 0 || blocks < 16) return;
+    run = xcd_order == 1 ? (blocks + 7u) / 8u : (uint32_t)xcd_order;
+    const uint32_t span = 8u * run;
+    grid = (blocks + span - 1u) / span * span;
+}
+
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
 __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_cap,
-                                                    uint32_t xcd_per) {
+                                                    uint32_t xcd_run) {
     extern __shared__ float4 s_dyn[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int waves = blockDim.x >> 6;
@@ -131,11 +196,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
                            s_dyn + (size_t)waves * stack_cap) + (size_t)waves * stack_cap) +
                        (size_t)wid * stack_cap;
 
-    // XCD-aware order: hardware deals workgroups round-robin over the 8 XCDs;
-    // remap so each XCD walks one contiguous run of Morton-ordered bodies and
-    // its L2 holds that region's sub-tree.
-    uint32_t block = blockIdx.x;
-    if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
+    const uint32_t block = logical_block(blockIdx.x, xcd_run);
 
     const uint32_t n = a.n;
     const uint32_t t = block * blockDim.x + threadIdx.x;  // sorted position
@@ -253,9 +314,11 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
         c = s_data[sp];
     }
 
+    BoxAcc box;
     if (valid)
-        kick_drift<kExact>(a, t, px, py, accel_out<kExact>(ax, p.grav),
-                           accel_out<kExact>(ay, p.grav));
+        box.add(kick_drift<kExact>(a, t, px, py, accel_out<kExact>(ax, p.grav),
+                                   accel_out<kExact>(ay, p.grav)));
+    fold_bbox(a.bbox_slots, box, t >> 6);
     if constexpr (kStats) {
         // wave sums, one atomic per wave and counter
         uint32_t s0 = n_acc, s1 = n_open, s2 = n_drop;
@@ -277,10 +340,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a, int stack_
 // 64 * kB Morton-consecutive bodies share one traversal (walk_common.h).
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, int kB>
 __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int stack_cap,
-                                                       uint32_t xcd_per) {
+                                                       uint32_t xcd_run) {
     const int lane = threadIdx.x & 63;
-    uint32_t block = blockIdx.x;
-    if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
+    const uint32_t block = logical_block(blockIdx.x, xcd_run);
     const uint32_t n = a.n;
     const uint32_t first = ((block * blockDim.x + threadIdx.x) >> 6) * (64u * kB);
     const uint32_t leaf0 = n - 1u;
@@ -383,12 +445,14 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
             }
         });
 
+    BoxAcc box;
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
         if (valid[b])
-            kick_drift<kExact>(a, t[b], px[b], py[b], accel_out<kExact>(ax[b], p.grav),
-                               accel_out<kExact>(ay[b], p.grav));
+            box.add(kick_drift<kExact>(a, t[b], px[b], py[b], accel_out<kExact>(ax[b], p.grav),
+                                       accel_out<kExact>(ay[b], p.grav)));
     }
+    fold_bbox(a.bbox_slots, box, first >> 6);
     if constexpr (kStats) {
         // the counters are wave-uniform: one atomic per wave and counter
         if (lane == 0) {
@@ -464,11 +528,10 @@ struct PairStack {
 // own leaf is accepted: its offset is exactly zero, so with a finite pull
 // factor m * inv^3 (the host checks that eps^2 and the largest mass keep it
 // finite) its term is an exact +0 and the sum is unchanged.
-template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf>
-__global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint32_t xcd_per) {
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32>
+__global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint32_t xcd_run) {
     const int lane = threadIdx.x & 63;
-    uint32_t block = blockIdx.x;
-    if (xcd_per) block = (blockIdx.x & 7u) * xcd_per + (blockIdx.x >> 3);
+    const uint32_t block = logical_block(blockIdx.x, xcd_run);
     const uint32_t n = a.n;
     const uint32_t t = block * blockDim.x + threadIdx.x;  // sorted position
     const bool valid = t < n;
@@ -495,12 +558,13 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
         inv = inv_dist<false>(rx, ry, eps2);
         return passes(c.w, inv, p.theta);
     };
-    // a + f r for the lanes of `acc` (f = m inv^3: add_pull's fast form)
+    // a + f r for the lanes of `acc` (f = m inv^3: add_pull's fast form);
+    // the other lanes add 0 * r, an exact zero for a finite offset (the sum
+    // is never -0), so the factor is selected instead of both sums
     auto accumulate = [&](float f, float rx, float ry, uint64_t acc) {
-        const float nx = __builtin_fmaf(f, rx, ax), ny = __builtin_fmaf(f, ry, ay);
-        const bool in = __builtin_amdgcn_inverse_ballot_w64(acc);
-        ax = in ? nx : ax;
-        ay = in ? ny : ay;
+        const float fs = __builtin_amdgcn_inverse_ballot_w64(acc) ? f : 0.0f;
+        ax = __builtin_fmaf(fs, rx, ax);
+        ay = __builtin_fmaf(fs, ry, ay);
     };
     PairStack st;
     st.put(0, 0u, 0ull);  // the sentinel
@@ -584,34 +648,38 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
             }
             // The next entry: the right child first, the left one waits on
             // the stack (written to the free slot every time, kept by
-            // advancing the top); with neither child opened, the top entry
-            // (read every time: a branch here would split the loop).  Flags
-            // are 0/1 integers: booleans here would be materialised as lane
-            // masks.
+            // advancing the top); with neither child opened, the top entry.
+            // Flags are 0/1 integers: booleans here would be materialised as
+            // lane masks.
             const uint32_t hl = nonzero_flag(openl), hr = nonzero_flag(openr);
-            uint32_t tnode;
-            uint64_t tmask;
-            st.get(sp - 1, tnode, tmask);
             st.put(sp, ch.x, openl);
             const uint32_t top = (uint32_t)sp + (hl & hr);
-            n_over += top >> 6;  // top == 64: the left child is dropped
-            if constexpr (kStats) {
-                if (top >> 6) {
-                    n_open -= __builtin_popcountll(openl);
-                    n_drop += __builtin_popcountll(openl);
+            if constexpr (kEnsemble || kStats) {
+                n_over += top >> 6;  // top == 64: the left child is dropped
+                if constexpr (kStats) {
+                    if (top >> 6) {
+                        n_open -= __builtin_popcountll(openl);
+                        n_drop += __builtin_popcountll(openl);
+                    }
                 }
             }
             const uint32_t any = hl | hr;
             uint32_t next = hr ? ch.y : ch.x;
             uint64_t nmask = hr ? openr : openl;
-            next = any ? next : tnode;
-            nmask = any ? nmask : tmask;  // the sentinel's empty mask ends the walk
             sp = any ? (int)min(top, (uint32_t)kPairStackTop) : sp - 1;
+            if (!any) st.get(sp, next, nmask);  // the sentinel's empty mask ends the walk
             // the next record's loads, issued before this node's pulls
-            rb = 4 * kPairStride * (size_t)next;
-            cl = make_float4(pcm_s[rb + 0], pcm_s[rb + 1], pcm_s[rb + 2], pcm_s[rb + 3]);
-            cr = make_float4(pcm_s[rb + 4], pcm_s[rb + 5], pcm_s[rb + 6], pcm_s[rb + 7]);
-            ch = make_uint2(pcmu_s[rb + 8], pcmu_s[rb + 9]);
+            {
+                const_as<float> *rf;
+                if constexpr (kOff32)  // byte offset below 2^32 (n <= 2^26): an SGPR offset
+                    rf = (const_as<float> *)((const_as<char> *)pcm_s + (uint32_t)(next * 64u));
+                else
+                    rf = pcm_s + 4 * kPairStride * (size_t)next;
+                const_as<uint32_t> *ru = (const_as<uint32_t> *)rf;
+                cl = make_float4(rf[0], rf[1], rf[2], rf[3]);
+                cr = make_float4(rf[4], rf[5], rf[6], rf[7]);
+                ch = make_uint2(ru[8], ru[9]);
+            }
             // keep the pulls behind the loads in the schedule
             __builtin_amdgcn_sched_barrier(0);
             accumulate(fr, rxr, ryr, accr);
@@ -622,8 +690,11 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
     }
     if (n_over) *a.overflow = 1u;
 
+    BoxAcc box;
     if (valid)
-        kick_drift<false>(a, t, px, py, accel_out<false>(ax, p.grav), accel_out<false>(ay, p.grav));
+        box.add(kick_drift<false>(a, t, px, py, accel_out<false>(ax, p.grav),
+                                  accel_out<false>(ay, p.grav)));
+    fold_bbox(a.bbox_slots, box, t >> 6);
     if constexpr (kStats) {
         // the counters are wave-uniform: one atomic per wave and counter
         if (lane == 0) {
@@ -638,31 +709,26 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
 int rs_stack_cap(int stack) { return stack < 64 ? 64 : (stack > 128 ? 128 : stack); }
 
 template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf>
-void launch_pair(const WalkArgs &a, int block, bool xcd, hipStream_t s) {
-    uint32_t grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
-    uint32_t per = 0;
-    if (xcd && grid >= 16) {
-        per = (grid + 7u) / 8u;
-        grid = per * 8u;  // blocks past the bodies exit at once
-    }
-    walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf><<<grid, block, 0, s>>>(a, per);
+void launch_pair(const WalkArgs &a, int block, int xcd, hipStream_t s) {
+    uint32_t grid, run;
+    xcd_grid((a.n + (unsigned)block - 1u) / (unsigned)block, xcd, grid, run);
+    if (a.n <= (1u << 26))
+        walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf, true><<<grid, block, 0, s>>>(a, run);
+    else
+        walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf, false><<<grid, block, 0, s>>>(a, run);
 }
 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, int kB>
-void launch_rs(const WalkArgs &a, int block, int stack, bool xcd, hipStream_t s) {
+void launch_rs(const WalkArgs &a, int block, int stack, int xcd, hipStream_t s) {
     const uint32_t per_block = (uint32_t)block * kB;
-    uint32_t grid = (a.n + per_block - 1u) / per_block;
-    uint32_t per = 0;
-    if (xcd && grid >= 16) {
-        per = (grid + 7u) / 8u;
-        grid = per * 8u;  // blocks past the bodies exit at once
-    }
+    uint32_t grid, run;
+    xcd_grid((a.n + per_block - 1u) / per_block, xcd, grid, run);
     walk_rs_kernel<kExact, kRefSoft, kEnsemble, kStats, kB>
-        <<<grid, block, 0, s>>>(a, rs_stack_cap(stack), per);
+        <<<grid, block, 0, s>>>(a, rs_stack_cap(stack), run);
 }
 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
-void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, int lane_bodies,
+void launch_t(const WalkArgs &a, int block, int stack, int xcd, int variant, int lane_bodies,
               hipStream_t s) {
     if constexpr (!kExact) {
         if (variant == 2) {
@@ -691,19 +757,15 @@ void launch_t(const WalkArgs &a, int block, int stack, bool xcd, int variant, in
             launch_rs<kExact, kRefSoft, kEnsemble, kStats, 1>(a, block, stack, xcd, s);
         return;
     }
-    uint32_t grid = (a.n + (unsigned)block - 1u) / (unsigned)block;
-    uint32_t per = 0;
-    if (xcd && grid >= 16) {
-        per = (grid + 7u) / 8u;
-        grid = per * 8u;  // blocks past the bodies exit at once
-    }
+    uint32_t grid, run;
+    xcd_grid((a.n + (unsigned)block - 1u) / (unsigned)block, xcd, grid, run);
     const size_t lds = (size_t)(block / kWave) * (size_t)stack *
                        (sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t));
-    walk_kernel<kExact, kRefSoft, kEnsemble, kStats><<<grid, block, lds, s>>>(a, stack, per);
+    walk_kernel<kExact, kRefSoft, kEnsemble, kStats><<<grid, block, lds, s>>>(a, stack, run);
 }
 
 template <bool kExact, bool kRefSoft>
-void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, bool xcd,
+void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, int xcd,
               int variant, int lb, hipStream_t s) {
     if (ens) {
         if (stats)
@@ -721,20 +783,20 @@ void launch_e(const WalkArgs &a, bool ens, bool stats, int block, int stack, boo
 }  // namespace
 
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 bool xcd_remap, int variant, int lane_bodies, hipStream_t s) {
+                 int xcd_order, int variant, int lane_bodies, hipStream_t s) {
     if (a.n == 0) return;
     const bool exact = a.p.math_mode != 0;
     const bool refsoft = a.p.soft_mode == 0;
     if (exact) {
         if (refsoft)
-            launch_e<true, true>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
+            launch_e<true, true>(a, ensemble, stats, block, stack, xcd_order, variant, lane_bodies, s);
         else
-            launch_e<true, false>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
+            launch_e<true, false>(a, ensemble, stats, block, stack, xcd_order, variant, lane_bodies, s);
     } else {
         if (refsoft)
-            launch_e<false, true>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
+            launch_e<false, true>(a, ensemble, stats, block, stack, xcd_order, variant, lane_bodies, s);
         else
-            launch_e<false, false>(a, ensemble, stats, block, stack, xcd_remap, variant, lane_bodies, s);
+            launch_e<false, false>(a, ensemble, stats, block, stack, xcd_order, variant, lane_bodies, s);
     }
 }
 

@@ -12,6 +12,20 @@ constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 constexpr int kSimShift = 30;     // ensemble keys: sim id << 30 | 30-bit Morton code
 constexpr int kMaxSims = 256;     // 8-bit sim prefix
 constexpr int kBBoxChunk = 4096;  // bodies per bounding-box partial
+constexpr int kBBoxSlots = 64;     // bounding-box accumulators filled by the walk
+
+// Order-preserving u32 image of a binary32 number (larger number, larger
+// image; -0 below +0) for atomic min/max.  Images of numbers are never 0,
+// so 0 stands for "no value": the bounding-box accumulators hold the
+// maximum of image(x) for the maximum and of ~image(x) for the minimum.
+__host__ __device__ inline uint32_t float_order(float x) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, x);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ inline float float_unorder(uint32_t u) {
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    return __builtin_bit_cast(float, b);
+}
 
 // Per-step physics parameters (the Metadata uniform, lbvh.wgsl:3-10).
 struct StepParams {
@@ -25,31 +39,46 @@ struct StepParams {
     int math_mode;
 };
 
+// ---------------------------------------------------------------- sort scratch
+constexpr int kSortTile = 4096;       // keys per one-sweep tile
+constexpr int kSortMaxPasses = 5;     // 38-bit ensemble keys at 8 bits per digit
+constexpr int kSortHistWords = kSortMaxPasses * 2048;  // [pass][2^radix_bits] digit counts
+struct SortScratch {
+    uint32_t *hist;      // [passes][2^radix_bits] digit counts (written by the encode)
+    uint32_t *lookback;  // [passes][tiles][2^radix_bits] decoupled look-back words
+    uint32_t *counters;  // [passes] dynamic tile ids
+    int64_t words;       // u32 words of the whole scratch
+};
+int64_t sort_tiles(int64_t n);
+int sort_passes(int key_bits, int radix_bits);
+int64_t sort_scratch_words(int64_t cap);  // enough for every radix width
+// Words to clear (from sc.hist on) after a sort of n keys, for the next one.
+int64_t sort_clear_words(int64_t n, int key_bits, int radix_bits);
+void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t cap);
+
 // ---------------------------------------------------------------- keys
 // bbox partials: grid (chunks, sims); then one wave per sim reduces them.
+// slots != nullptr (one sim): also set the walk's accumulators to this box.
 void launch_bbox(const float2 *pos, const int64_t *sim_off, int num_sims, int max_chunks,
-                 float4 *partial, float4 *bbox, hipStream_t s);
-// Morton encode.  keys32 (plain) or keys64 (ensemble, body_sim != nullptr).
-void launch_encode(const float2 *pos, int64_t n, int morton_mode, const float4 *bbox,
-                   const uint8_t *body_sim, uint32_t *keys32, uint64_t *keys64, uint32_t *idx,
+                 float4 *partial, float4 *bbox, uint32_t *slots, hipStream_t s);
+// Morton encode, keys32 (plain) or keys64 (ensemble, body_sim != nullptr),
+// counting the digits of every sort pass into hist ([passes][2^radix_bits],
+// which must be zero; passes = 0 counts nothing).  slots != nullptr: the
+// bounding box comes from the walk's accumulators (and is stored in bbox[0]).
+void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                   const uint32_t *slots, const uint8_t *body_sim, uint32_t *keys32,
+                   uint64_t *keys64, uint32_t *idx, uint32_t *hist, int passes, int radix_bits,
                    hipStream_t s);
 
 // ---------------------------------------------------------------- sort
-struct SortScratch {
-    uint32_t *hist;      // [passes][256] global digit counts -> exclusive offsets
-    uint32_t *lookback;  // [passes][tiles][256] decoupled look-back words
-    uint32_t *counters;  // [passes] dynamic tile ids
-    int64_t words;       // total u32 words of the three arrays (cleared per sort)
-};
-int64_t sort_tiles(int64_t n);
-int64_t sort_scratch_words(int64_t n, int passes);
-void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t n, int passes);
-// Stable LSD sort of (keys, vals) on the low `bits` bits; ping-pongs between
-// buffers a and b and returns which buffer (0 = a, 1 = b) holds the result.
+// Stable LSD sort of (keys, vals) on the low `key_bits` bits, radix_bits
+// (8, 10 or 11) per pass; the digit histograms must be in sc.hist.
+// Ping-pongs between buffers a and b and returns which buffer (0 = a, 1 = b)
+// holds the result.
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
-                    int64_t n, int bits, const SortScratch &sc, hipStream_t s);
+                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc, hipStream_t s);
 int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,
-                    int64_t n, int bits, const SortScratch &sc, hipStream_t s);
+                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc, hipStream_t s);
 
 // ---------------------------------------------------------------- tree
 // Node data is stored in child pairs: each internal node p has a 64-byte
@@ -90,9 +119,14 @@ struct TreeBufs {
     uint32_t *lists;     // the fill's top node list (fill_list_words)
     uint32_t *slot;      // [n-1] position of a top node in the top list
 };
-void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, hipStream_t s);
+// The builds also zero `clear_words` words at `clear` (the sort's scratch,
+// for the next step's encode and sort) and the kBBoxSlots x 4 words at
+// `slots` (the walk's bounding-box accumulators; may be null).
+void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, uint32_t *clear,
+                      int64_t clear_words, uint32_t *slots, hipStream_t s);
 void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
-                      uint32_t *sim_root, hipStream_t s);
+                      uint32_t *sim_root, uint32_t *clear, int64_t clear_words, uint32_t *slots,
+                      hipStream_t s);
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
                  const TreeBufs &t, bool climb, hipStream_t s);
 int64_t fill_count_words(int64_t cap);  // sizes of counts / top list for `cap` bodies
@@ -118,10 +152,13 @@ struct WalkArgs {
     // constant softening with eps^2 and the largest |mass| such that
     // |m| * eps^-3 stays finite (the offset is exactly zero)
     uint32_t self_zero;
+    // bounding-box accumulators of the drifted positions (the next step's
+    // keys), or null
+    uint32_t *bbox_slots;
     StepParams p;
 };
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,
-                 bool xcd_remap, int variant, int lane_bodies, hipStream_t s);
+                 int xcd_order, int variant, int lane_bodies, hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint
 struct AdjointArgs {

@@ -86,6 +86,7 @@ class NbodyTuning(ctypes.Structure):
         ("walk_variant", ctypes.c_int32),
         ("xcd_order", ctypes.c_int32),
         ("lane_bodies", ctypes.c_int32),
+        ("radix_bits", ctypes.c_int32),
     ]
 
 

@@ -1,13 +1,14 @@
-"""Launch autotuner for the tree walk.
+"""Launch autotuner for the tree walk and the radix sort.
 
 Times each candidate (walk variant: 0 LDS stack, 1 register stack, 2 pair
 walk; bodies per lane of the register-stack walk; workgroup size; stack
-entries; XCD order) on the handle's own bodies with the engine's HIP-event
-phase timers, restores the bodies after every trial, and applies the
-fastest.  Every candidate takes the same opening decisions; variants 0 and 1
-add each body's terms in the reference's order and agree bit for bit, the
-pair walk (fast mode) adds the same terms in its own order, so tuning never
-changes the physics beyond rounding.
+entries; XCD order; then the radix sort's digit width) on the handle's own
+bodies with the engine's HIP-event phase timers, restores the bodies after
+every trial, and applies the fastest.  Every candidate takes the same
+opening decisions and the sort is stable for every width; walk variants 0
+and 1 add each body's terms in the reference's order and agree bit for bit,
+the pair walk (fast mode) adds the same terms in its own order, so tuning
+never changes the physics beyond rounding.
 """
 from __future__ import annotations
 
@@ -15,7 +16,7 @@ import itertools
 
 DEFAULT_VARIANTS = (2, 1, 0)
 DEFAULT_LANE_BODIES = (1, 2, 4)
-DEFAULT_XCD = (1, 0)
+DEFAULT_XCD = (1, 0, 32)
 DEFAULT_BLOCKS = (64, 128, 256, 512)
 DEFAULT_STACKS = (64, 128)
 
@@ -30,14 +31,34 @@ def _candidates(variants, blocks, stacks, xcd, lane_bodies):
             yield (v, b, st, x, lb)
 
 
+DEFAULT_RADIX = (8, 10, 11)
+
+
+def _time(sim, mass, pos, vel, steps, phases):
+    """Mean device time (ms) per step of the given phases, from the bodies
+    (mass, pos, vel), after one warm-up step."""
+    sim.setBodies(mass, pos, vel)
+    sim.step(1)
+    sim.synchronize()
+    sim.phaseTimes(reset=True)
+    sim.step(steps)
+    sim.synchronize()
+    ms, n = sim.phaseTimes(reset=True)
+    return sum(ms[p] for p in phases) / max(n, 1)
+
+
 def autotune(sim, steps: int = 3, variants=DEFAULT_VARIANTS, blocks=DEFAULT_BLOCKS,
              stacks=DEFAULT_STACKS, xcd=DEFAULT_XCD, lane_bodies=DEFAULT_LANE_BODIES,
-             verbose: bool = False):
-    """Pick the fastest walk configuration for `sim` (Simulation or Ensemble).
+             radix=DEFAULT_RADIX, verbose: bool = False):
+    """Pick the fastest walk configuration and radix width for `sim`
+    (Simulation or Ensemble).
 
-    Returns (best, table) where best = (variant, block, stack, xcd_order,
-    lane_bodies) and table maps every tried configuration to its mean walk
-    time in ms."""
+    The walk tunables are timed on the walk phase, the radix width on the
+    encode and sort phases (whose digit counts and passes it sets).  Returns
+    (best, table) where best = (variant, block, stack, xcd_order,
+    lane_bodies, radix_bits) and table maps every tried walk configuration
+    (variant, block, stack, xcd_order, lane_bodies) to its mean walk time and
+    every ("radix_bits", r) to its mean encode + sort time, in ms."""
     mass, pos, vel = sim.getBodies()
     table = {}
     sim.setProfiling(True)
@@ -46,21 +67,22 @@ def autotune(sim, steps: int = 3, variants=DEFAULT_VARIANTS, blocks=DEFAULT_BLOC
             v, b, st, x, lb = cand
             sim.setTuning(walk_block=b, walk_stack=st, walk_variant=v, xcd_order=x,
                           lane_bodies=lb)
-            sim.setBodies(mass, pos, vel)
-            sim.step(1)  # warm-up
-            sim.synchronize()
-            sim.phaseTimes(reset=True)
-            sim.step(steps)
-            sim.synchronize()
-            ms, n = sim.phaseTimes(reset=True)
-            table[cand] = ms["walk"] / max(n, 1)
+            table[cand] = _time(sim, mass, pos, vel, steps, ("walk",))
             if verbose:
                 print(f"variant {v} block {b} stack {st} xcd {x} lane_bodies {lb}: walk "
                       f"{table[cand]:.4f} ms", flush=True)
+        walk = min(table, key=table.get)
+        sim.setTuning(walk_block=walk[1], walk_stack=walk[2], walk_variant=walk[0],
+                      xcd_order=walk[3], lane_bodies=walk[4])
+        for r in radix:
+            sim.setTuning(radix_bits=r)
+            table[("radix_bits", r)] = _time(sim, mass, pos, vel, steps, ("encode", "sort"))
+            if verbose:
+                print(f"radix_bits {r}: encode + sort {table[('radix_bits', r)]:.4f} ms",
+                      flush=True)
+        rb = min(radix, key=lambda r: table[("radix_bits", r)])
+        sim.setTuning(radix_bits=rb)
     finally:
         sim.setProfiling(False)
         sim.setBodies(mass, pos, vel)
-    best = min(table, key=table.get)
-    sim.setTuning(walk_block=best[1], walk_stack=best[2], walk_variant=best[0],
-                  xcd_order=best[3], lane_bodies=best[4])
-    return best, table
+    return walk + (rb,), table

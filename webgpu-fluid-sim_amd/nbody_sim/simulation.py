@@ -121,12 +121,13 @@ class _Handle:
         return {f: getattr(t, f) for f, _ in _lib.NbodyTuning._fields_}
 
     def setTuning(self, walk_block=None, walk_stack=None, walk_variant=None, xcd_order=None,
-                  lane_bodies=None):
-        """Launch tunables (the results are the same for every setting); None keeps."""
+                  lane_bodies=None, radix_bits=None):
+        """Launch tunables (nbody_tuning); None keeps a value.  Every setting takes
+        the same decisions; only walk_variant 2 (fast mode) sums in another order."""
         cur = self.getTuning()
         for k, v in (("walk_block", walk_block), ("walk_stack", walk_stack),
                      ("walk_variant", walk_variant), ("xcd_order", xcd_order),
-                     ("lane_bodies", lane_bodies)):
+                     ("lane_bodies", lane_bodies), ("radix_bits", radix_bits)):
             if v is not None:
                 cur[k] = int(v)
         t = _lib.NbodyTuning(**cur)
