@@ -114,13 +114,18 @@ constexpr int kSuper = 1 << kSuperShift;                         // leaves per s
 // range, so a node whose range lies in super-block s has its index there.
 constexpr uint32_t kLdsList = 1024;  // longest list the fill resolves in LDS
 constexpr uint32_t kScan = kSuper / 1024;  // node indices per thread of the scan
-__host__ __device__ inline uint32_t list3_cap(uint32_t num_super) { return num_super * 96u + 64u; }
 __host__ __device__ inline uint32_t num_supers(int64_t n) {
     return (uint32_t)((n + kSuper - 1) / kSuper);
 }
-// Counter words (tb.counts): [0] phase-2 done count, [1] top list length.
-// The top list (every node whose range spans super-blocks; tb.slot holds
-// each one's position) is tb.lists[0, list3_cap(S)), S = super-blocks.
+// The top nodes (ranges across super-blocks) whose index lies in
+// super-block s are the nodes whose range holds its left or right boundary:
+// two root-ward chains, at most 2 x the tree depth (< 2 x 97 for 38-bit
+// keys tie-broken on positions below 2^30).  Each super-block lists them in
+// its own region of kTopPer words of tb.lists, so no workgroup waits on
+// another to reserve space; tb.slot holds each top node's list position.
+constexpr uint32_t kTopPer = 256;
+// Counter words (tb.counts): [0] phase-2 done count, [2 + s] number of top
+// nodes of super-block s.
 
 __device__ __forceinline__ bool spans(uint2 r, int shift) { return (r.x >> shift) != (r.y >> shift); }
 
@@ -406,6 +411,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 struct ListLds {
     uint32_t node[kLdsList], pw[kLdsList], par[kLdsList], pend[kLdsList];
     float4 in[kLdsList][4];
+    float4 out[kLdsList][2];  // the entry's own {cm, aabb} once complete
 };
 
 // Index of `x` in the sorted node[0, m), or kNone.
@@ -421,31 +427,39 @@ __device__ __forceinline__ uint32_t find_node(const ListLds &ls, uint32_t m, uin
     return lo < m && ls.node[lo] == x ? lo : kNone;
 }
 
-// Resolve a list set up in ls (m <= kLdsList entries).  The entries with
-// no pending child start a climb in LDS: merge the children
-// (lbvh.wgsl:205-228), publish the node, hand it to the parent entry's
-// input and count the parent's arrival (workgroup-scope acquire/release);
-// the child that arrives last climbs on, the other stops.  No barriers and
-// no waiting: a level costs a merge and one LDS atomic.
+// Resolve a list set up in ls (m <= kLdsList entries; called by the whole
+// workgroup).  The entries with no pending child start a climb in LDS: merge
+// the children (lbvh.wgsl:205-228), keep the node in its entry, hand it to
+// the parent entry's input and count the parent's arrival (workgroup-scope
+// acquire/release); the child that arrives last climbs on, the other stops.
+// A level is one round of LDS reads, the merge and one LDS atomic: the
+// chains of the upper levels are long (tens of levels), so the nodes are
+// published to memory only afterwards, by all threads together.
 __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, uint32_t m) {
     const uint32_t e = threadIdx.x;
-    if (e >= m || ls.pend[e] != 0u) return;
-    uint32_t cur = e;
-    for (int level = 0; level < 128; ++level) {
-        float4 C, B;
-        merge_values(ls.in[cur][0], ls.in[cur][1], ls.in[cur][2], ls.in[cur][3], C, B);
-        const uint32_t pw = ls.pw[cur];
-        const uint32_t q = ls.par[cur];
-        publish(tb, ls.node[cur], pw, C, B);
-        if (q == kNone) break;
-        const uint32_t side = pw >> 31;
-        ls.in[q][2 * side] = C;
-        ls.in[q][2 * side + 1] = B;
-        if (__hip_atomic_fetch_sub(&ls.pend[q], 1u, __ATOMIC_ACQ_REL,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP) != 1u)
-            break;  // the sibling is still pending: it climbs on
-        cur = q;
+    if (e < m && ls.pend[e] == 0u) {
+        uint32_t cur = e;
+        for (int level = 0; level < 128; ++level) {
+            const float4 L = ls.in[cur][0], LB = ls.in[cur][1];
+            const float4 R = ls.in[cur][2], RB = ls.in[cur][3];
+            const uint32_t pw = ls.pw[cur], q = ls.par[cur];
+            float4 C, B;
+            merge_values(L, LB, R, RB, C, B);
+            ls.out[cur][0] = C;
+            ls.out[cur][1] = B;
+            if (q == kNone) break;
+            const uint32_t side = pw >> 31;
+            ls.in[q][2 * side] = C;
+            ls.in[q][2 * side + 1] = B;
+            if (__hip_atomic_fetch_sub(&ls.pend[q], 1u, __ATOMIC_ACQ_REL,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) != 1u)
+                break;  // the sibling is still pending: it climbs on
+            cur = q;
+        }
     }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < m; k += kFillBlock)
+        publish(tb, ls.node[k], ls.pw[k], ls.out[k][0], ls.out[k][1]);
 }
 
 // Phases 2 and 3.  Workgroup sb scans the leaf ranges of node indices
@@ -463,13 +477,12 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     __shared__ ListLds ls;
     __shared__ uint32_t sh[kFillBlock / 64 + 1];
     __shared__ uint32_t s_cnt[kScan * 16];
-    __shared__ uint32_t s_base, s_last;
+    __shared__ uint32_t s_pre[kFillBlock];  // top: list offsets of the super-blocks
+    __shared__ uint32_t s_last;
     const uint32_t t = threadIdx.x;
     const uint32_t sb = blockIdx.x;
     const uint32_t nint = (uint32_t)(n - 1);
     uint32_t *top_list = tb.lists;
-    uint32_t *top_count = tb.counts + 1;
-    const uint32_t top_cap = list3_cap(num_super);
 
     // scan: node index base_i + k * kFillBlock + t (coalesced), every load
     // issued before any is used (indices clamped into the array)
@@ -501,7 +514,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     const uint32_t excl = block_scan(v, sh, &total);
     if (t < kScan * 16) s_cnt[t] = excl;
     const uint32_t m2 = total & 0xFFFFu, n3 = total >> 16;
-    if (t == 0) s_base = n3 ? atomicAdd(top_count, n3) : 0u;
+    if (t == 0) tb.counts[2 + sb] = n3;
     const bool lds2 = m2 <= lds_limit;
     __syncthreads();
 #pragma unroll
@@ -516,11 +529,11 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
                         __builtin_amdgcn_mbcnt_hi((uint32_t)(b2 >> 32), below)] = i;
         }
         if ((bits3 >> k) & 1u) {
-            const uint32_t pos =
-                s_base + (o >> 16) +
-                __builtin_amdgcn_mbcnt_hi((uint32_t)(b3 >> 32),
-                                          __builtin_amdgcn_mbcnt_lo((uint32_t)b3, 0u));
-            if (pos < top_cap) top_list[pos] = i;
+            const uint32_t local =
+                (o >> 16) + __builtin_amdgcn_mbcnt_hi((uint32_t)(b3 >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)b3, 0u));
+            const uint32_t pos = sb * kTopPer + local;  // local < kTopPer (two chains)
+            top_list[pos] = i;
             tb.slot[i] = pos;
         }
     }
@@ -572,16 +585,38 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    const uint32_t m3 =
-        __hip_atomic_load(top_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // The top list: the super-blocks' regions, compacted through their
+    // counts' prefix sums (entry e lies in the last region starting at or
+    // before e).
+    const uint32_t S = num_super;
+    uint32_t m3 = 0xFFFFFFFFu;  // more super-blocks than threads: climb instead
+    if (S <= (uint32_t)kFillBlock) {
+        const uint32_t c = t < S ? tb.counts[2 + t] : 0u;
+        const uint32_t pre = block_scan(c, sh, &m3);
+        if (t < S) s_pre[t] = pre;
+        __syncthreads();
+    }
     if (m3 <= lds_limit) {
         if (t < m3) {
-            const uint32_t p = top_list[t];
+            uint32_t lo = 0, hi = S;  // last region with s_pre <= t
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= t)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t p = top_list[lo * kTopPer + (t - s_pre[lo])];
             const uint32_t pw = tb.parent[p];
             const uint2 ch = tb.child[p];
             const uint32_t par = pw & kParentMask;
             // every ancestor of a top node is a top node
-            ls.par[t] = par < nint ? tb.slot[par] : kNone;
+            uint32_t q = kNone;
+            if (par < nint) {
+                const uint32_t ps = tb.slot[par];
+                q = s_pre[ps / kTopPer] + ps % kTopPer;
+            }
+            ls.par[t] = q;
             ls.node[t] = p;
             uint32_t pend = 0;
 #pragma unroll
@@ -601,8 +636,8 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
         resolve_list(tb, ls, m3);
     } else {
         // climb from every child of a top node that is not a top node
-        const uint32_t cnt = m3 < top_cap ? m3 : top_cap;
-        for (uint32_t k = t; k < cnt; k += kFillBlock) {
+        for (uint32_t k = t; k < S * kTopPer; k += kFillBlock) {
+            if (k % kTopPer >= tb.counts[2 + k / kTopPer]) continue;
             const uint2 ch = tb.child[top_list[k]];
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
@@ -612,7 +647,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
         }
     }
     __syncthreads();
-    if (t < 2) tb.counts[t] = 0u;
+    if (t == 0) tb.counts[0] = 0u;
 }
 
 }  // namespace
@@ -644,7 +679,7 @@ void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int6
     }
 }
 
-int64_t fill_count_words(int64_t) { return 2; }
-int64_t fill_list_words(int64_t cap) { return list3_cap(num_supers(cap)); }
+int64_t fill_count_words(int64_t cap) { return 2 + (int64_t)num_supers(cap); }
+int64_t fill_list_words(int64_t cap) { return (int64_t)num_supers(cap) * kTopPer; }
 
 }  // namespace nbs
