@@ -150,16 +150,18 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
 
 /* Launch tunables.  Block size, stack size, XCD order and bodies per lane
  * never change a result.  Variants 0 and 1 add every body's terms in the
- * reference's order and agree bit for bit; variant 2 takes the same
- * decisions and adds the same terms in its own fixed order (fast mode only;
- * in the exact mode it runs variant 1).
+ * reference's order and agree bit for bit; variants 2 and 4 take the same
+ * decisions and add the same terms in their own fixed order, bit for bit
+ * alike (fast mode only; in the exact mode they run variant 1).
  *   walk_block   walk workgroup size (64..1024, multiple of 64)
  *   walk_stack   walk stack entries per wavefront (8..256; the register-stack
  *                walks use one or two rows of 64: [64, 128]; the pair walk
  *                always has 63)
  *   walk_variant 0: stack in LDS; 1: stack in VGPR lanes; 2: pair walk, both
  *                children of an opened node tested per iteration, stack of
- *                (node, mask) in VGPR lanes (default)
+ *                (node, mask) in VGPR lanes (default); 4: the pair walk run
+ *                by persistent wavefronts taking 64-body chunks from a
+ *                device queue (ensembles: largest simulations first)
  *   xcd_order    order of the walk's workgroups over the 8 XCDs (each has its
  *                own L2): 0 hardware order (round-robin); 1 each XCD walks one
  *                contiguous run of Morton-ordered bodies (default); k > 1

@@ -74,6 +74,41 @@ def test_pair_walk_same_decisions_own_order():
     assert np.array_equal(ex1[0], ex2[0]) and np.array_equal(ex1[1], ex2[1])
 
 
+def test_queue_walk_equals_pair_walk():
+    """Variant 4, the persistent work-queue walk, runs each 64-body chunk with
+    the pair walk's own code, in queue order: the same bits as variant 2,
+    plain and ensemble (largest simulations first), with and without
+    counters, eager and from a graph."""
+    bodies = nbody_sim.generate("exp-disk", 30000, seed=12, dispersion=0.05, mass_sigma=0.5)
+    cfg, _ = params(theta=0.7, dt=5e-5)
+    cfg = cfg.with_(mathMode="fast")
+    for stats in (False, True):
+        a = run(cfg, bodies, (256, 128, 2, 1, 1), stats=stats)
+        b = run(cfg, bodies, (256, 128, 4, 1, 1), stats=stats)
+        for x, y in zip(a[:3], b[:3]):
+            assert np.array_equal(x, y), stats
+        assert a[3] == b[3]
+    sizes = [700, 1, 2048, 333, 64, 9000, 5]
+    parts = [nbody_sim.generate("plummer", n, seed=60 + i) for i, n in enumerate(sizes)]
+    ens = [np.concatenate([p[i] for p in parts]) for i in range(3)]
+    cfg2, _ = params(theta=0.6)
+    a = run(cfg2, ens, (256, 128, 2, 1, 1), ensemble_sizes=sizes, stats=False)
+    b = run(cfg2, ens, (256, 128, 4, 1, 1), ensemble_sizes=sizes, stats=False)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    out = []
+    for graph in (False, True):
+        sim = nbody_sim.Simulation(cfg)
+        sim.setBodies(*bodies)
+        sim.setTuning(walk_variant=4)
+        sim.useGraph(graph)
+        sim.step(4)
+        sim.synchronize()
+        out.append(sim.getBodies()[1:])
+        sim.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_pair_walk_reference_softening_and_ensemble():
     """Variant 2 in the reference softening mode (own-leaf compare kept) and
     over an ensemble: the same decisions as the reference-order walk."""

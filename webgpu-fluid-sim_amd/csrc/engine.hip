@@ -80,6 +80,8 @@ struct nbody_sim {
     bool bbox_fresh = false;
     int max_chunks = 1;
     int64_t *sim_off = nullptr;
+    uint32_t *chunk_order = nullptr;  // ensembles: the queue walk's chunks, largest sims first
+    uint32_t resident_waves = 0;      // wavefronts the GPU holds at once
     uint8_t *body_sim = nullptr;
     uint32_t *sim_root = nullptr;
     float2 *accel = nullptr;
@@ -208,6 +210,7 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->sim_off = a.take<int64_t>(s->num_sims + 1);
         s->body_sim = s->ensemble ? a.take<uint8_t>(cap) : nullptr;
         s->sim_root = a.take<uint32_t>(s->num_sims);
+        s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
         s->accel = a.take<float2>(cap);
         s->stats = a.take<unsigned long long>(4);
         s->overflow = a.take<uint32_t>(1);
@@ -244,6 +247,37 @@ bool self_term_zero(const nbody_sim *s) {
     const double m = (double)s->mass_absmax;
     if (!(eps2 >= 1e-30) || !(m <= 1e30)) return false;  // also false for NaN
     return m * std::pow(eps2, -1.5) <= 1e30;
+}
+
+// The queue walk's chunk order for an ensemble: the chunks of 64 sorted
+// positions of its simulations, largest simulation first (sim s occupies
+// sorted positions [sim_off[s], sim_off[s+1]): its keys carry s as prefix);
+// a chunk shared by two simulations goes with the larger one.
+int upload_chunk_order(nbody_sim *s) {
+    const int64_t total = s->sim_off_host[s->num_sims];
+    const int64_t chunks = (total + 63) / 64;
+    std::vector<int> sims(s->num_sims);
+    for (int i = 0; i < s->num_sims; ++i) sims[i] = i;
+    std::stable_sort(sims.begin(), sims.end(), [&](int x, int y) {
+        return s->sim_off_host[x + 1] - s->sim_off_host[x] >
+               s->sim_off_host[y + 1] - s->sim_off_host[y];
+    });
+    std::vector<uint8_t> seen((size_t)chunks, 0);
+    std::vector<uint32_t> order;
+    order.reserve((size_t)chunks);
+    for (int sim : sims) {
+        const int64_t lo = s->sim_off_host[sim], hi = s->sim_off_host[sim + 1];
+        if (hi <= lo) continue;
+        for (int64_t c = lo / 64; c <= (hi - 1) / 64; ++c)
+            if (!seen[(size_t)c]) {
+                seen[(size_t)c] = 1;
+                order.push_back((uint32_t)c);
+            }
+    }
+    HIP_TRY(s, hipMemcpyAsync(s->chunk_order, order.data(), 4 * order.size(),
+                              hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NBODY_OK;
 }
 
 // Plain handles with bounding-box keys and two or more bodies carry the box
@@ -337,6 +371,9 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     w.n = (uint32_t)n;
     w.p = p;
     w.self_zero = self_term_zero(s) ? 1u : 0u;
+    w.queue = s->sort.counters + kWalkQueueWord;  // zeroed by the build
+    w.chunk_order = s->chunk_order;
+    w.resident = s->resident_waves;
     w.bbox_slots = slots ? s->bbox_slots : nullptr;
     launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
                 s->xcd_order, s->walk_variant, s->lane_bodies, st);
@@ -426,6 +463,11 @@ nbody_sim *create_common(const nbody_config *cfg, int32_t device, int *err, bool
     s->stream = s->own_stream;
     for (auto &ev : s->ev) (void)hipEventCreate(&ev);
     rc = allocate(s, capacity);
+    int cus = 0;
+    if (rc == NBODY_OK &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
+        s->resident_waves = (uint32_t)cus * 32u;  // 8 wavefronts per SIMD, 4 SIMDs per CU
+    if (rc == NBODY_OK && ensemble) rc = upload_chunk_order(s);
     if (rc != NBODY_OK) {
         g_last_error = s->last_error;
         set_err(rc);
@@ -769,8 +811,8 @@ int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_block must be a multiple of 64 in [64, 1024]");
     if (t->walk_stack < 8 || t->walk_stack > 256)
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_stack must be in [8, 256]");
-    if (t->walk_variant < 0 || t->walk_variant > 2)
-        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0, 1 or 2");
+    if (t->walk_variant < 0 || t->walk_variant > 4 || t->walk_variant == 3)
+        return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0, 1, 2 or 4");
     if (t->lane_bodies != 1 && t->lane_bodies != 2 && t->lane_bodies != 4)
         return fail(s, NBODY_ERR_INVALID_ARG, "lane_bodies must be 1, 2 or 4");
     if (t->xcd_order < 0 || t->xcd_order > 65536)

@@ -96,7 +96,8 @@ __global__ __launch_bounds__(64) void bbox_final_kernel(const int64_t *__restric
             v = make_uint4(~float_order(mnx), ~float_order(mny), float_order(mxx),
                            float_order(mxy));
         }
-        reinterpret_cast<uint4 *>(slots)[threadIdx.x] = v;  // kBBoxSlots == 64 lanes
+        for (int k = threadIdx.x; k < kBBoxSlots; k += 64)
+            reinterpret_cast<uint4 *>(slots)[k] = k == 0 ? v : make_uint4(0u, 0u, 0u, 0u);
     }
 }
 
@@ -106,9 +107,13 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 }
 
 // The bounding box held by the walk's accumulators (all 64 lanes of the
-// wave take part; one slot per lane).
+// wave take part; kBBoxSlots / 64 slots per lane).
 __device__ __forceinline__ float4 bbox_from_slots(const uint32_t *__restrict__ slots) {
-    const uint4 v = reinterpret_cast<const uint4 *>(slots)[threadIdx.x & 63];
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    for (int k = threadIdx.x & 63; k < kBBoxSlots; k += 64) {
+        const uint4 u = reinterpret_cast<const uint4 *>(slots)[k];
+        v = make_uint4(max(v.x, u.x), max(v.y, u.y), max(v.z, u.z), max(v.w, u.w));
+    }
     const uint32_t mnx = wave_max_u32(v.x), mny = wave_max_u32(v.y);
     const uint32_t mxx = wave_max_u32(v.z), mxy = wave_max_u32(v.w);
     // an empty accumulator reads as the identity box (+inf, -inf)

@@ -43,8 +43,10 @@ namespace {
 // is an LDS read: only nodes with wide leaf ranges go to memory, for their
 // first (farthest) probes.
 constexpr int kBuildThreads = 256;
+constexpr int kBuildPerThread = 1;  // nodes per thread (2: all wavefronts resident at once, measured slower)
+constexpr int kBuildNodes = kBuildThreads * kBuildPerThread;  // nodes per workgroup
 constexpr int kBuildHalo = 1024;
-constexpr int kBuildWindow = kBuildThreads + 2 * kBuildHalo;
+constexpr int kBuildWindow = kBuildNodes + 2 * kBuildHalo;
 template <typename K>
 struct KeyWindow {
     const K *__restrict__ g;  // all keys
@@ -54,13 +56,13 @@ struct KeyWindow {
         return (j >= lo && j < hi) ? s[j - lo] : g[j];
     }
 };
-// Load the window of the workgroup's nodes [b0, b0 + 256) into s_keys.
+// Load the window of the workgroup's nodes [b0, b0 + kBuildNodes) into s_keys.
 template <typename K>
 __device__ __forceinline__ KeyWindow<K> load_window(const K *__restrict__ keys, int64_t n,
                                                     K *s_keys) {
-    const int64_t b0 = (int64_t)blockIdx.x * kBuildThreads;
+    const int64_t b0 = (int64_t)blockIdx.x * kBuildNodes;
     const int64_t lo = b0 > kBuildHalo ? b0 - kBuildHalo : 0;
-    const int64_t hi = b0 + kBuildThreads + kBuildHalo < n ? b0 + kBuildThreads + kBuildHalo : n;
+    const int64_t hi = b0 + kBuildNodes + kBuildHalo < n ? b0 + kBuildNodes + kBuildHalo : n;
     for (int64_t j = lo + threadIdx.x; j < hi; j += kBuildThreads) s_keys[j - lo] = keys[j];
     __syncthreads();
     return KeyWindow<K>{keys, s_keys, lo, hi};
@@ -171,8 +173,9 @@ __device__ __forceinline__ void clear_words(uint32_t *__restrict__ p, int64_t wo
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0u;
-    if (slots && blockIdx.x == 0 && threadIdx.x < kBBoxSlots)
-        reinterpret_cast<uint4 *>(slots)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+    if (slots && blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < (uint32_t)kBBoxSlots; k += blockDim.x)
+            reinterpret_cast<uint4 *>(slots)[k] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
@@ -181,19 +184,21 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
     __shared__ uint32_t s_keys[kBuildWindow];
     clear_words(clear, clear_n, slots);
     const DeltaU32 delta{load_window(keys, n, s_keys), n};
-    const int64_t i = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
-    if (i >= n - 1) return;
-    const Split sp = karras_node(delta, i, n);
-    tb.child[i] = make_uint2(sp.lc, sp.rc);
-    // the walk reads the indices from the pair record
-    *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
-    tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
-    // lbvh.wgsl:155-161, with the fill's in-block flag
-    const uint32_t pw = (uint32_t)i | in_block(sp);
-    tb.parent[sp.lc] = pw;
-    tb.parent[sp.rc] = pw | kRightChild;
-    if (i == 0) tb.parent[0] = kNoNode;
-    tb.status[i] = 0u;  // lbvh.wgsl:164
+    for (int r = 0; r < kBuildPerThread; ++r) {
+        const int64_t i = (int64_t)blockIdx.x * kBuildNodes + r * kBuildThreads + threadIdx.x;
+        if (i >= n - 1) return;
+        const Split sp = karras_node(delta, i, n);
+        tb.child[i] = make_uint2(sp.lc, sp.rc);
+        // the walk reads the indices from the pair record
+        *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
+        tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
+        // lbvh.wgsl:155-161, with the fill's in-block flag
+        const uint32_t pw = (uint32_t)i | in_block(sp);
+        tb.parent[sp.lc] = pw;
+        tb.parent[sp.rc] = pw | kRightChild;
+        if (i == 0) tb.parent[0] = kNoNode;
+        tb.status[i] = 0u;  // lbvh.wgsl:164
+    }
 }
 
 __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t *__restrict__ keys,
@@ -207,28 +212,30 @@ __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t
     __shared__ uint64_t s_keys[kBuildWindow];
     clear_words(clear, clear_n, slots);
     const DeltaU64 delta{load_window(keys, n, s_keys), sim_off, n};
-    const int64_t i = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
-    if (i >= n - 1) return;
-    const Split sp = karras_node(delta, i, n);
-    tb.child[i] = make_uint2(sp.lc, sp.rc);
-    // the walk reads the indices from the pair record
-    *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
-    tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
-    const uint32_t pw = (uint32_t)i | in_block(sp);
-    tb.parent[sp.lc] = pw;
-    tb.parent[sp.rc] = pw | kRightChild;
-    tb.status[i] = 0u;
-    // A node spanning two sims sits above the sim level; a child whose
-    // range lies inside one sim is that sim's root.
-    const uint64_t s_lo = keys[sp.lo] >> kSimShift, s_hi = keys[sp.hi] >> kSimShift;
-    if (s_lo != s_hi) {
-        if ((keys[sp.gamma] >> kSimShift) == s_lo) sim_root[s_lo] = sp.lc;
-        if ((keys[sp.gamma + 1] >> kSimShift) == s_hi) sim_root[s_hi] = sp.rc;
-    }
-    if (i == 0) {
-        tb.parent[0] = kNoNode;
-        const uint64_t s0 = keys[0] >> kSimShift;
-        if (s0 == (keys[n - 1] >> kSimShift)) sim_root[s0] = 0u;
+    for (int r = 0; r < kBuildPerThread; ++r) {
+        const int64_t i = (int64_t)blockIdx.x * kBuildNodes + r * kBuildThreads + threadIdx.x;
+        if (i >= n - 1) return;
+        const Split sp = karras_node(delta, i, n);
+        tb.child[i] = make_uint2(sp.lc, sp.rc);
+        // the walk reads the indices from the pair record
+        *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
+        tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
+        const uint32_t pw = (uint32_t)i | in_block(sp);
+        tb.parent[sp.lc] = pw;
+        tb.parent[sp.rc] = pw | kRightChild;
+        tb.status[i] = 0u;
+        // A node spanning two sims sits above the sim level; a child whose
+        // range lies inside one sim is that sim's root.
+        const uint64_t s_lo = keys[sp.lo] >> kSimShift, s_hi = keys[sp.hi] >> kSimShift;
+        if (s_lo != s_hi) {
+            if ((keys[sp.gamma] >> kSimShift) == s_lo) sim_root[s_lo] = sp.lc;
+            if ((keys[sp.gamma + 1] >> kSimShift) == s_hi) sim_root[s_hi] = sp.rc;
+        }
+        if (i == 0) {
+            tb.parent[0] = kNoNode;
+            const uint64_t s0 = keys[0] >> kSimShift;
+            if (s0 == (keys[n - 1] >> kSimShift)) sim_root[s0] = 0u;
+        }
     }
 }
 
@@ -655,7 +662,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
 void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, uint32_t *clear,
                       int64_t clear_n, uint32_t *slots, hipStream_t s) {
     if (n < 2) return;
-    build_u32_kernel<<<(unsigned)((n - 1 + kBuildThreads - 1) / kBuildThreads), kBuildThreads, 0,
+    build_u32_kernel<<<(unsigned)((n - 1 + kBuildNodes - 1) / kBuildNodes), kBuildThreads, 0,
                        s>>>(keys, n, t, clear, clear_n, slots);
 }
 
@@ -663,7 +670,7 @@ void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, c
                       uint32_t *sim_root, uint32_t *clear, int64_t clear_n, uint32_t *slots,
                       hipStream_t s) {
     if (n < 2) return;
-    build_u64_kernel<<<(unsigned)((n - 1 + kBuildThreads - 1) / kBuildThreads), kBuildThreads, 0,
+    build_u64_kernel<<<(unsigned)((n - 1 + kBuildNodes - 1) / kBuildNodes), kBuildThreads, 0,
                        s>>>(keys, n, sim_off, t, sim_root, clear, clear_n, slots);
 }
 

@@ -528,12 +528,13 @@ struct PairStack {
 // own leaf is accepted: its offset is exactly zero, so with a finite pull
 // factor m * inv^3 (the host checks that eps^2 and the largest mass keep it
 // finite) its term is an exact +0 and the sum is unchanged.
+// The pair walk of one chunk: the 64 bodies at sorted positions [64c, 64c + 64),
+// one per lane of the calling wavefront.
 template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32>
-__global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint32_t xcd_run) {
+__device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chunk) {
     const int lane = threadIdx.x & 63;
-    const uint32_t block = logical_block(blockIdx.x, xcd_run);
     const uint32_t n = a.n;
-    const uint32_t t = block * blockDim.x + threadIdx.x;  // sorted position
+    const uint32_t t = chunk * 64u + (uint32_t)lane;  // sorted position
     const bool valid = t < n;
     const uint32_t leaf0 = n - 1u;
     const uint32_t my_leaf = leaf0 + t;
@@ -705,11 +706,51 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
     }
 }
 
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32>
+__global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint32_t xcd_run) {
+    const uint32_t block = logical_block(blockIdx.x, xcd_run);
+    pair_walk_chunk<kRefSoft, kEnsemble, kStats, kSelf, kOff32>(
+        a, (block * blockDim.x + threadIdx.x) >> 6);
+}
+
+// Persistent work-queue walk: `resident` wavefronts (as many as the GPU
+// holds at once) take chunks from a queue, each its own index first, then
+// resident + the next value of the queue counter (zeroed by the build with
+// the sort's scratch), until the chunks run out; the queue lists the chunks
+// in a.chunk_order (null: sorted order).  Ensembles list their largest
+// simulations' chunks first, so the longest work items start early and the
+// short ones fill the end of the launch.
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32>
+__global__ __launch_bounds__(256) void walk_pair_queue_kernel(const WalkArgs a, uint32_t resident) {
+    const uint32_t chunks = (a.n + 63u) / 64u;
+    uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    while (item < chunks) {
+        const uint32_t c = a.chunk_order ? a.chunk_order[item] : item;
+        pair_walk_chunk<kRefSoft, kEnsemble, kStats, kSelf, kOff32>(a, uniform_u32(c));
+        uint32_t next = 0;
+        if ((threadIdx.x & 63) == 0) next = atomicAdd(a.queue, 1u);
+        item = resident + uniform_u32(next);  // lane 0's value
+    }
+}
+
 // Register-stack capacity for a requested stack size: one or two rows of 64.
 int rs_stack_cap(int stack) { return stack < 64 ? 64 : (stack > 128 ? 128 : stack); }
 
 template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf>
-void launch_pair(const WalkArgs &a, int block, int xcd, hipStream_t s) {
+void launch_pair(const WalkArgs &a, int block, int xcd, bool queue, hipStream_t s) {
+    if (queue && a.queue && a.resident) {
+        const uint32_t chunks = (a.n + 63u) / 64u;
+        const uint32_t waves = chunks < a.resident ? chunks : a.resident;
+        const uint32_t grid = (waves + 3u) / 4u;  // 4 waves per workgroup
+        const uint32_t resident = grid * 4u;
+        if (a.n <= (1u << 26))
+            walk_pair_queue_kernel<kRefSoft, kEnsemble, kStats, kSelf, true>
+                <<<grid, 256, 0, s>>>(a, resident);
+        else
+            walk_pair_queue_kernel<kRefSoft, kEnsemble, kStats, kSelf, false>
+                <<<grid, 256, 0, s>>>(a, resident);
+        return;
+    }
     uint32_t grid, run;
     xcd_grid((a.n + (unsigned)block - 1u) / (unsigned)block, xcd, grid, run);
     if (a.n <= (1u << 26))
@@ -731,23 +772,24 @@ template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
 void launch_t(const WalkArgs &a, int block, int stack, int xcd, int variant, int lane_bodies,
               hipStream_t s) {
     if constexpr (!kExact) {
-        if (variant == 2) {
+        if (variant == 2 || variant == 4) {
             // the own-leaf compare is needed for the counters and whenever
             // the own leaf's term might not be an exact zero
+            const bool queue = variant == 4;
             if constexpr (kStats || kRefSoft) {
-                launch_pair<kRefSoft, kEnsemble, kStats, true>(a, block, xcd, s);
+                launch_pair<kRefSoft, kEnsemble, kStats, true>(a, block, xcd, queue, s);
             } else {
                 if (a.self_zero)
-                    launch_pair<kRefSoft, kEnsemble, kStats, false>(a, block, xcd, s);
+                    launch_pair<kRefSoft, kEnsemble, kStats, false>(a, block, xcd, queue, s);
                 else
-                    launch_pair<kRefSoft, kEnsemble, kStats, true>(a, block, xcd, s);
+                    launch_pair<kRefSoft, kEnsemble, kStats, true>(a, block, xcd, queue, s);
             }
             return;
         }
     }
-    // the pair walk sums in its own order: the exact mode, which keeps the
+    // the pair walks sum in their own order: the exact mode, which keeps the
     // reference's order, runs the register-stack walk instead
-    if (variant == 2) variant = 1;
+    if (variant >= 2) variant = 1;
     if (variant == 1) {
         if (lane_bodies == 4)
             launch_rs<kExact, kRefSoft, kEnsemble, kStats, 4>(a, block, stack, xcd, s);

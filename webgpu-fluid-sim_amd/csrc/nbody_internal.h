@@ -12,7 +12,7 @@ constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 constexpr int kSimShift = 30;     // ensemble keys: sim id << 30 | 30-bit Morton code
 constexpr int kMaxSims = 256;     // 8-bit sim prefix
 constexpr int kBBoxChunk = 4096;  // bodies per bounding-box partial
-constexpr int kBBoxSlots = 64;     // bounding-box accumulators filled by the walk
+constexpr int kBBoxSlots = 256;    // bounding-box accumulators filled by the walk
 
 // Order-preserving u32 image of a binary32 number (larger number, larger
 // image; -0 below +0) for atomic min/max.  Images of numbers are never 0,
@@ -43,6 +43,10 @@ struct StepParams {
 constexpr int kSortTile = 4096;       // keys per one-sweep tile
 constexpr int kSortMaxPasses = 5;     // 38-bit ensemble keys at 8 bits per digit
 constexpr int kSortHistWords = kSortMaxPasses * 2048;  // [pass][2^radix_bits] digit counts
+// The persistent walk's queue counter: a word of the sort's counter block
+// (zeroed with the sort's scratch by the build; the sort uses words
+// [0, kSortMaxPasses)).
+constexpr int kWalkQueueWord = 16;
 struct SortScratch {
     uint32_t *hist;      // [passes][2^radix_bits] digit counts (written by the encode)
     uint32_t *lookback;  // [passes][tiles][2^radix_bits] decoupled look-back words
@@ -155,6 +159,11 @@ struct WalkArgs {
     // bounding-box accumulators of the drifted positions (the next step's
     // keys), or null
     uint32_t *bbox_slots;
+    // persistent work-queue walk: the queue counter (zero at launch), the
+    // chunks' order (null: sorted order) and the wavefronts the GPU holds
+    uint32_t *queue;
+    const uint32_t *chunk_order;
+    uint32_t resident;
     StepParams p;
 };
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,

@@ -1,10 +1,11 @@
 """Launch autotuner for the tree walk and the radix sort.
 
 Times each candidate (walk variant: 0 LDS stack, 1 register stack, 2 pair
-walk; bodies per lane of the register-stack walk; workgroup size; stack
-entries; XCD order; then the radix sort's digit width) on the handle's own
-bodies with the engine's HIP-event phase timers, restores the bodies after
-every trial, and applies the fastest.  Every candidate takes the same
+walk, 4 pair walk run from a persistent work queue; bodies per lane of the
+register-stack walk; workgroup size; stack entries; XCD order; then the
+radix sort's digit width) on the handle's own bodies with the engine's
+HIP-event phase timers, restores the bodies after every trial, and applies
+the fastest.  Every candidate takes the same
 opening decisions and the sort is stable for every width; walk variants 0
 and 1 add each body's terms in the reference's order and agree bit for bit,
 the pair walk (fast mode) adds the same terms in its own order, so tuning
@@ -14,7 +15,7 @@ from __future__ import annotations
 
 import itertools
 
-DEFAULT_VARIANTS = (2, 1, 0)
+DEFAULT_VARIANTS = (2, 4, 1, 0)
 DEFAULT_LANE_BODIES = (1, 2, 4)
 DEFAULT_XCD = (1, 0, 32)
 DEFAULT_BLOCKS = (64, 128, 256, 512)
@@ -25,7 +26,7 @@ def _candidates(variants, blocks, stacks, xcd, lane_bodies):
     for v, b, st, x in itertools.product(variants, blocks, stacks, xcd):
         # the LDS-stack and pair walks have one body per lane, and the pair
         # walk's stack is fixed
-        if v == 2 and st != stacks[0]:
+        if v in (2, 4) and st != stacks[0]:
             continue
         for lb in (lane_bodies if v == 1 else (1,)):
             yield (v, b, st, x, lb)
