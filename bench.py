@@ -40,6 +40,10 @@ PMC_PROFILE = os.path.join(ROOT, "profiles", "step_800k_pmc.md")
 PEAK_HBM_GBS = 8000.0
 FLOPS_ACCEPT = 18          # per accepted interaction (constant softening, rsqrt = 1)
 FLOPS_OPEN = 9             # per opened node
+# The walk kernel this workload dispatches: pair walk, plain (one simulation),
+# no counters, reference softening off, own-leaf term exactly zero, 32-bit
+# record offsets.
+WALK_KERNEL = "walk_pair_kernel<false, false, false, false, true>"
 
 
 def walk_traffic():
@@ -49,7 +53,7 @@ def walk_traffic():
     try:
         with open(PMC_PROFILE) as f:
             for line in f:
-                if line.startswith("walk_rs_kernel<false, false, false, false, 1> HBM bytes per launch:"):
+                if line.startswith(WALK_KERNEL + " HBM bytes per launch:"):
                     return float(line.rsplit(":", 1)[1]), os.path.relpath(PMC_PROFILE, ROOT)
     except OSError:
         pass
@@ -271,7 +275,7 @@ def main():
                    "math_mode": "fast", "graph": True},
         "phases_ms": {k: round(v, 5) for k, v in phase_avg.items()},
         "roofline": {
-            "kernel": "walk_rs_kernel (tree walk + kick + drift)",
+            "kernel": WALK_KERNEL + " (tree walk + kick + drift)",
             "bound": "valu-fp32",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
