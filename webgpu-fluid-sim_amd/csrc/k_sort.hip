@@ -28,11 +28,13 @@ namespace nbs {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / kWave;
-constexpr int kItems = 16;
-constexpr int kTile = kThreads * kItems;
-static_assert(kTile == kSortTile, "tile size");
+// Workgroup size per digit width: 8-bit digits rank 4 keys per thread in
+// 1024-thread workgroups (16 wavefronts hide the load and ranking
+// latencies); 10- and 11-bit digits keep 256 threads x 16 keys, as their
+// per-wavefront digit counts (kWaves x 2^RB words) must fit in LDS.
+constexpr int kTile = kSortTile;
+template <int RB>
+constexpr int sort_threads() { return RB == 8 ? 1024 : 256; }
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
@@ -43,55 +45,63 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh4) {
+// Exclusive scan of one value per thread over a kThreads-thread block.
+template <int kThreads>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *shw) {
+    constexpr int kWaves = kThreads / kWave;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint32_t x = v;
     for (int o = 1; o < 64; o <<= 1) {
         uint32_t y = __shfl_up(x, o);
         if (lane >= o) x += y;
     }
-    if (lane == 63) sh4[wid] = x;
+    if (lane == 63) shw[wid] = x;
     __syncthreads();
     uint32_t before = 0;
     for (int w = 0; w < kWaves; ++w)
-        if (w < wid) before += sh4[w];
+        if (w < wid) before += shw[w];
     __syncthreads();
     return before + x - v;
 }
 
-// One pass over digit (k >> shift) & (2^RB - 1).  Thread t owns digits
-// [t * DPT, t * DPT + DPT) for the scans and the look-back.
-template <typename K, int RB>
+// One pass over digit (k >> shift) & (2^RB - 1).  Owner thread t (t <
+// kOwners) owns digits [t * DPT, t * DPT + DPT) for the scans and the
+// look-back.
+template <typename K, int RB, int kThreads>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_count,
     uint32_t *lookback, uint32_t *counter) {
     constexpr int D = 1 << RB;
-    constexpr int DPT = D / kThreads;
+    constexpr int kWaves = kThreads / kWave;
+    constexpr int kItems = kTile / kThreads;
+    constexpr int kOwners = D < kThreads ? D : kThreads;
+    constexpr int DPT = D / kOwners;
     constexpr uint32_t kMask = (uint32_t)D - 1u;
+    static_assert(kItems * kThreads == kTile, "tile size");
     static_assert(DPT >= 1 && DPT <= 8, "digits per thread");
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wcnt[kWaves * D];  // wave digit counts, then wave offsets
     __shared__ uint32_t s_lstart[D];         // digit start inside the tile
     __shared__ uint32_t s_gstart[D];         // digit start in the output
-    __shared__ uint32_t s_sh4[kWaves];
+    __shared__ uint32_t s_shw[kWaves];
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     for (int i = tid; i < kWaves * D; i += kThreads) s_wcnt[i] = 0;
+    const bool owner = tid < kOwners;
     // where each owned digit starts in the output (exclusive digit scan)
     uint32_t dbase[DPT];
     {
         uint32_t c[DPT], sum = 0;
 #pragma unroll
         for (int j = 0; j < DPT; ++j) {
-            c[j] = digit_count[tid * DPT + j];
+            c[j] = owner ? digit_count[tid * DPT + j] : 0u;
             sum += c[j];
         }
-        uint32_t run = block_exclusive_scan(sum, s_sh4);  // its barriers publish s_tile
+        uint32_t run = block_exclusive_scan<kThreads>(sum, s_shw);  // its barriers publish s_tile
 #pragma unroll
         for (int j = 0; j < DPT; ++j) {
             dbase[j] = run;
@@ -144,23 +154,27 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     uint32_t total[DPT], tsum = 0;
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {
-        const int d = tid * DPT + j;
         uint32_t t = 0;
+        if (owner) {
+            const int d = tid * DPT + j;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = s_wcnt[w * D + d];
-            s_wcnt[w * D + d] = t;
-            t += c;
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = s_wcnt[w * D + d];
+                s_wcnt[w * D + d] = t;
+                t += c;
+            }
         }
         total[j] = t;
         tsum += t;
     }
     {
-        uint32_t run = block_exclusive_scan(tsum, s_sh4);
+        uint32_t run = block_exclusive_scan<kThreads>(tsum, s_shw);
+        if (owner) {
 #pragma unroll
-        for (int j = 0; j < DPT; ++j) {
-            s_lstart[tid * DPT + j] = run;
-            run += total[j];
+            for (int j = 0; j < DPT; ++j) {
+                s_lstart[tid * DPT + j] = run;
+                run += total[j];
+            }
         }
     }
 
@@ -169,10 +183,11 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {
         prefix[j] = 0;
-        __hip_atomic_store(&mine[j], (tile == 0 ? kFlagPrefix : kFlagAgg) | total[j],
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (owner)
+            __hip_atomic_store(&mine[j], (tile == 0 ? kFlagPrefix : kFlagAgg) | total[j],
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (tile != 0) {
+    if (tile != 0 && owner) {
         // All owned digits look back together, each from its own
         // predecessor, reading a window of kLook predecessors per round trip
         // (their aggregates are summed in order up to the first inclusive
@@ -219,8 +234,10 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
             __hip_atomic_store(&mine[j], kFlagPrefix | (prefix[j] + total[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
+    if (owner) {
 #pragma unroll
-    for (int j = 0; j < DPT; ++j) s_gstart[tid * DPT + j] = dbase[j] + prefix[j];
+        for (int j = 0; j < DPT; ++j) s_gstart[tid * DPT + j] = dbase[j] + prefix[j];
+    }
     __syncthreads();
 
     // Scatter into the LDS tile in digit order, then stream out.
@@ -256,8 +273,9 @@ int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n,
     const int64_t tiles = sort_tiles(n);
     K *ki = keys_a, *ko = keys_b;
     uint32_t *vi = vals_a, *vo = vals_b;
+    constexpr int T = sort_threads<RB>();
     for (int p = 0; p < passes; ++p) {
-        onesweep_kernel<K, RB><<<(unsigned)tiles, kThreads, 0, s>>>(
+        onesweep_kernel<K, RB, T><<<(unsigned)tiles, T, 0, s>>>(
             ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
             sc.counters + p);
         K *tk = ki;
