@@ -170,7 +170,7 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *                1 (default), 2 or 4; a wavefront walks 64 * lane_bodies
  *                Morton-consecutive bodies together (the LDS and pair walks
  *                ignore it, walking one body per lane)
- *   radix_bits   digit width of the radix sort: 8 (default), 10 or 11 bits
+ *   radix_bits   digit width of the radix sort: 8, 10 (default) or 11 bits
  *                per pass (30-bit keys: 4, 3 or 3 passes) */
 typedef struct nbody_tuning {
     int32_t walk_block;

@@ -58,7 +58,7 @@ struct nbody_sim {
     int64_t cap = 0, n = 0;
     float mass_absmax = 0.0f;  // largest |mass| uploaded (NaN if any is not a number)
     int key_bits = 32;
-    int radix_bits = 8;  // sort digit width (nbody_tuning.radix_bits)
+    int radix_bits = 10;  // sort digit width (nbody_tuning.radix_bits)
 
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -326,13 +326,15 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
                   s->ensemble ? (uint64_t *)s->keys[0] : nullptr, s->vals[0], s->sort.hist,
                   passes, s->radix_bits, st);
     mark(2);
-    // phase 2: stable radix sort (simulation.ts:400)
+    // phase 2: stable radix sort (simulation.ts:400); its last pass also
+    // writes the sorted leaf records
+    const LeafGather lg{s->pos, s->mass, s->tree.leaf};
     if (s->ensemble)
         s->result_buf = launch_sort_u64((uint64_t *)s->keys[0], s->vals[0], (uint64_t *)s->keys[1],
-                                        s->vals[1], n, s->key_bits, s->radix_bits, s->sort, st);
+                                        s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
     else
         s->result_buf = launch_sort_u32((uint32_t *)s->keys[0], s->vals[0], (uint32_t *)s->keys[1],
-                                        s->vals[1], n, s->key_bits, s->radix_bits, s->sort, st);
+                                        s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
     const uint32_t *idx = s->vals[s->result_buf];
     mark(3);
     // phase 3: Karras build (lbvh.wgsl:85-165); it also zeroes the sort's

@@ -17,7 +17,9 @@
 //      matching (RB ballots per key), publishes its per-digit counts,
 //      resolves its global offsets by decoupled look-back, and scatters
 //      through an LDS staging tile so that stores of one digit run are
-//      contiguous.
+//      contiguous;
+//   2. the last pass also writes the tree's sorted leaf records, gathering
+//      each body's position and mass as soon as its index is loaded.
 // The histograms, tile counters and look-back words must be zero when a
 // sort starts; the tree build that follows every sort clears them for the
 // next step (sort_clear_words), so no clearing kernel runs per step.
@@ -28,13 +30,13 @@ namespace nbs {
 
 namespace {
 
-// Workgroup size per digit width: 8-bit digits rank 4 keys per thread in
-// 1024-thread workgroups (16 wavefronts hide the load and ranking
-// latencies); 10- and 11-bit digits keep 256 threads x 16 keys, as their
+// Workgroup size per digit width: 8- and 10-bit digits rank 4 keys per
+// thread in 1024-thread workgroups (16 wavefronts hide the load and ranking
+// latencies); 11-bit digits keep 256 threads x 16 keys, as the
 // per-wavefront digit counts (kWaves x 2^RB words) must fit in LDS.
 constexpr int kTile = kSortTile;
 template <int RB>
-constexpr int sort_threads() { return RB == 8 ? 1024 : 256; }
+constexpr int sort_threads() { return RB <= 10 ? 1024 : 256; }
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
@@ -67,11 +69,16 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
 // One pass over digit (k >> shift) & (2^RB - 1).  Owner thread t (t <
 // kOwners) owns digits [t * DPT, t * DPT + DPT) for the scans and the
 // look-back.
-template <typename K, int RB, int kThreads>
+//
+// kLeaf (the last pass): each item's body record {pos, mass, 0} is gathered
+// right after its load and written to the leaf records at the item's output
+// position once the look-back has placed it, so the gathers' latency hides
+// behind the ranking and the look-back.
+template <typename K, int RB, int kThreads, bool kLeaf>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_count,
-    uint32_t *lookback, uint32_t *counter) {
+    uint32_t *lookback, uint32_t *counter, LeafGather lg) {
     constexpr int D = 1 << RB;
     constexpr int kWaves = kThreads / kWave;
     constexpr int kItems = kTile / kThreads;
@@ -124,6 +131,16 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         k[it] = ok ? kin[p] : (K)0;
         v[it] = ok ? vin[p] : 0u;
         dg[it] = ok ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
+    }
+    float4 rec[kLeaf ? kItems : 1];
+    if constexpr (kLeaf) {
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            if (dg[it] < (uint32_t)D) {
+                const float2 q = lg.pos[v[it]];
+                rec[it] = make_float4(q.x, q.y, lg.mass[v[it]], 0.0f);
+            }
+        }
     }
 
     // Stable rank inside the wave: peers = lanes holding the same digit.
@@ -248,6 +265,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
             const uint32_t lp = s_lstart[dd] + s_wcnt[wid * D + dd] + rank[it];
             s_keys[lp] = k[it];
             s_vals[lp] = v[it];
+            if constexpr (kLeaf) lg.leaf[s_gstart[dd] + (lp - s_lstart[dd])] = rec[it];
         }
     }
     __syncthreads();
@@ -268,16 +286,21 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
 
 template <typename K, int RB>
 int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int passes,
-            const SortScratch &sc, hipStream_t s) {
+            const SortScratch &sc, const LeafGather &lg, hipStream_t s) {
     constexpr int D = 1 << RB;
     const int64_t tiles = sort_tiles(n);
     K *ki = keys_a, *ko = keys_b;
     uint32_t *vi = vals_a, *vo = vals_b;
     constexpr int T = sort_threads<RB>();
     for (int p = 0; p < passes; ++p) {
-        onesweep_kernel<K, RB, T><<<(unsigned)tiles, T, 0, s>>>(
-            ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
-            sc.counters + p);
+        if (p == passes - 1 && lg.leaf)
+            onesweep_kernel<K, RB, T, true><<<(unsigned)tiles, T, 0, s>>>(
+                ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
+                sc.counters + p, lg);
+        else
+            onesweep_kernel<K, RB, T, false><<<(unsigned)tiles, T, 0, s>>>(
+                ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
+                sc.counters + p, lg);
         K *tk = ki;
         ki = ko;
         ko = tk;
@@ -290,12 +313,14 @@ int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n,
 
 template <typename K>
 int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int key_bits,
-              int radix_bits, const SortScratch &sc, hipStream_t s) {
+              int radix_bits, const SortScratch &sc, const LeafGather &lg, hipStream_t s) {
     if (n <= 1) return 0;
     const int passes = sort_passes(key_bits, radix_bits);
-    if (radix_bits == 11) return sort_rb<K, 11>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, s);
-    if (radix_bits == 10) return sort_rb<K, 10>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, s);
-    return sort_rb<K, 8>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, s);
+    if (radix_bits == 11)
+        return sort_rb<K, 11>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
+    if (radix_bits == 10)
+        return sort_rb<K, 10>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
+    return sort_rb<K, 8>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
 }
 
 }  // namespace
@@ -325,14 +350,14 @@ void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t cap) {
 
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
-                    hipStream_t s) {
-    return sort_impl<uint32_t>(keys_a, vals_a, keys_b, vals_b, n, key_bits, radix_bits, sc, s);
+                    const LeafGather &lg, hipStream_t s) {
+    return sort_impl<uint32_t>(keys_a, vals_a, keys_b, vals_b, n, key_bits, radix_bits, sc, lg, s);
 }
 
 int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
-                    hipStream_t s) {
-    return sort_impl<uint64_t>(keys_a, vals_a, keys_b, vals_b, n, key_bits, radix_bits, sc, s);
+                    const LeafGather &lg, hipStream_t s) {
+    return sort_impl<uint64_t>(keys_a, vals_a, keys_b, vals_b, n, key_bits, radix_bits, sc, lg, s);
 }
 
 }  // namespace nbs

@@ -47,37 +47,66 @@ constexpr int kBuildPerThread = 1;  // nodes per thread (2: all wavefronts resid
 constexpr int kBuildNodes = kBuildThreads * kBuildPerThread;  // nodes per workgroup
 constexpr int kBuildHalo = 1024;
 constexpr int kBuildWindow = kBuildNodes + 2 * kBuildHalo;
+// A global-memory read the compiler cannot merge with another load; it
+// waits for its data in the same asm block.
+__device__ __forceinline__ uint32_t global_read(const uint32_t *p) {
+    uint32_t v;
+    asm volatile("global_load_dword %0, %1, off\n\ts_waitcnt vmcnt(0)" : "=v"(v) : "v"(p) : "memory");
+    return v;
+}
+__device__ __forceinline__ uint64_t global_read(const uint64_t *p) {
+    uint64_t v;
+    asm volatile("global_load_dwordx2 %0, %1, off\n\ts_waitcnt vmcnt(0)" : "=v"(v) : "v"(p) : "memory");
+    return v;
+}
+
+// Indices are 32-bit: positions stay below 2^30 (the parent words' limit).
 template <typename K>
 struct KeyWindow {
     const K *__restrict__ g;  // all keys
-    const K *s;               // LDS copy of keys [lo, hi)
-    int64_t lo, hi;
-    __device__ __forceinline__ K operator[](int64_t j) const {
-        return (j >= lo && j < hi) ? s[j - lo] : g[j];
+    const K *s;               // LDS copy of keys [lo, lo + len)
+    int lo, len;
+    // an LDS read for every lane (position 0 outside the window), then a
+    // memory read for the lanes outside it, written so that the compiler
+    // cannot merge the two into one load from a selected address (every
+    // probe would become a flat load)
+    __device__ __forceinline__ K operator[](int j) const {
+        const bool in = (uint32_t)(j - lo) < (uint32_t)len;
+        K v = s[in ? j - lo : 0];
+        if (!in) v = global_read(g + j);
+        return v;
     }
 };
 // Load the window of the workgroup's nodes [b0, b0 + kBuildNodes) into s_keys.
 template <typename K>
-__device__ __forceinline__ KeyWindow<K> load_window(const K *__restrict__ keys, int64_t n,
+__device__ __forceinline__ KeyWindow<K> load_window(const K *__restrict__ keys, int n,
                                                     K *s_keys) {
-    const int64_t b0 = (int64_t)blockIdx.x * kBuildNodes;
-    const int64_t lo = b0 > kBuildHalo ? b0 - kBuildHalo : 0;
-    const int64_t hi = b0 + kBuildNodes + kBuildHalo < n ? b0 + kBuildNodes + kBuildHalo : n;
-    for (int64_t j = lo + threadIdx.x; j < hi; j += kBuildThreads) s_keys[j - lo] = keys[j];
+    const int b0 = (int)blockIdx.x * kBuildNodes;
+    const int lo = b0 > kBuildHalo ? b0 - kBuildHalo : 0;
+    const int hi = b0 + kBuildNodes + kBuildHalo < n ? b0 + kBuildNodes + kBuildHalo : n;
+    for (int j = lo + (int)threadIdx.x; j < hi; j += kBuildThreads) s_keys[j - lo] = keys[j];
     __syncthreads();
-    return KeyWindow<K>{keys, s_keys, lo, hi};
+    return KeyWindow<K>{keys, s_keys, lo, hi - lo};
 }
 
+// lbvh.wgsl:72-83: the common prefix length of sorted keys i and j, -1 for
+// j outside [0, n); equal keys compare their positions instead.  A node's
+// searches compare every probe with its own key, read once (at(i)).
 struct DeltaU32 {
     KeyWindow<uint32_t> k;
-    int64_t n;
-    // lbvh.wgsl:72-83
-    __device__ __forceinline__ int operator()(int64_t i, int64_t j) const {
-        if (j < 0 || j >= n) return -1;
-        const uint32_t a = k[i], b = k[j];
-        if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
-        return __clz((int)(a ^ b));
-    }
+    int n;
+    struct Node {
+        const DeltaU32 &d;
+        uint32_t ki;
+        int i;
+        __device__ __forceinline__ int operator()(int j) const {
+            if ((uint32_t)j >= (uint32_t)d.n) return -1;
+            const uint32_t x = ki ^ d.k[j];
+            const int by_key = __clz((int)x), by_pos = 32 + __clz(i ^ j);
+            return x ? by_key : by_pos;
+        }
+    };
+    __device__ __forceinline__ Node at(int i) const { return Node{*this, k[i], i}; }
 };
 
 // Ensemble keys carry the sim id above bit 30.  Equal keys share a sim and
@@ -86,20 +115,26 @@ struct DeltaU32 {
 struct DeltaU64 {
     KeyWindow<uint64_t> k;
     const int64_t *__restrict__ sim_off;
-    int64_t n;
-    __device__ __forceinline__ int operator()(int64_t i, int64_t j) const {
-        if (j < 0 || j >= n) return -1;
-        const uint64_t a = k[i], b = k[j];
-        if (a == b) {
-            const int64_t off = sim_off[a >> kSimShift];
-            return 64 + __clz((int)((uint32_t)(i - off) ^ (uint32_t)(j - off)));
+    int n;
+    struct Node {
+        const DeltaU64 &d;
+        uint64_t ki;
+        int i;
+        __device__ __forceinline__ int operator()(int j) const {
+            if ((uint32_t)j >= (uint32_t)d.n) return -1;
+            const uint64_t b = d.k[j];
+            if (ki == b) {
+                const int off = (int)d.sim_off[ki >> kSimShift];
+                return 64 + __clz((i - off) ^ (j - off));
+            }
+            return __clzll((long long)(ki ^ b));
         }
-        return __clzll((long long)(a ^ b));
-    }
+    };
+    __device__ __forceinline__ Node at(int i) const { return Node{*this, k[i], i}; }
 };
 
 struct Split {
-    int64_t lo, hi, gamma;
+    int lo, hi, gamma;
     uint32_t lc, rc;
 };
 
@@ -133,30 +168,31 @@ __device__ __forceinline__ bool spans(uint2 r, int shift) { return (r.x >> shift
 
 // kInBlock when the node's leaf range lies in one phase-1 block.
 __device__ __forceinline__ uint32_t in_block(const Split &sp) {
-    return ((uint64_t)sp.lo >> kBlockShift) == ((uint64_t)sp.hi >> kBlockShift) ? kInBlock : 0u;
+    return ((uint32_t)sp.lo >> kBlockShift) == ((uint32_t)sp.hi >> kBlockShift) ? kInBlock : 0u;
 }
 
 template <typename Delta>
-__device__ __forceinline__ Split karras_node(const Delta &delta, int64_t i, int64_t n) {
-    // lbvh.wgsl:101-152
-    const int d = delta(i, i + 1) > delta(i, i - 1) ? 1 : -1;
-    const int delta_min = delta(i, i - d);
-    int64_t l_max = 2;
-    while (delta(i, i + d * l_max) > delta_min) l_max *= 2;
-    int64_t l = 0;
-    for (int64_t t = l_max / 2; t >= 1; t /= 2)
-        if (delta(i, i + d * (l + t)) > delta_min) l += t;
-    const int64_t j = i + d * l;
-    const int delta_node = delta(i, j);
-    int64_t s = 0;
-    int64_t t = (l + 1) / 2;
+__device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n) {
+    // lbvh.wgsl:101-152; probe i + d * x is written as a select on d
+    const auto dl = delta.at(i);
+    const int d = dl(i + 1) > dl(i - 1) ? 1 : -1;
+    const int delta_min = dl(i - d);
+    int l_max = 2;
+    while (dl(d > 0 ? i + l_max : i - l_max) > delta_min) l_max *= 2;
+    int l = 0;
+    for (int t = l_max >> 1; t >= 1; t >>= 1)
+        if (dl(d > 0 ? i + l + t : i - l - t) > delta_min) l += t;
+    const int j = d > 0 ? i + l : i - l;
+    const int delta_node = dl(j);
+    int s = 0;
+    int t = (l + 1) >> 1;
     for (;;) {
-        if (delta(i, i + d * (s + t)) > delta_node) s += t;
+        if (dl(d > 0 ? i + s + t : i - s - t) > delta_node) s += t;
         if (t == 1) break;
-        t = (t + 1) / 2;
+        t = (t + 1) >> 1;
     }
     Split r;
-    r.gamma = i + d * s + (d < 0 ? d : 0);
+    r.gamma = d > 0 ? i + s : i - s - 1;
     r.lo = i < j ? i : j;
     r.hi = i < j ? j : i;
     r.lc = (r.lo == r.gamma) ? (uint32_t)(n - 1 + r.gamma) : (uint32_t)r.gamma;
@@ -178,16 +214,30 @@ __device__ __forceinline__ void clear_words(uint32_t *__restrict__ p, int64_t wo
             reinterpret_cast<uint4 *>(slots)[k] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// Leaf record {x, y, mass, 0} of sorted position i (lbvh.wgsl:179-187).
+// The sort's last pass writes them (its gathers overlap its ranking); a
+// single body, which has no sort, is gathered by the fill.
+__device__ __forceinline__ float4 gather_leaf(const TreeBufs &tb, const uint32_t *__restrict__ idx,
+                                              const float *__restrict__ mass,
+                                              const float2 *__restrict__ pos, int i) {
+    const uint32_t b = idx[i];
+    const float2 p = pos[b];
+    const float4 c = make_float4(p.x, p.y, mass[b], 0.0f);
+    tb.leaf[i] = c;
+    return c;
+}
+
 __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
     const uint32_t *__restrict__ keys, int64_t n, TreeBufs tb, uint32_t *__restrict__ clear,
     int64_t clear_n, uint32_t *__restrict__ slots) {
     __shared__ uint32_t s_keys[kBuildWindow];
     clear_words(clear, clear_n, slots);
-    const DeltaU32 delta{load_window(keys, n, s_keys), n};
+    const int n32 = (int)n;
+    const DeltaU32 delta{load_window(keys, n32, s_keys), n32};
     for (int r = 0; r < kBuildPerThread; ++r) {
-        const int64_t i = (int64_t)blockIdx.x * kBuildNodes + r * kBuildThreads + threadIdx.x;
-        if (i >= n - 1) return;
-        const Split sp = karras_node(delta, i, n);
+        const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
+        if (i >= n32 - 1) return;
+        const Split sp = karras_node(delta, i, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
         // the walk reads the indices from the pair record
         *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
@@ -211,11 +261,12 @@ __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t
                                                         uint32_t *__restrict__ slots) {
     __shared__ uint64_t s_keys[kBuildWindow];
     clear_words(clear, clear_n, slots);
-    const DeltaU64 delta{load_window(keys, n, s_keys), sim_off, n};
+    const int n32 = (int)n;
+    const DeltaU64 delta{load_window(keys, n32, s_keys), sim_off, n32};
     for (int r = 0; r < kBuildPerThread; ++r) {
-        const int64_t i = (int64_t)blockIdx.x * kBuildNodes + r * kBuildThreads + threadIdx.x;
-        if (i >= n - 1) return;
-        const Split sp = karras_node(delta, i, n);
+        const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
+        if (i >= n32 - 1) return;
+        const Split sp = karras_node(delta, i, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
         // the walk reads the indices from the pair record
         *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
@@ -248,13 +299,12 @@ __device__ __forceinline__ void merge_values(const float4 &L, const float4 &LB, 
     B.z = fmaxf(LB.z, RB.z);
     B.w = fmaxf(LB.w, RB.w);
     C.z = L.z + R.z;
-    if (C.z > 0.0f) {
-        C.x = (L.x * L.z + R.x * R.z) / C.z;
-        C.y = (L.y * L.z + R.y * R.z) / C.z;
-    } else {
-        C.x = 0.0f;
-        C.y = 0.0f;
-    }
+    // both quotients, then a select: a branch here would hold back the
+    // children's loads until the mass test
+    const float qx = (L.x * L.z + R.x * R.z) / C.z;
+    const float qy = (L.y * L.z + R.y * R.z) / C.z;
+    C.x = C.z > 0.0f ? qx : 0.0f;
+    C.y = C.z > 0.0f ? qy : 0.0f;
     C.w = fmaxf(B.z - B.x, B.w - B.y);
 }
 
@@ -285,9 +335,10 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
                                                                 int64_t n, TreeBufs tb) {
     __shared__ uint2 s_child[kFillBlock];  // internal node base+j
     __shared__ uint32_t s_par_int[kFillBlock];
-    __shared__ float4 s_cm_int[kFillBlock];
-    __shared__ float4 s_bb_int[kFillBlock];
-    __shared__ float4 s_cm_leaf[kFillBlock];
+    // {cm, mass, length}: internal node base+j at j, leaf base+j at
+    // kFillBlock+j; AABBs of the internal nodes (a leaf's is its position)
+    __shared__ float4 s_cm[2 * kFillBlock];
+    __shared__ float4 s_bb[kFillBlock];
     __shared__ uint32_t s_arrive[kFillBlock];  // 2: the node is complete
 
     const uint32_t j = threadIdx.x;
@@ -300,58 +351,68 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
         s_par_int[j] = tb.parent[t];
     }
     uint32_t pw_leaf = kNoNode;
+    float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (t < (uint64_t)n) {
         pw_leaf = n > 1 ? tb.parent[nint + t] : kNoNode;
-        const uint32_t b = idx[t];
-        const float2 p = pos[b];
-        // lbvh.wgsl:179-187
-        const float4 C = make_float4(p.x, p.y, mass[b], 0.0f);
-        s_cm_leaf[j] = C;
-        tb.leaf[t] = C;
+        // the build gathered the leaf records; a single body has no build
+        C = n == 1 ? gather_leaf(tb, idx, mass, pos, 0) : tb.leaf[t];
+        s_cm[kFillBlock + j] = C;
     }
     __syncthreads();
 
     // lbvh.wgsl:191-232, restricted to the nodes inside this block: a node
-    // whose parent word lacks kInBlock is a top of the block's forest.
+    // whose parent word lacks kInBlock is a top of the block's forest.  The
+    // climbing thread keeps its node in registers; the child that arrives
+    // first left its record in LDS, so a level is the arrival count (its
+    // wait also covers the parent's child pair and parent word, read
+    // beforehand), one read of the sibling and the merge.
     if (t < (uint64_t)n && n > 1) {
+        float4 B = make_float4(C.x, C.y, C.x, C.y);
         uint32_t pw = pw_leaf;
-        for (int level = 0; level < 128; ++level) {
-            if (!(pw & kInBlock)) break;
-            const uint32_t par = pw & kParentMask;
-            const uint32_t lp = par - base;
-            if (lp >= (uint32_t)kFillBlock) break;  // corrupt parent word
+        uint32_t lp = (pw & kParentMask) - base;
+        bool up = (pw & kInBlock) && lp < (uint32_t)kFillBlock;
+        uint2 ch = make_uint2(0u, 0u);
+        uint32_t ppw = 0u;
+        if (up) {
+            ch = s_child[lp];
+            ppw = s_par_int[lp];
+        }
+        for (int level = 0; up && level < 128; ++level) {
             const uint32_t prev = __hip_atomic_fetch_add(&s_arrive[lp], 1u, __ATOMIC_ACQ_REL,
                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             if (prev == 0u) break;
-            const uint2 ch = s_child[lp];
-            float4 L, LB, R, RB;
-            if (ch.x >= nint) {
-                L = s_cm_leaf[ch.x - nint - base];
-                LB = make_float4(L.x, L.y, L.x, L.y);
-            } else {
-                L = s_cm_int[ch.x - base];
-                LB = s_bb_int[ch.x - base];
+            const bool right = (pw >> 31) != 0u;
+            const uint32_t sib = right ? ch.x : ch.y;
+            const bool sleaf = sib >= nint;
+            const uint32_t si = sleaf ? sib - nint - base + (uint32_t)kFillBlock : sib - base;
+            // one read from a selected LDS address (a leaf's record holds
+            // its AABB corner), so both reads issue together
+            const float4 S = s_cm[si];
+            float4 SR = *(sleaf ? &s_cm[si] : &s_bb[si]);
+            // keep the read whole (else .zw become a second, conditional read)
+            asm volatile("" : "+v"(SR.z), "+v"(SR.w));
+            const float4 SB = sleaf ? make_float4(SR.x, SR.y, SR.x, SR.y) : SR;
+            float4 NC, NB;
+            merge_values(right ? S : C, right ? SB : B, right ? C : S, right ? B : SB, NC, NB);
+            C = NC;
+            B = NB;
+            s_cm[lp] = C;
+            s_bb[lp] = B;
+            const uint32_t par = lp + base;
+            pw = ppw;
+            lp = (pw & kParentMask) - base;
+            up = par != 0u && (pw & kInBlock) && lp < (uint32_t)kFillBlock;
+            if (up) {
+                ch = s_child[lp];
+                ppw = s_par_int[lp];
             }
-            if (ch.y >= nint) {
-                R = s_cm_leaf[ch.y - nint - base];
-                RB = make_float4(R.x, R.y, R.x, R.y);
-            } else {
-                R = s_cm_int[ch.y - base];
-                RB = s_bb_int[ch.y - base];
-            }
-            float4 C, B;
-            merge_values(L, LB, R, RB, C, B);
-            s_cm_int[lp] = C;
-            s_bb_int[lp] = B;
-            if (par == 0u) break;
-            pw = s_par_int[lp];
         }
     }
     __syncthreads();
 
     // Write the block's completed nodes where their readers look for them.
     if (t < (uint64_t)n) {
-        float4 c = s_cm_leaf[j];
+        float4 c = s_cm[kFillBlock + j];
         c.w = kLeafLength;
         if (n == 1)
             publish(tb, 0u, 0u, c, make_float4(c.x, c.y, c.x, c.y));
@@ -359,7 +420,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
             publish(tb, nint + t, pw_leaf, c, make_float4(c.x, c.y, c.x, c.y));
     }
     if (t < nint && s_arrive[j] == 2u)
-        publish(tb, t, t ? s_par_int[j] : 0u, s_cm_int[j], s_bb_int[j]);
+        publish(tb, t, t ? s_par_int[j] : 0u, s_cm[j], s_bb[j]);
 }
 
 // Climb from `cur` (its data already published) while the parents' leaf
@@ -439,17 +500,18 @@ __device__ __forceinline__ uint32_t find_node(const ListLds &ls, uint32_t m, uin
 // the children (lbvh.wgsl:205-228), keep the node in its entry, hand it to
 // the parent entry's input and count the parent's arrival (workgroup-scope
 // acquire/release); the child that arrives last climbs on, the other stops.
-// A level is one round of LDS reads, the merge and one LDS atomic: the
-// chains of the upper levels are long (tens of levels), so the nodes are
-// published to memory only afterwards, by all threads together.
+// The climbing thread keeps its node in registers: a level is the merge,
+// the LDS writes with the parent's fixed words read alongside, the arrival
+// count and one read of the sibling's record.  The chains of the upper
+// levels are long (tens of levels), so the nodes are published to memory
+// only afterwards, by all threads together.
 __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, uint32_t m) {
     const uint32_t e = threadIdx.x;
     if (e < m && ls.pend[e] == 0u) {
         uint32_t cur = e;
+        float4 L = ls.in[cur][0], LB = ls.in[cur][1], R = ls.in[cur][2], RB = ls.in[cur][3];
+        uint32_t pw = ls.pw[cur], q = ls.par[cur];
         for (int level = 0; level < 128; ++level) {
-            const float4 L = ls.in[cur][0], LB = ls.in[cur][1];
-            const float4 R = ls.in[cur][2], RB = ls.in[cur][3];
-            const uint32_t pw = ls.pw[cur], q = ls.par[cur];
             float4 C, B;
             merge_values(L, LB, R, RB, C, B);
             ls.out[cur][0] = C;
@@ -458,10 +520,20 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
             const uint32_t side = pw >> 31;
             ls.in[q][2 * side] = C;
             ls.in[q][2 * side + 1] = B;
+            // the parent's own words are fixed: read before the arrival
+            const uint32_t qpw = ls.pw[q], qq = ls.par[q];
             if (__hip_atomic_fetch_sub(&ls.pend[q], 1u, __ATOMIC_ACQ_REL,
                                        __HIP_MEMORY_SCOPE_WORKGROUP) != 1u)
                 break;  // the sibling is still pending: it climbs on
+            // the sibling's record, there before its arrival (or preloaded)
+            const float4 S = ls.in[q][2 * (side ^ 1u)], SB = ls.in[q][2 * (side ^ 1u) + 1];
+            L = side ? S : C;
+            LB = side ? SB : B;
+            R = side ? C : S;
+            RB = side ? B : SB;
             cur = q;
+            pw = qpw;
+            q = qq;
         }
     }
     __syncthreads();

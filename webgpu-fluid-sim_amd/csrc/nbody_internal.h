@@ -79,10 +79,19 @@ void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
 // (8, 10 or 11) per pass; the digit histograms must be in sc.hist.
 // Ping-pongs between buffers a and b and returns which buffer (0 = a, 1 = b)
 // holds the result.
+// The last pass also writes the sorted leaf records leaf[t] = {pos, mass, 0}
+// of body vals[t] when `leaf` is set (its gathers overlap the ranking).
+struct LeafGather {
+    const float2 *pos;
+    const float *mass;
+    float4 *leaf;
+};
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
-                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc, hipStream_t s);
+                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
+                    const LeafGather &lg, hipStream_t s);
 int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,
-                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc, hipStream_t s);
+                    int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
+                    const LeafGather &lg, hipStream_t s);
 
 // ---------------------------------------------------------------- tree
 // Node data is stored in child pairs: each internal node p has a 64-byte

@@ -32,7 +32,7 @@ def _candidates(variants, blocks, stacks, xcd, lane_bodies):
             yield (v, b, st, x, lb)
 
 
-DEFAULT_RADIX = (8, 10, 11)
+DEFAULT_RADIX = (10, 8, 11)
 
 
 def _time(sim, mass, pos, vel, steps, phases):
