@@ -166,7 +166,7 @@ __device__ __forceinline__ uint32_t code_bbox30(float2 p, float4 b) {
 // The sort needs these counts before its first pass; counting here saves a
 // kernel and a read of the keys.
 constexpr int kEncodeThreads = 1024;
-constexpr int kEncodeTile = 16 * kEncodeThreads;
+constexpr int kEncodeTile = 4 * kEncodeThreads;
 template <int kMode, bool kEnsemble, int RB>
 __global__ __launch_bounds__(kEncodeThreads) void encode_kernel(const float2 *__restrict__ pos, int64_t n,
                                                      float4 *__restrict__ bbox,

@@ -145,30 +145,64 @@ constexpr int kBlockShift = 10;
 constexpr int kSuperShift = 15;
 constexpr int kFillBlock = 1 << kBlockShift;
 constexpr int kSuper = 1 << kSuperShift;                         // leaves per super-block
-// The fill's upper levels: the phase-2 workgroup of super-block s finds
-// its phase-2 nodes (and the top nodes) by their leaf ranges among node
-// indices [s * kSuper, (s+1) * kSuper): a node's index is an end of its
-// range, so a node whose range lies in super-block s has its index there.
+// The fill's upper levels: the build lists every node whose leaf range
+// spans a block boundary but lies in one super-block in that super-block's
+// phase-2 list, and every node whose range spans super-blocks in the top
+// list (fill_mid_kernel resolves them); tb.slot holds each listed node's
+// position in its list.  A node's index is an end of its range, so a
+// phase-2 node of super-block s has its index in [s * kSuper, (s+1) *
+// kSuper): a phase-2 list has at most kSuper entries.  The top nodes whose
+// index lies in one super-block are the nodes whose range holds its left
+// or right boundary, two root-ward chains of at most the tree depth (< 97
+// for 38-bit keys tie-broken on positions below 2^30), so the top list has
+// fewer than kTopPer entries per super-block.
 constexpr uint32_t kLdsList = 1024;  // longest list the fill resolves in LDS
-constexpr uint32_t kScan = kSuper / 1024;  // node indices per thread of the scan
 __host__ __device__ inline uint32_t num_supers(int64_t n) {
     return (uint32_t)((n + kSuper - 1) / kSuper);
 }
-// The top nodes (ranges across super-blocks) whose index lies in
-// super-block s are the nodes whose range holds its left or right boundary:
-// two root-ward chains, at most 2 x the tree depth (< 2 x 97 for 38-bit
-// keys tie-broken on positions below 2^30).  Each super-block lists them in
-// its own region of kTopPer words of tb.lists, so no workgroup waits on
-// another to reserve space; tb.slot holds each top node's list position.
 constexpr uint32_t kTopPer = 256;
-// Counter words (tb.counts): [0] phase-2 done count, [2 + s] number of top
-// nodes of super-block s.
+// tb.counts: [0] phase-2 workgroups done; the length of list l (phase-2
+// list of super-block l < S, the top list l = S) at list_count(l), one
+// 128-byte line each, as the builds' atomics on one line would serialize.
+// tb.lists: the phase-2 lists (kSuper words each), then the top list.
+__host__ __device__ inline uint32_t list_count(uint32_t l) { return 32u * (l + 1u); }
 
 __device__ __forceinline__ bool spans(uint2 r, int shift) { return (r.x >> shift) != (r.y >> shift); }
 
 // kInBlock when the node's leaf range lies in one phase-1 block.
 __device__ __forceinline__ uint32_t in_block(const Split &sp) {
     return ((uint32_t)sp.lo >> kBlockShift) == ((uint32_t)sp.hi >> kBlockShift) ? kInBlock : 0u;
+}
+
+// Add node i (leaf range sp.lo..sp.hi) to its upper-level list, if any.
+// Called by every active lane of the wavefront; the lanes adding to one
+// list share one atomic.  A list is in no particular order.
+__device__ __forceinline__ void list_node(const TreeBufs &tb, uint32_t i, const Split &sp,
+                                          int n) {
+    const uint32_t lo = (uint32_t)sp.lo, hi = (uint32_t)sp.hi;
+    const bool top = (lo >> kSuperShift) != (hi >> kSuperShift);
+    const bool listed = top || (lo >> kBlockShift) != (hi >> kBlockShift);
+    const uint32_t S = num_supers(n);
+    const uint32_t lid = top ? S : (i >> kSuperShift);  // S: the top list
+    const int lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(listed);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)lid, leader);
+        const uint64_t peers = __ballot(listed && lid == l);
+        uint32_t pos0 = 0;
+        if (lane == leader)
+            pos0 = atomicAdd(&tb.counts[list_count(l)], (uint32_t)__builtin_popcountll(peers));
+        pos0 = (uint32_t)__builtin_amdgcn_readlane((int)pos0, leader);
+        if (listed && lid == l) {
+            const uint32_t pos =
+                pos0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+            tb.lists[(size_t)l * kSuper + pos] = i;  // the top list follows the S phase-2 lists
+            tb.slot[i] = pos;
+        }
+        todo &= ~peers;
+    }
 }
 
 template <typename Delta>
@@ -238,6 +272,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
         if (i >= n32 - 1) return;
         const Split sp = karras_node(delta, i, n32);
+        list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
         // the walk reads the indices from the pair record
         *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
@@ -267,6 +302,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
         if (i >= n32 - 1) return;
         const Split sp = karras_node(delta, i, n32);
+        list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
         // the walk reads the indices from the pair record
         *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
@@ -339,73 +375,70 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
     // kFillBlock+j; AABBs of the internal nodes (a leaf's is its position)
     __shared__ float4 s_cm[2 * kFillBlock];
     __shared__ float4 s_bb[kFillBlock];
-    __shared__ uint32_t s_arrive[kFillBlock];  // 2: the node is complete
+    __shared__ uint32_t s_arrive[kFillBlock];  // arrivals; 2 once the node is complete
 
     const uint32_t j = threadIdx.x;
     const uint32_t base = blockIdx.x * kFillBlock;
     const uint32_t nint = (uint32_t)(n - 1);  // internal nodes
     const uint32_t t = base + j;
     s_arrive[j] = 0u;
+    uint2 ch = make_uint2(0u, 0u), rg = make_uint2(0u, 0u);
     if (t < nint) {
-        s_child[j] = tb.child[t];
+        ch = tb.child[t];
+        rg = tb.range[t];
+        s_child[j] = ch;
         s_par_int[j] = tb.parent[t];
     }
     uint32_t pw_leaf = kNoNode;
-    float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (t < (uint64_t)n) {
         pw_leaf = n > 1 ? tb.parent[nint + t] : kNoNode;
-        // the build gathered the leaf records; a single body has no build
-        C = n == 1 ? gather_leaf(tb, idx, mass, pos, 0) : tb.leaf[t];
-        s_cm[kFillBlock + j] = C;
+        // the sort gathered the leaf records; a single body has no sort
+        s_cm[kFillBlock + j] = n == 1 ? gather_leaf(tb, idx, mass, pos, 0) : tb.leaf[t];
     }
     __syncthreads();
 
-    // lbvh.wgsl:191-232, restricted to the nodes inside this block: a node
-    // whose parent word lacks kInBlock is a top of the block's forest.  The
-    // climbing thread keeps its node in registers; the child that arrives
-    // first left its record in LDS, so a level is the arrival count (its
-    // wait also covers the parent's child pair and parent word, read
-    // beforehand), one read of the sibling and the merge.
-    if (t < (uint64_t)n && n > 1) {
-        float4 B = make_float4(C.x, C.y, C.x, C.y);
-        uint32_t pw = pw_leaf;
-        uint32_t lp = (pw & kParentMask) - base;
-        bool up = (pw & kInBlock) && lp < (uint32_t)kFillBlock;
-        uint2 ch = make_uint2(0u, 0u);
-        uint32_t ppw = 0u;
-        if (up) {
-            ch = s_child[lp];
-            ppw = s_par_int[lp];
-        }
-        for (int level = 0; up && level < 128; ++level) {
-            const uint32_t prev = __hip_atomic_fetch_add(&s_arrive[lp], 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (prev == 0u) break;
+    // lbvh.wgsl:191-232, restricted to the nodes whose leaf range lies in
+    // this block.  A climb starts at each such node with two leaf children
+    // and carries the node in registers.  At the parent, a leaf sibling's
+    // record is in LDS from the start; an internal sibling is another
+    // climb's, so the two meet at an arrival count and the second one
+    // climbs on.  A level is one read of the parent's child pair and
+    // parent word, then of the sibling (after the count for an internal
+    // sibling), and the merge; a parent word without kInBlock ends it.
+    if (t < nint && ch.x >= nint && ch.y >= nint && !spans(rg, kBlockShift)) {
+        const float4 L = s_cm[kFillBlock + (ch.x - nint - base)];
+        const float4 R = s_cm[kFillBlock + (ch.y - nint - base)];
+        float4 C, B;
+        merge_values(L, make_float4(L.x, L.y, L.x, L.y), R, make_float4(R.x, R.y, R.x, R.y), C, B);
+        uint32_t cur = j, pw = s_par_int[j];
+        for (int level = 0; level < 128; ++level) {
+            s_cm[cur] = C;
+            s_bb[cur] = B;
+            s_arrive[cur] = 2u;  // complete (no climb arrives here any more)
+            if (base + cur == 0u || !(pw & kInBlock)) break;  // the root, or a block top
+            const uint32_t lp = (pw & kParentMask) - base;
+            if (lp >= (uint32_t)kFillBlock) break;  // corrupt parent word
+            const uint2 pch = s_child[lp];
+            const uint32_t ppw = s_par_int[lp];
             const bool right = (pw >> 31) != 0u;
-            const uint32_t sib = right ? ch.x : ch.y;
-            const bool sleaf = sib >= nint;
-            const uint32_t si = sleaf ? sib - nint - base + (uint32_t)kFillBlock : sib - base;
-            // one read from a selected LDS address (a leaf's record holds
-            // its AABB corner), so both reads issue together
-            const float4 S = s_cm[si];
-            float4 SR = *(sleaf ? &s_cm[si] : &s_bb[si]);
-            // keep the read whole (else .zw become a second, conditional read)
-            asm volatile("" : "+v"(SR.z), "+v"(SR.w));
-            const float4 SB = sleaf ? make_float4(SR.x, SR.y, SR.x, SR.y) : SR;
+            const uint32_t sib = right ? pch.x : pch.y;
+            float4 S, SB;
+            if (sib < nint) {
+                if (__hip_atomic_fetch_add(&s_arrive[lp], 1u, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+                    break;  // the sibling's climb is still on its way: it goes on
+                S = s_cm[sib - base];
+                SB = s_bb[sib - base];
+            } else {
+                S = s_cm[kFillBlock + (sib - nint - base)];
+                SB = make_float4(S.x, S.y, S.x, S.y);
+            }
             float4 NC, NB;
             merge_values(right ? S : C, right ? SB : B, right ? C : S, right ? B : SB, NC, NB);
             C = NC;
             B = NB;
-            s_cm[lp] = C;
-            s_bb[lp] = B;
-            const uint32_t par = lp + base;
+            cur = lp;
             pw = ppw;
-            lp = (pw & kParentMask) - base;
-            up = par != 0u && (pw & kInBlock) && lp < (uint32_t)kFillBlock;
-            if (up) {
-                ch = s_child[lp];
-                ppw = s_par_int[lp];
-            }
         }
     }
     __syncthreads();
@@ -449,62 +482,27 @@ __device__ __forceinline__ void climb_wg(const TreeBufs &tb, uint32_t nint, uint
     }
 }
 
-// Exclusive prefix sum over the kFillBlock threads of a workgroup; *total
-// receives the sum.  sh: kFillBlock / 64 + 1 words of LDS.
-__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *sh, uint32_t *total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    __syncthreads();
-    if (lane == 63) sh[wid] = x;
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (int w = 0; w < kFillBlock / 64; ++w) {
-        const uint32_t t = sh[w];
-        before += w < wid ? t : 0u;
-        all += t;
-    }
-    *total = all;
-    return before + x - v;
-}
-
 // One upper-level node list in LDS: entry e is node node[e] with parent
 // word pw[e], its parent's entry par[e] (kNone: the parent is not in the
-// list), the number of its children still pending, and its children's
-// {cm, aabb} records (left, right).
+// list), the number of its children that are entries (pend), an arrival
+// count, and its children's {cm, aabb} records (left, right).
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 struct ListLds {
-    uint32_t node[kLdsList], pw[kLdsList], par[kLdsList], pend[kLdsList];
+    uint32_t node[kLdsList], pw[kLdsList], par[kLdsList], pend[kLdsList], cnt[kLdsList];
     float4 in[kLdsList][4];
     float4 out[kLdsList][2];  // the entry's own {cm, aabb} once complete
 };
 
-// Index of `x` in the sorted node[0, m), or kNone.
-__device__ __forceinline__ uint32_t find_node(const ListLds &ls, uint32_t m, uint32_t x) {
-    uint32_t lo = 0, hi = m;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ls.node[mid] < x)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo < m && ls.node[lo] == x ? lo : kNone;
-}
-
 // Resolve a list set up in ls (m <= kLdsList entries; called by the whole
-// workgroup).  The entries with no pending child start a climb in LDS: merge
-// the children (lbvh.wgsl:205-228), keep the node in its entry, hand it to
-// the parent entry's input and count the parent's arrival (workgroup-scope
-// acquire/release); the child that arrives last climbs on, the other stops.
-// The climbing thread keeps its node in registers: a level is the merge,
-// the LDS writes with the parent's fixed words read alongside, the arrival
-// count and one read of the sibling's record.  The chains of the upper
-// levels are long (tens of levels), so the nodes are published to memory
-// only afterwards, by all threads together.
+// workgroup).  The entries with no child among the entries start a climb in
+// LDS that carries the node in registers: merge the children (lbvh.wgsl:
+// 205-228), keep the node in its entry and go to the parent entry.  A
+// parent with one entry child has its other input preloaded, so the climb
+// goes straight on; a parent with two meets both climbs at an arrival count
+// (workgroup-scope acquire/release), and the second one goes on.  The
+// chains of the upper levels are long (tens of levels, mostly parents with
+// one entry child), so the nodes are published to memory only afterwards,
+// by all threads together.
 __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, uint32_t m) {
     const uint32_t e = threadIdx.x;
     if (e < m && ls.pend[e] == 0u) {
@@ -512,21 +510,34 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
         float4 L = ls.in[cur][0], LB = ls.in[cur][1], R = ls.in[cur][2], RB = ls.in[cur][3];
         uint32_t pw = ls.pw[cur], q = ls.par[cur];
         for (int level = 0; level < 128; ++level) {
+            // the parent's fixed words and its other input are read before
+            // the merge, which hides their latency
+            const uint32_t side = pw >> 31;
+            uint32_t qpw = 0u, qq = kNone, qpend = 0u;
+            float4 S = make_float4(0.0f, 0.0f, 0.0f, 0.0f), SB = S;
+            if (q != kNone) {
+                qpw = ls.pw[q];
+                qq = ls.par[q];
+                qpend = ls.pend[q];
+                S = ls.in[q][2 * (side ^ 1u)];
+                SB = ls.in[q][2 * (side ^ 1u) + 1];
+            }
             float4 C, B;
             merge_values(L, LB, R, RB, C, B);
             ls.out[cur][0] = C;
             ls.out[cur][1] = B;
             if (q == kNone) break;
-            const uint32_t side = pw >> 31;
-            ls.in[q][2 * side] = C;
-            ls.in[q][2 * side + 1] = B;
-            // the parent's own words are fixed: read before the arrival
-            const uint32_t qpw = ls.pw[q], qq = ls.par[q];
-            if (__hip_atomic_fetch_sub(&ls.pend[q], 1u, __ATOMIC_ACQ_REL,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP) != 1u)
-                break;  // the sibling is still pending: it climbs on
-            // the sibling's record, there before its arrival (or preloaded)
-            const float4 S = ls.in[q][2 * (side ^ 1u)], SB = ls.in[q][2 * (side ^ 1u) + 1];
+            if (qpend == 2u) {
+                // both children are entries: the one that arrives second
+                // climbs on, with the other's record read after the count
+                ls.in[q][2 * side] = C;
+                ls.in[q][2 * side + 1] = B;
+                if (__hip_atomic_fetch_add(&ls.cnt[q], 1u, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+                    break;
+                S = ls.in[q][2 * (side ^ 1u)];
+                SB = ls.in[q][2 * (side ^ 1u) + 1];
+            }
             L = side ? S : C;
             LB = side ? SB : B;
             R = side ? C : S;
@@ -541,113 +552,64 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
         publish(tb, ls.node[k], ls.pw[k], ls.out[k][0], ls.out[k][1]);
 }
 
-// Phases 2 and 3.  Workgroup sb scans the leaf ranges of node indices
-// [sb * kSuper, (sb+1) * kSuper): its phase-2 nodes (range inside the
-// super-block, across a block boundary) become an LDS list in index order,
-// its top nodes (range across super-blocks) go to the global top list.  The
-// phase-2 list is resolved in LDS (or, longer than lds_limit, by climbing
-// from the phase-2 nodes' other children).  The workgroup that finishes last (a
-// device-scope fence orders each workgroup's writes before its count)
-// resolves the top list the same way (or climbs from the top nodes'
-// other children) and clears the counters.
+// Set up entry t < m of a list of nodes (list[t]) in ls: its parent's
+// entry (tb.slot) when the parent is listed, the number of its children
+// that are entries, and both children's records as the last fill phase
+// left them (an entry child's is replaced by its climb).  `shift`:
+// kBlockShift for a phase-2 list, kSuperShift for the top list.
+__device__ __forceinline__ void setup_entry(const TreeBufs &tb, ListLds &ls,
+                                            const uint32_t *list, uint32_t t, uint32_t nint,
+                                            int shift) {
+    const uint32_t p = list[t];
+    const uint32_t pw = tb.parent[p];
+    const uint2 ch = tb.child[p];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        ls.in[t][2 * side] = tb.pcm[kPairStride * (size_t)p + side];
+        ls.in[t][2 * side + 1] = tb.paabb[2 * (size_t)p + side];
+    }
+    const uint32_t par = pw & kParentMask;
+    uint32_t q = kNone;
+    // a top node's parent is a top node; a phase-2 node's parent is listed
+    // with it unless it spans super-blocks
+    if (par < nint && (shift == kSuperShift || !spans(tb.range[par], kSuperShift)))
+        q = tb.slot[par];
+    const uint32_t pend = (ch.x < nint && spans(tb.range[ch.x], shift) ? 1u : 0u) +
+                          (ch.y < nint && spans(tb.range[ch.y], shift) ? 1u : 0u);
+    ls.node[t] = p;
+    ls.pw[t] = pw;
+    ls.par[t] = q;
+    ls.pend[t] = pend;
+    ls.cnt[t] = 0u;
+}
+
+// Phases 2 and 3.  Workgroup sb resolves the phase-2 list of super-block sb
+// in LDS (or, longer than lds_limit, climbs from the phase-2 nodes' other
+// children).  The workgroup that finishes last (a device-scope fence orders
+// each workgroup's writes before its count) resolves the top list the same
+// way (or climbs from the top nodes' other children) and clears the
+// counters for the next build.
 __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBufs tb,
                                                               uint32_t num_super,
                                                               uint32_t lds_limit) {
     __shared__ ListLds ls;
-    __shared__ uint32_t sh[kFillBlock / 64 + 1];
-    __shared__ uint32_t s_cnt[kScan * 16];
-    __shared__ uint32_t s_pre[kFillBlock];  // top: list offsets of the super-blocks
     __shared__ uint32_t s_last;
     const uint32_t t = threadIdx.x;
     const uint32_t sb = blockIdx.x;
     const uint32_t nint = (uint32_t)(n - 1);
-    uint32_t *top_list = tb.lists;
-
-    // scan: node index base_i + k * kFillBlock + t (coalesced), every load
-    // issued before any is used (indices clamped into the array)
-    const uint32_t lane = t & 63, w = t >> 6;
-    const uint32_t base_i = sb * (uint32_t)kSuper;
-    uint2 r[kScan];
-#pragma unroll
-    for (uint32_t k = 0; k < kScan; ++k) r[k] = tb.range[min(base_i + k * kFillBlock + t, nint - 1u)];
-    uint32_t bits2 = 0, bits3 = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kScan; ++k) {
-        if (base_i + k * kFillBlock + t < nint && spans(r[k], kBlockShift)) {
-            if (spans(r[k], kSuperShift))
-                bits3 |= 1u << k;
-            else
-                bits2 |= 1u << k;
-        }
-    }
-    // list positions in index order: counts per (k, wave), phase-2 in the
-    // low and top in the high half-word, one scan
-#pragma unroll
-    for (uint32_t k = 0; k < kScan; ++k) {
-        const uint64_t b2 = __ballot((bits2 >> k) & 1u), b3 = __ballot((bits3 >> k) & 1u);
-        if (lane == 0) s_cnt[k * 16 + w] = __popcll(b2) | (uint32_t)__popcll(b3) << 16;
-    }
-    __syncthreads();
-    const uint32_t v = t < kScan * 16 ? s_cnt[t] : 0u;
-    uint32_t total;
-    const uint32_t excl = block_scan(v, sh, &total);
-    if (t < kScan * 16) s_cnt[t] = excl;
-    const uint32_t m2 = total & 0xFFFFu, n3 = total >> 16;
-    if (t == 0) tb.counts[2 + sb] = n3;
-    const bool lds2 = m2 <= lds_limit;
-    __syncthreads();
-#pragma unroll
-    for (uint32_t k = 0; k < kScan; ++k) {
-        const uint64_t b2 = __ballot((bits2 >> k) & 1u), b3 = __ballot((bits3 >> k) & 1u);
-        const uint32_t o = s_cnt[k * 16 + w];
-        const uint32_t i = base_i + k * kFillBlock + t;
-        const uint32_t below = __builtin_amdgcn_mbcnt_lo((uint32_t)b2, 0u);
-        if ((bits2 >> k) & 1u) {
-            if (lds2)
-                ls.node[(o & 0xFFFFu) +
-                        __builtin_amdgcn_mbcnt_hi((uint32_t)(b2 >> 32), below)] = i;
-        }
-        if ((bits3 >> k) & 1u) {
-            const uint32_t local =
-                (o >> 16) + __builtin_amdgcn_mbcnt_hi((uint32_t)(b3 >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)b3, 0u));
-            const uint32_t pos = sb * kTopPer + local;  // local < kTopPer (two chains)
-            top_list[pos] = i;
-            tb.slot[i] = pos;
-        }
-    }
-    __syncthreads();
-
-    if (lds2) {
-        if (t < m2) {
-            const uint32_t p = ls.node[t];
-            const uint32_t pw = tb.parent[p];
-            const uint2 ch = tb.child[p];
-            const uint32_t par = pw & kParentMask;
-            // a parent spanning super-blocks is not in the list (not found)
-            ls.par[t] = par < nint ? find_node(ls, m2, par) : kNone;
-            uint32_t pend = 0;
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t c = side ? ch.y : ch.x;
-                if (c < nint && find_node(ls, m2, c) != kNone) {
-                    ++pend;
-                } else {
-                    ls.in[t][2 * side] = tb.pcm[kPairStride * (size_t)p + side];
-                    ls.in[t][2 * side + 1] = tb.paabb[2 * (size_t)p + side];
-                }
-            }
-            ls.pw[t] = pw;
-            ls.pend[t] = pend;
-        }
+    const uint32_t S = num_super;
+    const uint32_t *list2 = tb.lists + (size_t)sb * kSuper;
+    const uint32_t m2 = tb.counts[list_count(sb)];
+    if (m2 <= lds_limit) {
+        if (t < m2) setup_entry(tb, ls, list2, t, nint, kBlockShift);
         __syncthreads();
         resolve_list(tb, ls, m2);
     } else {
         // climb from every child of a phase-2 node that is not one itself
         const uint32_t lo = sb * (uint32_t)kSuper;
         const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
-        for (uint32_t b = bits2; b; b &= b - 1) {
-            const uint2 ch = tb.child[base_i + __builtin_ctz(b) * kFillBlock + t];
+        for (uint32_t k = t; k < m2; k += kFillBlock) {
+            const uint2 ch = tb.child[list2[k]];
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
                 const uint32_t c = side ? ch.y : ch.x;
@@ -659,65 +621,21 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     __syncthreads();
     if (t == 0) {
         __threadfence();
-        s_last = atomicAdd(&tb.counts[0], 1u) == num_super - 1u;
+        s_last = atomicAdd(&tb.counts[0], 1u) == S - 1u;
     }
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    // The top list: the super-blocks' regions, compacted through their
-    // counts' prefix sums (entry e lies in the last region starting at or
-    // before e).
-    const uint32_t S = num_super;
-    uint32_t m3 = 0xFFFFFFFFu;  // more super-blocks than threads: climb instead
-    if (S <= (uint32_t)kFillBlock) {
-        const uint32_t c = t < S ? tb.counts[2 + t] : 0u;
-        const uint32_t pre = block_scan(c, sh, &m3);
-        if (t < S) s_pre[t] = pre;
-        __syncthreads();
-    }
+    const uint32_t *top = tb.lists + (size_t)S * kSuper;
+    const uint32_t m3 = tb.counts[list_count(S)];
     if (m3 <= lds_limit) {
-        if (t < m3) {
-            uint32_t lo = 0, hi = S;  // last region with s_pre <= t
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_pre[mid] <= t)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const uint32_t p = top_list[lo * kTopPer + (t - s_pre[lo])];
-            const uint32_t pw = tb.parent[p];
-            const uint2 ch = tb.child[p];
-            const uint32_t par = pw & kParentMask;
-            // every ancestor of a top node is a top node
-            uint32_t q = kNone;
-            if (par < nint) {
-                const uint32_t ps = tb.slot[par];
-                q = s_pre[ps / kTopPer] + ps % kTopPer;
-            }
-            ls.par[t] = q;
-            ls.node[t] = p;
-            uint32_t pend = 0;
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t c = side ? ch.y : ch.x;
-                if (c < nint && spans(tb.range[c], kSuperShift)) {
-                    ++pend;
-                } else {
-                    ls.in[t][2 * side] = tb.pcm[kPairStride * (size_t)p + side];
-                    ls.in[t][2 * side + 1] = tb.paabb[2 * (size_t)p + side];
-                }
-            }
-            ls.pw[t] = pw;
-            ls.pend[t] = pend;
-        }
+        if (t < m3) setup_entry(tb, ls, top, t, nint, kSuperShift);
         __syncthreads();
         resolve_list(tb, ls, m3);
     } else {
         // climb from every child of a top node that is not a top node
-        for (uint32_t k = t; k < S * kTopPer; k += kFillBlock) {
-            if (k % kTopPer >= tb.counts[2 + k / kTopPer]) continue;
-            const uint2 ch = tb.child[top_list[k]];
+        for (uint32_t k = t; k < m3; k += kFillBlock) {
+            const uint2 ch = tb.child[top[k]];
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
                 const uint32_t c = side ? ch.y : ch.x;
@@ -726,6 +644,8 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
         }
     }
     __syncthreads();
+    // the next build counts from zero
+    for (uint32_t k = t; k <= S; k += kFillBlock) tb.counts[list_count(k)] = 0u;
     if (t == 0) tb.counts[0] = 0u;
 }
 
@@ -758,7 +678,9 @@ void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int6
     }
 }
 
-int64_t fill_count_words(int64_t cap) { return 2 + (int64_t)num_supers(cap); }
-int64_t fill_list_words(int64_t cap) { return (int64_t)num_supers(cap) * kTopPer; }
+int64_t fill_count_words(int64_t cap) { return list_count(num_supers(cap)) + 1; }
+int64_t fill_list_words(int64_t cap) {
+    return (int64_t)num_supers(cap) * (kSuper + kTopPer);
+}
 
 }  // namespace nbs
