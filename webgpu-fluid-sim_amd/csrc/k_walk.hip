@@ -463,46 +463,6 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
     }
 }
 
-// Stack of the pair walk: entry k = (node, lane mask) in lane k of three
-// VGPRs (see RegStack).  The node's record is not kept: an entry is a node
-// that some lanes open, and what the walk needs next is its pair record,
-// loaded when the entry is popped.  Lane 0 holds a sentinel (node 0, empty
-// mask): popping it ends the walk, so a pop needs no emptiness test and the
-// popped node always has a record to load.  Lane 63 stays free: the walk
-// writes the slot above the top on every iteration and advances the top only
-// when it pushes, which keeps the three registers in place (a conditional
-// write makes the compiler copy them on every iteration).  The stack holds at
-// most one pending left child per level of the current path, so its 62
-// entries serve any tree of depth up to 63 (30-bit keys tie-broken on
-// positions below 2^30 stay below that; deeper trees raise the overflow flag).
-constexpr int kPairStackTop = 63;  // highest top (lanes 0..62 in use)
-
-// 1 when the wave-uniform mask m is not empty, computed on the scalar unit
-// (written as C++, the compiler turns the flag into a vector select and a
-// readfirstlane).  An ALU compare and select only: no memory access.
-__device__ __forceinline__ uint32_t nonzero_flag(uint64_t m) {
-    uint32_t r;
-    asm volatile("s_cmp_lg_u64 %1, 0\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(m) : "scc");
-    return r;
-}
-__device__ __forceinline__ float uniform_f32(float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-struct PairStack {
-    int node, mlo, mhi;
-    __device__ __forceinline__ void put(int l, uint32_t nd, uint64_t m) {
-        node = nbs_writelane((int)nd, l, node);
-        mlo = nbs_writelane((int)(uint32_t)m, l, mlo);
-        mhi = nbs_writelane((int)(uint32_t)(m >> 32), l, mhi);
-    }
-    __device__ __forceinline__ void get(int l, uint32_t &nd, uint64_t &m) const {
-        nd = (uint32_t)__builtin_amdgcn_readlane(node, l);
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(mlo, l);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(mhi, l);
-        m = ((uint64_t)hi << 32) | lo;
-    }
-};
-
 // Pair walk (fast arithmetic, one body per lane).  The loop's unit is an
 // opened node: its pair record holds both children's records and indices,
 // so both children are tested in one iteration, from one scalar load, with
