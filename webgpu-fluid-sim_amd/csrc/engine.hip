@@ -7,6 +7,7 @@
 //
 // HBM layout (capacity C bodies, all 256-byte aligned):
 //   mass f32[C], pos float2[C], vel float2[C]      -- by body index
+//   rank u32[C] sorted position                    -- by body index (sort)
 //   keys[2] u32|u64[C], vals[2] u32[C]             -- sort ping-pong
 //   leaf float4[C] {x, y, mass, 0} in sorted order   -- the walk's own bodies
 //   pcm float4[4C] pair records: children's {com, mass, length} + indices (walk)
@@ -66,6 +67,7 @@ struct nbody_sim {
 
     float *mass = nullptr;
     float2 *pos = nullptr, *vel = nullptr;
+    uint32_t *rank = nullptr;  // sorted position of each body (sort, for the build)
     void *keys[2] = {nullptr, nullptr};
     uint32_t *vals[2] = {nullptr, nullptr};
     TreeBufs tree{};
@@ -188,6 +190,7 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->mass = a.take<float>(cap);
         s->pos = a.take<float2>(cap);
         s->vel = a.take<float2>(cap);
+        s->rank = a.take<uint32_t>(cap);
         for (int b = 0; b < 2; ++b) {
             s->keys[b] = key_size == 8 ? (void *)a.take<uint64_t>(cap) : (void *)a.take<uint32_t>(cap);
             s->vals[b] = a.take<uint32_t>(cap);
@@ -327,8 +330,8 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
                   passes, s->radix_bits, st);
     mark(2);
     // phase 2: stable radix sort (simulation.ts:400); its last pass also
-    // writes the sorted leaf records
-    const LeafGather lg{s->pos, s->mass, s->tree.leaf};
+    // writes each body's sorted position
+    const LeafGather lg{s->rank};
     if (s->ensemble)
         s->result_buf = launch_sort_u64((uint64_t *)s->keys[0], s->vals[0], (uint64_t *)s->keys[1],
                                         s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
@@ -337,14 +340,15 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
                                         s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
     const uint32_t *idx = s->vals[s->result_buf];
     mark(3);
-    // phase 3: Karras build (lbvh.wgsl:85-165); it also zeroes the sort's
-    // scratch for the next step
+    // phase 3: Karras build (lbvh.wgsl:85-165); it also places the sorted
+    // leaf records and zeroes the sort's scratch for the next step
     const int64_t clear = sort_clear_words(n, s->key_bits, s->radix_bits);
+    const LeafPlace lpl{s->pos, s->mass, s->rank};
     if (s->ensemble)
-        launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree,
+        launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree, lpl,
                          s->sim_root, s->sort.hist, clear, nullptr, st);
     else
-        launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, s->sort.hist,
+        launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, lpl, s->sort.hist,
                          clear, slots ? s->bbox_slots : nullptr, st);
     mark(4);
     // phase 4: bottom-up fill (lbvh.wgsl:168-233)

@@ -18,8 +18,8 @@
 //      resolves its global offsets by decoupled look-back, and scatters
 //      through an LDS staging tile so that stores of one digit run are
 //      contiguous;
-//   2. the last pass also writes the tree's sorted leaf records, gathering
-//      each body's position and mass as soon as its index is loaded.
+//   2. the last pass also writes each body's output position (rank), from
+//      which the build places the tree's sorted leaf records.
 // The histograms, tile counters and look-back words must be zero when a
 // sort starts; the tree build that follows every sort clears them for the
 // next step (sort_clear_words), so no clearing kernel runs per step.
@@ -70,10 +70,8 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
 // kOwners) owns digits [t * DPT, t * DPT + DPT) for the scans and the
 // look-back.
 //
-// kLeaf (the last pass): each item's body record {pos, mass, 0} is gathered
-// right after its load and written to the leaf records at the item's output
-// position once the look-back has placed it, so the gathers' latency hides
-// behind the ranking and the look-back.
+// kLeaf (the last pass): each item's body also gets its output position,
+// rank[body] (a scattered write, which no barrier waits for).
 template <typename K, int RB, int kThreads, bool kLeaf>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
@@ -131,16 +129,6 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         k[it] = ok ? kin[p] : (K)0;
         v[it] = ok ? vin[p] : 0u;
         dg[it] = ok ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
-    }
-    float4 rec[kLeaf ? kItems : 1];
-    if constexpr (kLeaf) {
-#pragma unroll
-        for (int it = 0; it < kItems; ++it) {
-            if (dg[it] < (uint32_t)D) {
-                const float2 q = lg.pos[v[it]];
-                rec[it] = make_float4(q.x, q.y, lg.mass[v[it]], 0.0f);
-            }
-        }
     }
 
     // Stable rank inside the wave: peers = lanes holding the same digit.
@@ -265,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
             const uint32_t lp = s_lstart[dd] + s_wcnt[wid * D + dd] + rank[it];
             s_keys[lp] = k[it];
             s_vals[lp] = v[it];
-            if constexpr (kLeaf) lg.leaf[s_gstart[dd] + (lp - s_lstart[dd])] = rec[it];
+            if constexpr (kLeaf) lg.rank[v[it]] = s_gstart[dd] + (lp - s_lstart[dd]);
         }
     }
     __syncthreads();
@@ -293,7 +281,7 @@ int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n,
     uint32_t *vi = vals_a, *vo = vals_b;
     constexpr int T = sort_threads<RB>();
     for (int p = 0; p < passes; ++p) {
-        if (p == passes - 1 && lg.leaf)
+        if (p == passes - 1 && lg.rank)
             onesweep_kernel<K, RB, T, true><<<(unsigned)tiles, T, 0, s>>>(
                 ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
                 sc.counters + p, lg);

@@ -248,9 +248,10 @@ __device__ __forceinline__ void clear_words(uint32_t *__restrict__ p, int64_t wo
             reinterpret_cast<uint4 *>(slots)[k] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// Leaf record {x, y, mass, 0} of sorted position i (lbvh.wgsl:179-187).
-// The sort's last pass writes them (its gathers overlap its ranking); a
-// single body, which has no sort, is gathered by the fill.
+// Leaf record {x, y, mass, 0} of sorted position i (lbvh.wgsl:179-187),
+// gathered by body index: the fill's way for a single body (no sort, no
+// build).  Otherwise the build places every body's record at its sorted
+// position (place_leaf).
 __device__ __forceinline__ float4 gather_leaf(const TreeBufs &tb, const uint32_t *__restrict__ idx,
                                               const float *__restrict__ mass,
                                               const float2 *__restrict__ pos, int i) {
@@ -261,9 +262,16 @@ __device__ __forceinline__ float4 gather_leaf(const TreeBufs &tb, const uint32_t
     return c;
 }
 
+// Body b's leaf record at its sorted position: coalesced reads by body, one
+// scattered 16-byte write.
+__device__ __forceinline__ void place_leaf(const TreeBufs &tb, const LeafPlace &lpl, int b) {
+    const float2 p = lpl.pos[b];
+    tb.leaf[lpl.rank[b]] = make_float4(p.x, p.y, lpl.mass[b], 0.0f);
+}
+
 __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
-    const uint32_t *__restrict__ keys, int64_t n, TreeBufs tb, uint32_t *__restrict__ clear,
-    int64_t clear_n, uint32_t *__restrict__ slots) {
+    const uint32_t *__restrict__ keys, int64_t n, TreeBufs tb, LeafPlace lpl,
+    uint32_t *__restrict__ clear, int64_t clear_n, uint32_t *__restrict__ slots) {
     __shared__ uint32_t s_keys[kBuildWindow];
     clear_words(clear, clear_n, slots);
     const int n32 = (int)n;
@@ -271,6 +279,8 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
     for (int r = 0; r < kBuildPerThread; ++r) {
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
         if (i >= n32 - 1) return;
+        place_leaf(tb, lpl, i);
+        if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
         const Split sp = karras_node(delta, i, n32);
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
@@ -289,7 +299,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
 __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t *__restrict__ keys,
                                                         int64_t n,
                                                         const int64_t *__restrict__ sim_off,
-                                                        TreeBufs tb,
+                                                        TreeBufs tb, LeafPlace lpl,
                                                         uint32_t *__restrict__ sim_root,
                                                         uint32_t *__restrict__ clear,
                                                         int64_t clear_n,
@@ -301,6 +311,8 @@ __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t
     for (int r = 0; r < kBuildPerThread; ++r) {
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
         if (i >= n32 - 1) return;
+        place_leaf(tb, lpl, i);
+        if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
         const Split sp = karras_node(delta, i, n32);
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
@@ -651,19 +663,21 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
 
 }  // namespace
 
-void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, uint32_t *clear,
+void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, const LeafPlace &lpl,
+                      uint32_t *clear,
                       int64_t clear_n, uint32_t *slots, hipStream_t s) {
     if (n < 2) return;
     build_u32_kernel<<<(unsigned)((n - 1 + kBuildNodes - 1) / kBuildNodes), kBuildThreads, 0,
-                       s>>>(keys, n, t, clear, clear_n, slots);
+                       s>>>(keys, n, t, lpl, clear, clear_n, slots);
 }
 
 void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
+                      const LeafPlace &lpl,
                       uint32_t *sim_root, uint32_t *clear, int64_t clear_n, uint32_t *slots,
                       hipStream_t s) {
     if (n < 2) return;
     build_u64_kernel<<<(unsigned)((n - 1 + kBuildNodes - 1) / kBuildNodes), kBuildThreads, 0,
-                       s>>>(keys, n, sim_off, t, sim_root, clear, clear_n, slots);
+                       s>>>(keys, n, sim_off, t, lpl, sim_root, clear, clear_n, slots);
 }
 
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,

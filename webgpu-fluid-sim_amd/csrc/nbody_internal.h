@@ -79,12 +79,10 @@ void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
 // (8, 10 or 11) per pass; the digit histograms must be in sc.hist.
 // Ping-pongs between buffers a and b and returns which buffer (0 = a, 1 = b)
 // holds the result.
-// The last pass also writes the sorted leaf records leaf[t] = {pos, mass, 0}
-// of body vals[t] when `leaf` is set (its gathers overlap the ranking).
+// The last pass also writes each body's sorted position, rank[vals[t]] = t,
+// when `rank` is set (the build then places the leaf records).
 struct LeafGather {
-    const float2 *pos;
-    const float *mass;
-    float4 *leaf;
+    uint32_t *rank;
 };
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
@@ -132,14 +130,21 @@ struct TreeBufs {
     uint32_t *lists;     // the fill's top node list (fill_list_words)
     uint32_t *slot;      // [n-1] position of a top node in the top list
 };
-// The builds also zero `clear_words` words at `clear` (the sort's scratch,
-// for the next step's encode and sort) and the kBBoxSlots x 4 words at
-// `slots` (the walk's bounding-box accumulators; may be null).
-void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, uint32_t *clear,
-                      int64_t clear_words, uint32_t *slots, hipStream_t s);
+// The builds also write the sorted leaf records t.leaf[rank[b]] = {pos[b],
+// mass[b], 0} (body order in, scattered out), zero `clear_words` words at
+// `clear` (the sort's scratch, for the next step's encode and sort) and the
+// kBBoxSlots x 4 words at `slots` (the walk's bounding-box accumulators; may
+// be null).
+struct LeafPlace {
+    const float2 *pos;
+    const float *mass;
+    const uint32_t *rank;
+};
+void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, const LeafPlace &lp,
+                      uint32_t *clear, int64_t clear_words, uint32_t *slots, hipStream_t s);
 void launch_build_u64(const uint64_t *keys, int64_t n, const int64_t *sim_off, const TreeBufs &t,
-                      uint32_t *sim_root, uint32_t *clear, int64_t clear_words, uint32_t *slots,
-                      hipStream_t s);
+                      const LeafPlace &lp, uint32_t *sim_root, uint32_t *clear,
+                      int64_t clear_words, uint32_t *slots, hipStream_t s);
 void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int64_t n,
                  const TreeBufs &t, bool climb, hipStream_t s);
 int64_t fill_count_words(int64_t cap);  // sizes of counts / top list for `cap` bodies
