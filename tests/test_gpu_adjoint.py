@@ -64,10 +64,12 @@ def test_adjoint_matches_frozen_fp64_autograd(math):
 
 
 def test_adjoint_lane_bodies_agree():
-    """Bodies per lane change only the order of the node-cotangent atomics."""
+    """EXACT mode (the register-stack VJP walk): bodies per lane change only
+    the order of the node-cotangent atomics."""
     n, steps, every = 5000, 6, 3
     mass, pos, vel = nbody_sim.generate("plummer", n, seed=9)
     cfg, _ = params(theta=0.6, dt=1e-4, eps=1e-3)
+    cfg = cfg.with_(mathMode="exact")
     rng = np.random.default_rng(1)
     wx = rng.normal(size=(n, 2)).astype(np.float32)
     wv = rng.normal(size=(n, 2)).astype(np.float32)
@@ -77,6 +79,26 @@ def test_adjoint_lane_bodies_agree():
         assert np.array_equal(pT, pT2)
         for a, b in zip(got, ref):
             assert rel(a, b) < 1e-5, lb
+
+
+def test_adjoint_fast_matches_exact_many_nodes():
+    """FAST mode (the pair-walk VJP with per-workgroup cotangent tables) at a
+    small theta, where a workgroup's wavefronts accept more distinct nodes
+    than its table holds (the rest go straight to memory): the gradients
+    agree with EXACT mode's register-stack VJP walk up to the two modes'
+    arithmetic and their few theta-boundary decisions."""
+    n, steps, every = 20000, 2, 2
+    mass, pos, vel = nbody_sim.generate("exp-disk", n, seed=11, dispersion=0.05, mass_sigma=0.5)
+    cfg, _ = params(theta=0.3, dt=1e-4, eps=1e-3)
+    rng = np.random.default_rng(2)
+    wx = rng.normal(size=(n, 2)).astype(np.float32)
+    wv = rng.normal(size=(n, 2)).astype(np.float32)
+    fast, _, _ = gpu_grads(cfg.with_(mathMode="fast"), mass, pos, vel, steps, every, wx, wv)
+    exact, _, _ = gpu_grads(cfg.with_(mathMode="exact"), mass, pos, vel, steps, every, wx, wv)
+    errs = [rel(a, b) for a, b in zip(fast, exact)]
+    print("fast vs exact adjoint (rel L2) x, v, m:", errs)
+    assert all(np.all(np.isfinite(a)) for a in fast)
+    assert errs[0] < 1e-3 and errs[1] < 1e-5 and errs[2] < 1e-3
 
 
 def test_adjoint_central_differences_64_coordinates():
