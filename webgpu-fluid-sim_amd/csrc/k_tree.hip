@@ -501,6 +501,7 @@ __device__ __forceinline__ void climb_wg(const TreeBufs &tb, uint32_t nint, uint
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 struct ListLds {
     uint32_t node[kLdsList], pw[kLdsList], par[kLdsList], pend[kLdsList], cnt[kLdsList];
+    uint32_t start[kLdsList], nstart;  // the entries with no child among the entries
     float4 in[kLdsList][4];
     float4 out[kLdsList][2];  // the entry's own {cm, aabb} once complete
 };
@@ -516,9 +517,16 @@ struct ListLds {
 // one entry child), so the nodes are published to memory only afterwards,
 // by all threads together.
 __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, uint32_t m) {
-    const uint32_t e = threadIdx.x;
-    if (e < m && ls.pend[e] == 0u) {
-        uint32_t cur = e;
+    // The climbs start from the entries without entry children, gathered into
+    // the first threads: the few wavefronts that climb (one for the top list)
+    // then share their CU with no other climbing wavefront.
+    if (threadIdx.x == 0) ls.nstart = 0u;
+    __syncthreads();
+    if (threadIdx.x < m && ls.pend[threadIdx.x] == 0u)
+        ls.start[atomicAdd(&ls.nstart, 1u)] = threadIdx.x;
+    __syncthreads();
+    if (threadIdx.x < ls.nstart) {
+        uint32_t cur = ls.start[threadIdx.x];
         float4 L = ls.in[cur][0], LB = ls.in[cur][1], R = ls.in[cur][2], RB = ls.in[cur][3];
         uint32_t pw = ls.pw[cur], q = ls.par[cur];
         for (int level = 0; level < 128; ++level) {
