@@ -388,12 +388,15 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
     __shared__ float4 s_cm[2 * kFillBlock];
     __shared__ float4 s_bb[kFillBlock];
     __shared__ uint32_t s_arrive[kFillBlock];  // arrivals; 2 once the node is complete
+    __shared__ uint32_t s_start[kFillBlock];   // the climbs' start nodes, compacted
+    __shared__ uint32_t s_nstart;
 
     const uint32_t j = threadIdx.x;
     const uint32_t base = blockIdx.x * kFillBlock;
     const uint32_t nint = (uint32_t)(n - 1);  // internal nodes
     const uint32_t t = base + j;
     s_arrive[j] = 0u;
+    if (j == 0) s_nstart = 0u;
     uint2 ch = make_uint2(0u, 0u), rg = make_uint2(0u, 0u);
     if (t < nint) {
         ch = tb.child[t];
@@ -416,13 +419,20 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
     // climb's, so the two meet at an arrival count and the second one
     // climbs on.  A level is one read of the parent's child pair and
     // parent word, then of the sibling (after the count for an internal
-    // sibling), and the merge; a parent word without kInBlock ends it.
-    if (t < nint && ch.x >= nint && ch.y >= nint && !spans(rg, kBlockShift)) {
-        const float4 L = s_cm[kFillBlock + (ch.x - nint - base)];
-        const float4 R = s_cm[kFillBlock + (ch.y - nint - base)];
+    // sibling), and the merge; a parent word without kInBlock ends it.  The
+    // start nodes are gathered into the first threads, so only the few
+    // wavefronts that climb are active.
+    if (t < nint && ch.x >= nint && ch.y >= nint && !spans(rg, kBlockShift))
+        s_start[atomicAdd(&s_nstart, 1u)] = j;
+    __syncthreads();
+    if (j < s_nstart) {
+        const uint32_t c0 = s_start[j];
+        const uint2 cch = s_child[c0];
+        const float4 L = s_cm[kFillBlock + (cch.x - nint - base)];
+        const float4 R = s_cm[kFillBlock + (cch.y - nint - base)];
         float4 C, B;
         merge_values(L, make_float4(L.x, L.y, L.x, L.y), R, make_float4(R.x, R.y, R.x, R.y), C, B);
-        uint32_t cur = j, pw = s_par_int[j];
+        uint32_t cur = c0, pw = s_par_int[c0];
         for (int level = 0; level < 128; ++level) {
             s_cm[cur] = C;
             s_bb[cur] = B;
