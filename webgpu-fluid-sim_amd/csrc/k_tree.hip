@@ -42,10 +42,10 @@ namespace {
 // searches of a node probe its neighbourhood first, so almost every probe
 // is an LDS read: only nodes with wide leaf ranges go to memory, for their
 // first (farthest) probes.
-constexpr int kBuildThreads = 256;
+constexpr int kBuildThreads = 512;
 constexpr int kBuildPerThread = 1;  // nodes per thread (2: all wavefronts resident at once, measured slower)
 constexpr int kBuildNodes = kBuildThreads * kBuildPerThread;  // nodes per workgroup
-constexpr int kBuildHalo = 1024;
+constexpr int kBuildHalo = 512;
 constexpr int kBuildWindow = kBuildNodes + 2 * kBuildHalo;
 // A global-memory read the compiler cannot merge with another load; it
 // waits for its data in the same asm block.
