@@ -137,13 +137,18 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     for (int it = 0; it < kItems; ++it) {
         const uint32_t d = dg[it];
         const bool ok = d < (uint32_t)D;
-        uint64_t peers = __ballot(ok);
+        const uint64_t valid = __ballot(ok);
+        uint32_t plo = (uint32_t)valid, phi = (uint32_t)(valid >> 32);
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
+            // m = all ones where the lane's bit b is set; a lane keeps the
+            // lanes whose bit agrees with its own: ~(ballot ^ m).
+            const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+            const uint64_t bb = __ballot(m != 0u);
+            plo &= ~((uint32_t)bb ^ m);
+            phi &= ~((uint32_t)(bb >> 32) ^ m);
         }
+        const uint64_t peers = ((uint64_t)phi << 32) | plo;
         const uint32_t below = lanes_below(peers);
         const uint32_t cnt = (uint32_t)__popcll(peers);
         uint32_t before = 0;
