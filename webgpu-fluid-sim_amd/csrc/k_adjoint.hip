@@ -245,7 +245,9 @@ __global__ __launch_bounds__(256) void adjoint_pair_kernel(const AdjointArgs a) 
                                     uniform_f32(pcm_s[rb + 6]), uniform_f32(pcm_s[rb + 7]));
             uint2 ch = make_uint2(uniform_u32(pcmu_s[rb + 8]), uniform_u32(pcmu_s[rb + 9]));
             uint32_t n_over = 0;
+            PairPad pad = {};
             for (;;) {
+                keep_live(pad);  // one 64-byte load per record
                 float rxl, ryl, invl, rxr, ryr, invr;
                 const uint64_t bl = __ballot(probe(cl, rxl, ryl, invl));
                 const uint64_t br = __ballot(probe(cr, rxr, ryr, invr));
@@ -275,6 +277,7 @@ __global__ __launch_bounds__(256) void adjoint_pair_kernel(const AdjointArgs a) 
                     cl = make_float4(rf[0], rf[1], rf[2], rf[3]);
                     cr = make_float4(rf[4], rf[5], rf[6], rf[7]);
                     ch = make_uint2(ru[8], ru[9]);
+                    load_pad(pad, ru);
                 }
                 if (accl | accr) {
                     // both children's terms and their six wavefront sums together

@@ -541,17 +541,14 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
         uint32_t pw = ls.pw[cur], q = ls.par[cur];
         for (int level = 0; level < 128; ++level) {
             // the parent's fixed words and its other input are read before
-            // the merge, which hides their latency
+            // the merge, which hides their latency; they are read
+            // unconditionally (entry 0 stands in above the list's last
+            // entry, and is not used), so no test waits for them
             const uint32_t side = pw >> 31;
-            uint32_t qpw = 0u, qq = kNone, qpend = 0u;
-            float4 S = make_float4(0.0f, 0.0f, 0.0f, 0.0f), SB = S;
-            if (q != kNone) {
-                qpw = ls.pw[q];
-                qq = ls.par[q];
-                qpend = ls.pend[q];
-                S = ls.in[q][2 * (side ^ 1u)];
-                SB = ls.in[q][2 * (side ^ 1u) + 1];
-            }
+            const uint32_t qs = q == kNone ? 0u : q;
+            const uint32_t qpw = ls.pw[qs], qq = ls.par[qs], qpend = ls.pend[qs];
+            float4 S = ls.in[qs][2 * (side ^ 1u)];
+            float4 SB = ls.in[qs][2 * (side ^ 1u) + 1];
             float4 C, B;
             merge_values(L, LB, R, RB, C, B);
             ls.out[cur][0] = C;

@@ -591,7 +591,9 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
         float4 cr = make_float4(uniform_f32(pcm_s[rb + 4]), uniform_f32(pcm_s[rb + 5]),
                                 uniform_f32(pcm_s[rb + 6]), uniform_f32(pcm_s[rb + 7]));
         uint2 ch = make_uint2(uniform_u32(pcmu_s[rb + 8]), uniform_u32(pcmu_s[rb + 9]));
+        PairPad pad = {};
         for (;;) {
+            keep_live(pad);  // one 64-byte load per record
             float rxl, ryl, invl, rxr, ryr, invr;
             const uint64_t bl = __ballot(probe(cl, rxl, ryl, invl));
             const uint64_t br = __ballot(probe(cr, rxr, ryr, invr));
@@ -640,6 +642,7 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
                 cl = make_float4(rf[0], rf[1], rf[2], rf[3]);
                 cr = make_float4(rf[4], rf[5], rf[6], rf[7]);
                 ch = make_uint2(ru[8], ru[9]);
+                load_pad(pad, ru);
             }
             // keep the pulls behind the loads in the schedule
             __builtin_amdgcn_sched_barrier(0);
