@@ -24,9 +24,11 @@
 //       finds, among its node indices, the nodes whose range lies in the
 //       super-block across a block boundary, and completes them by LDS
 //       climbs; it also lists the top nodes (range across super-blocks).
-//     phase 3 (the last fill_mid workgroup to finish): completes the top.
-//   A kernel boundary orders phases 1 and 2; one device-scope fence per
-//   phase-2 workgroup orders phases 2 and 3.
+//     phase 3 (one more fill_mid workgroup, at the same time): completes the
+//       top nodes, each climb waiting only for the super-blocks whose
+//       phase-2 records it merges.
+//   A kernel boundary orders phases 1 and 2; a device-scope fence and a
+//   done flag per phase-2 workgroup order phases 2 and 3.
 //   The merge formula does not depend on which child arrives last, so the
 //   result is deterministic and equals a sequential post-order fill.  The
 //   reference's random leaf rotation (`offset`, lbvh.wgsl:171) is therefore
@@ -161,11 +163,13 @@ __host__ __device__ inline uint32_t num_supers(int64_t n) {
     return (uint32_t)((n + kSuper - 1) / kSuper);
 }
 constexpr uint32_t kTopPer = 256;
-// tb.counts: [0] phase-2 workgroups done; the length of list l (phase-2
-// list of super-block l < S, the top list l = S) at list_count(l), one
-// 128-byte line each, as the builds' atomics on one line would serialize.
+// tb.counts: the length of list l (phase-2 list of super-block l < S, the
+// top list l = S) at list_count(l), one 128-byte line each, as the builds'
+// atomics on one line would serialize; the next word of super-block l's
+// line is its done flag (its phase-2 nodes are published).
 // tb.lists: the phase-2 lists (kSuper words each), then the top list.
 __host__ __device__ inline uint32_t list_count(uint32_t l) { return 32u * (l + 1u); }
+__host__ __device__ inline uint32_t done_flag(uint32_t sb) { return list_count(sb) + 1u; }
 
 __device__ __forceinline__ bool spans(uint2 r, int shift) { return (r.x >> shift) != (r.y >> shift); }
 
@@ -507,14 +511,40 @@ __device__ __forceinline__ void climb_wg(const TreeBufs &tb, uint32_t nint, uint
 // One upper-level node list in LDS: entry e is node node[e] with parent
 // word pw[e], its parent's entry par[e] (kNone: the parent is not in the
 // list), the number of its children that are entries (pend), an arrival
-// count, and its children's {cm, aabb} records (left, right).
+// count, and its children's {cm, aabb} records (left, right).  src[e]
+// (top list): per side (bits 0-15 left, 16-31 right), 1 + the super-block
+// whose phase-2 workgroup produces that child's record, or 0.
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 struct ListLds {
     uint32_t node[kLdsList], pw[kLdsList], par[kLdsList], pend[kLdsList], cnt[kLdsList];
+    uint32_t src[kLdsList];
     uint32_t start[kLdsList], nstart;  // the entries with no child among the entries
     float4 in[kLdsList][4];
     float4 out[kLdsList][2];  // the entry's own {cm, aabb} once complete
 };
+
+// A float4 read at device scope (past this XCD's L2, which may hold the
+// line from before another XCD's write).
+__device__ __forceinline__ float4 load_device_f4(const float4 *p) {
+    const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+    const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)),
+                       __uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
+}
+
+// The record of child `side` of top node `node` when a phase-2 workgroup
+// produces it: wait for that super-block's done flag (sb1 = 1 + the
+// super-block), then read the record at device scope.
+__device__ __forceinline__ void wait_input(const TreeBufs &tb, uint32_t sb1, uint32_t node,
+                                           uint32_t side, float4 &V, float4 &VB) {
+    const uint32_t *flag = tb.counts + done_flag(sb1 - 1u);
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+        __builtin_amdgcn_s_sleep(2);
+    asm volatile("" ::: "memory");  // the record reads stay behind the flag
+    V = load_device_f4(tb.pcm + kPairStride * (size_t)node + side);
+    VB = load_device_f4(tb.paabb + 2 * (size_t)node + side);
+}
 
 // Resolve a list set up in ls (m <= kLdsList entries; called by the whole
 // workgroup).  The entries with no child among the entries start a climb in
@@ -525,7 +555,9 @@ struct ListLds {
 // (workgroup-scope acquire/release), and the second one goes on.  The
 // chains of the upper levels are long (tens of levels, mostly parents with
 // one entry child), so the nodes are published to memory only afterwards,
-// by all threads together.
+// by all threads together.  kTop: the top list, whose inputs from the
+// phase-2 workgroups (src) are read when their super-block is done.
+template <bool kTop>
 __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, uint32_t m) {
     // The climbs start from the entries without entry children, gathered into
     // the first threads: the few wavefronts that climb (one for the top list)
@@ -535,10 +567,57 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
     if (threadIdx.x < m && ls.pend[threadIdx.x] == 0u)
         ls.start[atomicAdd(&ls.nstart, 1u)] = threadIdx.x;
     __syncthreads();
-    if (threadIdx.x < ls.nstart) {
+    const uint32_t nstart = ls.nstart;
+    // Top list: the last wavefront loads the phase-2 inputs into LDS as their
+    // super-blocks finish and then clears the entry's src (release), so a
+    // climb waits on LDS; when the climbs need that wavefront too, each climb
+    // reads its own phase-2 inputs.  A wavefront's LDS operations execute in
+    // order, so a climb that reads src as 0 and then an input reads it
+    // complete.
+    constexpr uint32_t kLoader = kFillBlock - 64;
+    const bool loaders = kTop && nstart <= kLoader;
+    if (kTop && loaders && threadIdx.x >= kLoader) {
+        for (uint32_t e = threadIdx.x - kLoader; e < m; e += 64) {
+            const uint32_t sc = ls.src[e];
+            if (!sc) continue;
+#pragma unroll
+            for (uint32_t side = 0; side < 2; ++side) {
+                const uint32_t sb1 = (sc >> (16u * side)) & 0xFFFFu;
+                if (!sb1) continue;
+                float4 V, VB;
+                wait_input(tb, sb1, ls.node[e], side, V, VB);
+                ls.in[e][2 * side] = V;
+                ls.in[e][2 * side + 1] = VB;
+            }
+            __hip_atomic_store(&ls.src[e], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    // input `side` of entry e, produced by a phase-2 workgroup
+    auto mid_input = [&](uint32_t e, uint32_t side, float4 &V, float4 &VB) {
+        if (loaders) {
+            while ((__hip_atomic_load(&ls.src[e], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >>
+                    (16u * side)) &
+                   0xFFFFu)
+                __builtin_amdgcn_s_sleep(1);
+            V = ls.in[e][2 * side];
+            VB = ls.in[e][2 * side + 1];
+        } else {
+            wait_input(tb, (ls.src[e] >> (16u * side)) & 0xFFFFu, ls.node[e], side, V, VB);
+        }
+    };
+    if (threadIdx.x < nstart) {
         uint32_t cur = ls.start[threadIdx.x];
+        uint32_t sc = 0u;
+        if constexpr (kTop) {
+            sc = __hip_atomic_load(&ls.src[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            asm volatile("" ::: "memory");  // src before the inputs
+        }
         float4 L = ls.in[cur][0], LB = ls.in[cur][1], R = ls.in[cur][2], RB = ls.in[cur][3];
         uint32_t pw = ls.pw[cur], q = ls.par[cur];
+        if constexpr (kTop) {
+            if (sc & 0xFFFFu) mid_input(cur, 0u, L, LB);
+            if (sc >> 16) mid_input(cur, 1u, R, RB);
+        }
         for (int level = 0; level < 128; ++level) {
             // the parent's fixed words and its other input are read before
             // the merge, which hides their latency; they are read
@@ -546,6 +625,13 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
             // entry, and is not used), so no test waits for them
             const uint32_t side = pw >> 31;
             const uint32_t qs = q == kNone ? 0u : q;
+            uint32_t sq = 0u;
+            if constexpr (kTop) {
+                sq = (__hip_atomic_load(&ls.src[qs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >>
+                      (16u * (side ^ 1u))) &
+                     0xFFFFu;
+                asm volatile("" ::: "memory");  // src before the input
+            }
             const uint32_t qpw = ls.pw[qs], qq = ls.par[qs], qpend = ls.pend[qs];
             float4 S = ls.in[qs][2 * (side ^ 1u)];
             float4 SB = ls.in[qs][2 * (side ^ 1u) + 1];
@@ -554,6 +640,9 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
             ls.out[cur][0] = C;
             ls.out[cur][1] = B;
             if (q == kNone) break;
+            if constexpr (kTop) {
+                if (sq) mid_input(q, side ^ 1u, S, SB);
+            }
             if (qpend == 2u) {
                 // both children are entries: the one that arrives second
                 // climbs on, with the other's record read after the count
@@ -601,65 +690,93 @@ __device__ __forceinline__ void setup_entry(const TreeBufs &tb, ListLds &ls,
     // with it unless it spans super-blocks
     if (par < nint && (shift == kSuperShift || !spans(tb.range[par], kSuperShift)))
         q = tb.slot[par];
-    const uint32_t pend = (ch.x < nint && spans(tb.range[ch.x], shift) ? 1u : 0u) +
-                          (ch.y < nint && spans(tb.range[ch.y], shift) ? 1u : 0u);
+    uint32_t pend = 0u, src = 0u;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t c = side ? ch.y : ch.x;
+        if (c >= nint) continue;
+        const uint2 rc = tb.range[c];
+        if (spans(rc, shift))
+            ++pend;
+        else if (shift == kSuperShift && spans(rc, kBlockShift))
+            src |= ((rc.x >> kSuperShift) + 1u) << (16 * side);  // a phase-2 node
+    }
     ls.node[t] = p;
     ls.pw[t] = pw;
     ls.par[t] = q;
     ls.pend[t] = pend;
+    ls.src[t] = src;
     ls.cnt[t] = 0u;
 }
 
-// Phases 2 and 3.  Workgroup sb resolves the phase-2 list of super-block sb
-// in LDS (or, longer than lds_limit, climbs from the phase-2 nodes' other
-// children).  The workgroup that finishes last (a device-scope fence orders
-// each workgroup's writes before its count) resolves the top list the same
-// way (or climbs from the top nodes' other children) and clears the
-// counters for the next build.
+// Phases 2 and 3, S + 1 workgroups.  Workgroup sb < S resolves the phase-2
+// list of super-block sb in LDS (or, longer than lds_limit, climbs from the
+// phase-2 nodes' other children), then, behind a device-scope fence, sets
+// its done flag.  Workgroup S resolves the top list at the same time: a
+// climb that needs a record from a phase-2 workgroup waits for that
+// super-block's flag and reads the record at device scope, so the top
+// levels resolve while the lower super-blocks still run (the deep chains of
+// a dense region are not on one path with every super-block).  A top list
+// longer than lds_limit waits for every flag and climbs from the top nodes'
+// other children.  Workgroup S, the last one dispatched, is the only one
+// that waits, so every flag it waits for is set.  It then clears the
+// counters and flags for the next build.
 __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBufs tb,
                                                               uint32_t num_super,
                                                               uint32_t lds_limit) {
     __shared__ ListLds ls;
-    __shared__ uint32_t s_last;
     const uint32_t t = threadIdx.x;
     const uint32_t sb = blockIdx.x;
     const uint32_t nint = (uint32_t)(n - 1);
     const uint32_t S = num_super;
-    const uint32_t *list2 = tb.lists + (size_t)sb * kSuper;
-    const uint32_t m2 = tb.counts[list_count(sb)];
-    if (m2 <= lds_limit) {
-        if (t < m2) setup_entry(tb, ls, list2, t, nint, kBlockShift);
-        __syncthreads();
-        resolve_list(tb, ls, m2);
-    } else {
-        // climb from every child of a phase-2 node that is not one itself
-        const uint32_t lo = sb * (uint32_t)kSuper;
-        const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
-        for (uint32_t k = t; k < m2; k += kFillBlock) {
-            const uint2 ch = tb.child[list2[k]];
+    if (sb < S) {
+        const uint32_t *list2 = tb.lists + (size_t)sb * kSuper;
+        const uint32_t m2 = tb.counts[list_count(sb)];
+        if (m2 <= lds_limit) {
+            if (t < m2) setup_entry(tb, ls, list2, t, nint, kBlockShift);
+            __syncthreads();
+            resolve_list<false>(tb, ls, m2);
+        } else {
+            // climb from every child of a phase-2 node that is not one itself
+            const uint32_t lo = sb * (uint32_t)kSuper;
+            const uint32_t hi = lo + kSuper - 1 < nint ? lo + kSuper - 1 : nint;
+            for (uint32_t k = t; k < m2; k += kFillBlock) {
+                const uint2 ch = tb.child[list2[k]];
 #pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t c = side ? ch.y : ch.x;
-                if (!(c < nint && spans(tb.range[c], kBlockShift))) climb_wg(tb, nint, c, lo, hi);
+                for (int side = 0; side < 2; ++side) {
+                    const uint32_t c = side ? ch.y : ch.x;
+                    if (!(c < nint && spans(tb.range[c], kBlockShift))) climb_wg(tb, nint, c, lo, hi);
+                }
             }
         }
+        // every thread's stores have completed (reached L2) before the
+        // fence writes the L2 back
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t == 0) {
+            __threadfence();
+            __hip_atomic_store(tb.counts + done_flag(sb), 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
     }
-
-    __syncthreads();
-    if (t == 0) {
-        __threadfence();
-        s_last = atomicAdd(&tb.counts[0], 1u) == S - 1u;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
     const uint32_t *top = tb.lists + (size_t)S * kSuper;
     const uint32_t m3 = tb.counts[list_count(S)];
+    auto wait_all = [&]() {
+        for (uint32_t k = t; k < S; k += kFillBlock)
+            while (__hip_atomic_load(tb.counts + done_flag(k), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) == 0u)
+                __builtin_amdgcn_s_sleep(2);
+        __syncthreads();
+    };
     if (m3 <= lds_limit) {
         if (t < m3) setup_entry(tb, ls, top, t, nint, kSuperShift);
         __syncthreads();
-        resolve_list(tb, ls, m3);
+        resolve_list<true>(tb, ls, m3);
+        wait_all();
     } else {
+        wait_all();
+        __threadfence();  // acquire: the phase-2 records, past this XCD's L2
         // climb from every child of a top node that is not a top node
         for (uint32_t k = t; k < m3; k += kFillBlock) {
             const uint2 ch = tb.child[top[k]];
@@ -671,9 +788,11 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
         }
     }
     __syncthreads();
-    // the next build counts from zero
-    for (uint32_t k = t; k <= S; k += kFillBlock) tb.counts[list_count(k)] = 0u;
-    if (t == 0) tb.counts[0] = 0u;
+    // the next build counts from zero; the next fill waits on fresh flags
+    for (uint32_t k = t; k <= S; k += kFillBlock) {
+        tb.counts[list_count(k)] = 0u;
+        if (k < S) tb.counts[done_flag(k)] = 0u;
+    }
 }
 
 }  // namespace
@@ -702,8 +821,8 @@ void launch_fill(const uint32_t *idx, const float *mass, const float2 *pos, int6
     fill_local_kernel<<<blocks, kFillBlock, 0, s>>>(idx, mass, pos, n, t);
     if (n > kFillBlock) {
         const uint32_t num_super = num_supers(n);
-        fill_mid_kernel<<<num_super, kFillBlock, 0, s>>>(n, t, num_super,
-                                                         climb ? 0u : kLdsList);
+        fill_mid_kernel<<<num_super + 1, kFillBlock, 0, s>>>(n, t, num_super,
+                                                             climb ? 0u : kLdsList);
     }
 }
 
