@@ -109,11 +109,15 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // The bounding box held by the walk's accumulators (all 64 lanes of the
 // wave take part; kBBoxSlots / 64 slots per lane).
 __device__ __forceinline__ float4 bbox_from_slots(const uint32_t *__restrict__ slots) {
+    static_assert(kBBoxSlots % 64 == 0, "slots per lane");
+    uint4 u[kBBoxSlots / 64];  // all loads in flight before the first max
+#pragma unroll
+    for (int j = 0; j < kBBoxSlots / 64; ++j)
+        u[j] = reinterpret_cast<const uint4 *>(slots)[j * 64 + (threadIdx.x & 63)];
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    for (int k = threadIdx.x & 63; k < kBBoxSlots; k += 64) {
-        const uint4 u = reinterpret_cast<const uint4 *>(slots)[k];
-        v = make_uint4(max(v.x, u.x), max(v.y, u.y), max(v.z, u.z), max(v.w, u.w));
-    }
+#pragma unroll
+    for (int j = 0; j < kBBoxSlots / 64; ++j)
+        v = make_uint4(max(v.x, u[j].x), max(v.y, u[j].y), max(v.z, u[j].z), max(v.w, u[j].w));
     const uint32_t mnx = wave_max_u32(v.x), mny = wave_max_u32(v.y);
     const uint32_t mxx = wave_max_u32(v.z), mxy = wave_max_u32(v.w);
     // an empty accumulator reads as the identity box (+inf, -inf)
@@ -159,7 +163,7 @@ __device__ __forceinline__ uint32_t code_bbox30(float2 p, float4 b) {
     return part1by1(x) | (part1by1(y) << 1);
 }
 
-// Keys of bodies [blockIdx.x * kEncodeTile, +kEncodeTile), 16 per thread
+// Keys of bodies [blockIdx.x * kEncodeTile, +kEncodeTile), 4 per thread
 // (item it of thread t is body base + it * 1024 + t: coalesced), and the
 // digit counts of every sort pass for them: LDS histograms, then one global
 // atomic per non-empty bin per workgroup (large workgroups keep those few).
@@ -180,6 +184,16 @@ __global__ __launch_bounds__(kEncodeThreads) void encode_kernel(const float2 *__
     constexpr uint32_t kMask = (uint32_t)D - 1u;
     __shared__ uint32_t sh[kSortMaxPasses * D];
     const int bins = passes * D;
+    constexpr int kItems = kEncodeTile / kEncodeThreads;
+    const int64_t base = (int64_t)blockIdx.x * kEncodeTile;
+    // the positions are loaded first, so their latency overlaps the box's
+    // (the slots come from the walk's device-scope atomics, past the L2)
+    float2 p[kItems];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t i = base + it * kEncodeThreads + threadIdx.x;
+        p[it] = i < n ? pos[i] : make_float2(0.0f, 0.0f);
+    }
     for (int i = threadIdx.x; i < bins; i += kEncodeThreads) sh[i] = 0u;
     float4 box = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if constexpr (kMode == 1 && !kEnsemble) {
@@ -191,22 +205,20 @@ __global__ __launch_bounds__(kEncodeThreads) void encode_kernel(const float2 *__
         }
     }
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kEncodeTile;
-#pragma unroll 4
-    for (int it = 0; it < kEncodeTile / kEncodeThreads; ++it) {
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
         const int64_t i = base + it * kEncodeThreads + threadIdx.x;
         if (i >= n) break;
-        const float2 p = pos[i];
         uint64_t key;
         if constexpr (kEnsemble) {
             const uint32_t s = body_sim[i];
-            key = ((uint64_t)s << kSimShift) | code_bbox30(p, bbox[s]);
+            key = ((uint64_t)s << kSimShift) | code_bbox30(p[it], bbox[s]);
             keys64[i] = key;
         } else if constexpr (kMode == 0) {
-            key = code_reference(p);
+            key = code_reference(p[it]);
             keys32[i] = (uint32_t)key;
         } else {
-            key = code_bbox30(p, box);
+            key = code_bbox30(p[it], box);
             keys32[i] = (uint32_t)key;
         }
         idx[i] = (uint32_t)i;
