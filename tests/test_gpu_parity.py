@@ -143,6 +143,21 @@ def test_fill_upper_levels(climb):
     assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
 
 
+def test_step_bit_exact_800k():
+    """The bench's 800k-body exponential disk (25 super-blocks, a 27-level
+    top chain resolved beside the phase-2 workgroups): keys, permutation,
+    tree, accelerations and the new state equal the oracle bit for bit on the
+    second step (the first step's fill reset the counters and flags)."""
+    mass, pos, vel = nbody_sim.generate("exp-disk", 800_000, seed=1, scale=0.2, dispersion=0.05,
+                                        mass_sigma=0.5)
+    cfg, P = params(theta=0.7, dt=5e-5, eps=1e-3)
+    g = gpu_step(cfg, mass, pos, vel, nsteps=2)
+    r = oracle_step(P, mass, pos, vel, nsteps=2)
+    _compare_tree(g, r)
+    assert np.array_equal(g["accel"], r["accel"])
+    assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
+
+
 def test_coincident_bodies():
     n = 500
     pos = np.zeros((n, 2), np.float32)
