@@ -97,27 +97,14 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     for (int i = tid; i < kWaves * D; i += kThreads) s_wcnt[i] = 0;
     const bool owner = tid < kOwners;
-    // where each owned digit starts in the output (exclusive digit scan)
-    uint32_t dbase[DPT];
-    {
-        uint32_t c[DPT], sum = 0;
-#pragma unroll
-        for (int j = 0; j < DPT; ++j) {
-            c[j] = owner ? digit_count[tid * DPT + j] : 0u;
-            sum += c[j];
-        }
-        uint32_t run = block_exclusive_scan<kThreads>(sum, s_shw);  // its barriers publish s_tile
-#pragma unroll
-        for (int j = 0; j < DPT; ++j) {
-            dbase[j] = run;
-            run += c[j];
-        }
-    }
+    __syncthreads();  // publishes s_tile
     const uint32_t tile = s_tile;
     const int64_t base = (int64_t)tile * kTile;
 
     // Wave-striped load: item it of lane l in wave w is tile position
-    // w*64*kItems + it*64 + l, so (w, it, l) order is input order.
+    // w*64*kItems + it*64 + l, so (w, it, l) order is input order.  The
+    // pass's digit counts are loaded with the keys: the barriers of their
+    // scan wait for both at once.
     K k[kItems];
     uint32_t v[kItems];
     uint32_t dg[kItems];
@@ -128,7 +115,27 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         const bool ok = p < n;
         k[it] = ok ? kin[p] : (K)0;
         v[it] = ok ? vin[p] : 0u;
-        dg[it] = ok ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
+    }
+    // where each owned digit starts in the output (exclusive digit scan)
+    uint32_t dbase[DPT];
+    {
+        uint32_t c[DPT], sum = 0;
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            c[j] = owner ? digit_count[tid * DPT + j] : 0u;
+            sum += c[j];
+        }
+        uint32_t run = block_exclusive_scan<kThreads>(sum, s_shw);
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            dbase[j] = run;
+            run += c[j];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t p = base + (int64_t)wid * (64 * kItems) + it * 64 + lane;
+        dg[it] = p < n ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
     }
 
     // Stable rank inside the wave: peers = lanes holding the same digit.
