@@ -24,9 +24,24 @@ namespace nbs {
 
 namespace {
 
+// Sum over the wavefront (all lanes active): DPP adds inside each row of 16
+// lanes (lane ^ 1, lane ^ 2, the other quad, the other half-row), then the
+// four row sums, ((r0 + r1) + (r2 + r3)).  A butterfly of __shfl_xor goes
+// through the LDS crossbar (ds_bpermute): six of them per sum.
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f32(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xF, 0xF, false));
+}
 __device__ __forceinline__ float wave_sum(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f32<0x141>(v);  // row_half_mirror
+    v += dpp_f32<0x140>(v);  // row_mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
 }
 
 __global__ __launch_bounds__(256) void adjoint_pre_kernel(const float2 *__restrict__ lx,
