@@ -108,6 +108,25 @@ int64_t nbody_get_num_bodies(const nbody_sim *sim);
  * with kick, drift) on the handle's stream.  Asynchronous. */
 int nbody_step(nbody_sim *sim, int32_t nsteps);
 
+/* Allocation-free step.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Enqueues `nsteps` plain
+ * steps, exactly as nbody_step would for a handle with this config, default
+ * tuning and the same bodies, of n <= NBODY_STATIC_MAX_BODIES bodies whose
+ * mass (f32[n]), pos and vel (f32[2n], x0,y0,x1,y1,...) are caller-owned
+ * DEVICE memory on the current HIP device; pos and vel are updated in place.
+ * Every intermediate buffer lives in one fixed 16 MiB device array per GPU
+ * (held from the library's load to the end of the process); the call makes
+ * no hipMalloc/hipFree, no memset node and no host synchronisation, so it
+ * can be captured into a HIP graph.  `hip_stream` is a hipStream_t of the
+ * current device (NULL: the null stream).  Calls on one GPU share its array:
+ * calls whose work overlaps on the device (other streams, other host
+ * threads' streams) race and must be ordered by the caller.  Larger n:
+ * NBODY_ERR_CAPACITY.  At these sizes the walk stack cannot overflow. */
+#define NBODY_STATIC_MAX_BODIES 65536
+int nbody_step_static(const nbody_config *cfg, int64_t n, const float *mass, float *pos_xy,
+                      float *vel_xy, int32_t nsteps, void *hip_stream);
+
 /* Wait for queued work; reports a walk stack overflow seen since the last call. */
 int nbody_synchronize(nbody_sim *sim);
 

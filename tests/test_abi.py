@@ -116,3 +116,14 @@ def test_null_handle_errors():
     assert L.nbody_step(None, 1) == _lib.ERR_INVALID_ARG
     assert L.nbody_get_num_bodies(None) == -1
     assert L.nbody_destroy(None) == 0
+
+
+def test_static_step_rejects_sizes_beyond_its_scratch():
+    """nbody_step_static refuses n above its fixed scratch before any device call."""
+    L = _lib.lib()
+    c = nbody_sim.Config().to_c()
+    rc = L.nbody_step_static(ctypes.byref(c), _lib.STATIC_MAX_BODIES + 1, None, None, None, 1, None)
+    assert rc == _lib.ERR_CAPACITY
+    assert str(_lib.STATIC_MAX_BODIES) in _lib.last_error(None)
+    assert L.nbody_step_static(ctypes.byref(c), -1, None, None, None, 1, None) == _lib.ERR_INVALID_ARG
+    assert L.nbody_step_static(ctypes.byref(c), 0, None, None, None, 1, None) == _lib.OK

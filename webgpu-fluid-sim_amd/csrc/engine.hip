@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -171,6 +172,46 @@ void release_graph(nbody_sim *s) {
     s->graph_dirty = true;
 }
 
+// Carve every buffer for `cap` bodies out of `a` (a null base: sizes only).
+// external_bodies: mass, pos and vel belong to the caller (not laid out).
+void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
+    const size_t key_size = s->ensemble ? 8 : 4;
+    const int64_t nodes = 2 * cap - 1;
+    a.used = 0;
+    if (!external_bodies) {
+        s->mass = a.take<float>(cap);
+        s->pos = a.take<float2>(cap);
+        s->vel = a.take<float2>(cap);
+    }
+    s->rank = a.take<uint32_t>(cap);
+    for (int b = 0; b < 2; ++b) {
+        s->keys[b] = key_size == 8 ? (void *)a.take<uint64_t>(cap) : (void *)a.take<uint32_t>(cap);
+        s->vals[b] = a.take<uint32_t>(cap);
+    }
+    s->tree.leaf = a.take<float4>(cap);
+    s->tree.pcm = a.take<float4>(kPairStride * cap);
+    s->tree.paabb = a.take<float4>(2 * cap);
+    s->tree.root = a.take<float4>(2);
+    s->tree.child = a.take<uint2>(cap);
+    s->tree.range = a.take<uint2>(cap);
+    s->tree.parent = a.take<uint32_t>(nodes);
+    s->tree.status = a.take<uint32_t>(cap);
+    s->tree.counts = a.take<uint32_t>(fill_count_words(cap));
+    s->tree.lists = a.take<uint32_t>(fill_list_words(cap));
+    s->tree.slot = a.take<uint32_t>(cap);
+    s->sort_words = a.take<uint32_t>(sort_scratch_words(cap));
+    s->bbox_partial = a.take<float4>((int64_t)s->max_chunks * s->num_sims);
+    s->bbox = a.take<float4>(s->num_sims);
+    s->bbox_slots = a.take<uint32_t>(4 * kBBoxSlots);
+    s->sim_off = a.take<int64_t>(s->num_sims + 1);
+    s->body_sim = s->ensemble ? a.take<uint8_t>(cap) : nullptr;
+    s->sim_root = a.take<uint32_t>(s->num_sims);
+    s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
+    s->accel = a.take<float2>(cap);
+    s->stats = a.take<unsigned long long>(4);
+    s->overflow = a.take<uint32_t>(1);
+}
+
 // (Re)allocate every buffer for `cap` bodies.  Contents are not preserved.
 int allocate(nbody_sim *s, int64_t cap) {
     release_graph(s);
@@ -179,57 +220,22 @@ int allocate(nbody_sim *s, int64_t cap) {
         s->arena = Arena{};
     }
     cap = std::max<int64_t>(cap, 1);
-    const size_t key_size = s->ensemble ? 8 : 4;
-    const int64_t nodes = 2 * cap - 1;
     s->max_chunks = (int)std::max<int64_t>(1, (cap + kBBoxChunk - 1) / kBBoxChunk);
     const int64_t sort_words = sort_scratch_words(cap);
     // Size pass: lay out with a null base, then allocate the total.
-    for (int pass = 0; pass < 2; ++pass) {
-        Arena &a = s->arena;
-        a.used = 0;
-        s->mass = a.take<float>(cap);
-        s->pos = a.take<float2>(cap);
-        s->vel = a.take<float2>(cap);
-        s->rank = a.take<uint32_t>(cap);
-        for (int b = 0; b < 2; ++b) {
-            s->keys[b] = key_size == 8 ? (void *)a.take<uint64_t>(cap) : (void *)a.take<uint32_t>(cap);
-            s->vals[b] = a.take<uint32_t>(cap);
-        }
-        s->tree.leaf = a.take<float4>(cap);
-        s->tree.pcm = a.take<float4>(kPairStride * cap);
-        s->tree.paabb = a.take<float4>(2 * cap);
-        s->tree.root = a.take<float4>(2);
-        s->tree.child = a.take<uint2>(cap);
-        s->tree.range = a.take<uint2>(cap);
-        s->tree.parent = a.take<uint32_t>(nodes);
-        s->tree.status = a.take<uint32_t>(cap);
-        s->tree.counts = a.take<uint32_t>(fill_count_words(cap));
-        s->tree.lists = a.take<uint32_t>(fill_list_words(cap));
-        s->tree.slot = a.take<uint32_t>(cap);
-        s->sort_words = a.take<uint32_t>(sort_words);
-        s->bbox_partial = a.take<float4>((int64_t)s->max_chunks * s->num_sims);
-        s->bbox = a.take<float4>(s->num_sims);
-        s->bbox_slots = a.take<uint32_t>(4 * kBBoxSlots);
-        s->sim_off = a.take<int64_t>(s->num_sims + 1);
-        s->body_sim = s->ensemble ? a.take<uint8_t>(cap) : nullptr;
-        s->sim_root = a.take<uint32_t>(s->num_sims);
-        s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
-        s->accel = a.take<float2>(cap);
-        s->stats = a.take<unsigned long long>(4);
-        s->overflow = a.take<uint32_t>(1);
-        if (pass == 0) {
-            a.size = a.used;
-            void *p = nullptr;
-            hipError_t e = hipMalloc(&p, a.size);
-            if (e != hipSuccess) {
-                s->cap = 0;
-                return fail(s, NBODY_ERR_CAPACITY,
-                            "hipMalloc of " + std::to_string(a.size) + " bytes failed: " +
-                                hipGetErrorString(e));
-            }
-            a.base = static_cast<char *>(p);
-        }
+    Arena &a = s->arena;
+    layout(s, a, cap, false);
+    a.size = a.used;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, a.size);
+    if (e != hipSuccess) {
+        s->cap = 0;
+        return fail(s, NBODY_ERR_CAPACITY,
+                    "hipMalloc of " + std::to_string(a.size) + " bytes failed: " +
+                        hipGetErrorString(e));
     }
+    a.base = static_cast<char *>(p);
+    layout(s, a, cap, false);
     s->bbox_fresh = false;
     sort_bind_scratch(s->sort, s->sort_words, cap);
     HIP_TRY(s, hipMemsetAsync(s->sort_words, 0, 4 * (size_t)sort_words, s->stream));
@@ -976,6 +982,127 @@ int nbody_adjoint_backward(nbody_sim *s, const float *gpos_T, const float *gvel_
     HIP_TRY(s, hipMemcpyAsync(s->vel, s->adj_end + n, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
     s->graph_dirty = true;
     return nbody_synchronize(s);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free step
+// nbody_step_static keeps every intermediate buffer of a step in `scratch`,
+// a file-scope device array of kStaticScratchBytes that each GPU holds from
+// the library's load to the end of the process.  Its address on a device
+// comes from hipGetSymbolAddress, once per device (the array's name is a
+// host-side stand-in, not a device address); every call on a device shares
+// that device's array and clears the words that must start at zero.
+namespace {
+constexpr size_t kStaticScratchBytes = size_t(16) << 20;
+constexpr int kStaticMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+
+__global__ void static_extent_kernel(int64_t *sim_off, int64_t n) {
+    sim_off[0] = 0;
+    sim_off[1] = n;
+}
+
+struct StaticDevice {
+    std::mutex lock;
+    char *base = nullptr;  // this device's `scratch`
+    nbody_sim sim;         // buffer views into base; never owns memory
+};
+StaticDevice g_static[kStaticMaxDevices];
+
+// First call on device `dev` (current): the array's device address, the
+// engine defaults, and a check that the largest layout fits the array.
+int static_setup(StaticDevice &d, int dev) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetSymbolAddress(scratch)");
+    nbody_sim *s = &d.sim;
+    s->device = dev;
+    s->ensemble = false;
+    s->num_sims = 1;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+        s->resident_waves = (uint32_t)cus * 32u;
+    s->max_chunks = (int)((NBODY_STATIC_MAX_BODIES + kBBoxChunk - 1) / kBBoxChunk);
+    Arena a;
+    layout(s, a, NBODY_STATIC_MAX_BODIES, true);
+    if (a.used > kStaticScratchBytes)
+        return fail(nullptr, NBODY_ERR_CAPACITY,
+                    "nbody_step_static: the layout of the largest step needs " +
+                        std::to_string(a.used) + " bytes, more than the static scratch");
+    d.base = static_cast<char *>(ptr);
+    return NBODY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbody_step_static(const nbody_config *cfg, int64_t n, const float *mass, float *pos_xy,
+                      float *vel_xy, int32_t nsteps, void *hip_stream) {
+    int rc = validate_config(cfg);
+    if (rc != NBODY_OK) return rc;
+    if (n < 0 || nsteps < 0)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "nbody_step_static: n and nsteps must be >= 0");
+    if (n > NBODY_STATIC_MAX_BODIES)
+        return fail(nullptr, NBODY_ERR_CAPACITY,
+                    "nbody_step_static: n = " + std::to_string(n) +
+                        " exceeds the static scratch limit of " +
+                        std::to_string(NBODY_STATIC_MAX_BODIES) + " bodies");
+    if (n > 0 && (!mass || !pos_xy || !vel_xy))
+        return fail(nullptr, NBODY_ERR_INVALID_ARG,
+                    "nbody_step_static: mass/pos/vel must be device pointers");
+    if (n == 0 || nsteps == 0) return NBODY_OK;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
+    if (dev < 0 || dev >= kStaticMaxDevices)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "nbody_step_static: device index too large");
+    StaticDevice &d = g_static[dev];
+    std::lock_guard<std::mutex> guard(d.lock);
+    if (!d.base) {
+        rc = static_setup(d, dev);
+        if (rc != NBODY_OK) return rc;
+    }
+    nbody_sim *s = &d.sim;
+    s->cfg = *cfg;
+    s->key_bits = cfg->morton_mode == NBODY_MORTON_BBOX30 ? 30 : 32;
+    s->n = n;
+    s->cap = n;
+    s->max_chunks = (int)std::max<int64_t>(1, (n + kBBoxChunk - 1) / kBBoxChunk);
+    Arena a;
+    a.base = d.base;
+    layout(s, a, n, true);  // within the largest layout (every size grows with n)
+    s->mass = const_cast<float *>(mass);
+    s->pos = reinterpret_cast<float2 *>(pos_xy);
+    s->vel = reinterpret_cast<float2 *>(vel_xy);
+    sort_bind_scratch(s->sort, s->sort_words, n);
+    s->stream = static_cast<hipStream_t>(hip_stream);
+    s->mass_absmax = NAN;  // masses are on the device: keep the own-leaf compare
+    s->bbox_fresh = false;
+    s->use_graph = false;
+    s->profiling = false;
+    s->debug = 0;
+    // words that must start at zero (another call may have left them set)
+    launch_clear_u32(s->sort_words, sort_scratch_words(n), s->stream);
+    launch_clear_u32(s->tree.counts, fill_count_words(n), s->stream);
+    launch_clear_u32(reinterpret_cast<uint32_t *>(s->stats), 8, s->stream);
+    launch_clear_u32(s->overflow, 1, s->stream);
+    static_extent_kernel<<<1, 1, 0, s->stream>>>(s->sim_off, n);
+    for (int32_t k = 0; k < nsteps; ++k) {
+        rc = enqueue_step(s);
+        if (rc != NBODY_OK) {
+            g_last_error = s->last_error;
+            return rc;
+        }
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(nullptr, e, "nbody_step_static: kernel launch");
+    return NBODY_OK;
 }
 
 const char *nbody_last_error(const nbody_sim *s) {

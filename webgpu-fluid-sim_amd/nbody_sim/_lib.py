@@ -60,8 +60,11 @@ EXPORTS = (
     "nbody_set_profiling", "nbody_phase_times", "nbody_set_tuning", "nbody_get_tuning",
     "nbody_last_error",
     "nbody_generate", "nbody_philox4x32", "nbody_version", "nbody_adjoint_forward",
-    "nbody_adjoint_backward",
+    "nbody_adjoint_backward", "nbody_step_static",
 )
+
+# Largest body count of nbody_step_static (its fixed device scratch).
+STATIC_MAX_BODIES = 65536
 
 
 class NbodyConfig(ctypes.Structure):
@@ -150,6 +153,7 @@ def lib() -> ctypes.CDLL:
         "nbody_version": (ctypes.c_char_p, []),
         "nbody_adjoint_forward": (ctypes.c_int, [vp, i32, i32]),
         "nbody_adjoint_backward": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+        "nbody_step_static": (ctypes.c_int, [cfgp, i64, vp, vp, vp, i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
