@@ -65,13 +65,23 @@ def log(msg):
 
 
 def workload():
-    import nbody_sim
+    from nbody_sim import workloads as W
 
-    mass, pos, vel = nbody_sim.generate("exp-disk", N_BODIES, seed=SEED, scale=0.2,
-                                        dispersion=0.05, mass_sigma=0.5)
-    cfg = nbody_sim.Config(gravConstant=1.0, bhTheta=THETA, deltaTime=DT, epsilon=EPS,
-                           softeningMode="constant", mortonMode="bbox30", mathMode="fast")
-    return cfg, mass, pos, vel
+    w = W.WORKLOADS["plain_800k"]
+    assert (w.n, w.theta, w.dt, w.eps, w.seed) == (N_BODIES, THETA, DT, EPS, SEED)
+    mass, pos, vel = w.bodies()
+    return w.config("fast"), mass, pos, vel
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when the environment sets it (the GPU box
+    allots a share of its host cores per GPU and sets it to that share)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        avail = min(avail, int(env))
+    return max(1, avail)
 
 
 def cpu_baseline(mass, pos, vel, budget_s=15.0):
@@ -79,7 +89,7 @@ def cpu_baseline(mass, pos, vel, budget_s=15.0):
     repeated until ~budget_s of work (at least one step)."""
     from oracle import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     P = O.Params(grav_constant=1.0, delta_time=DT, bh_theta=THETA, epsilon=EPS,
                  softening_mode=O.SOFTEN_CONSTANT, morton_mode=O.MORTON_BBOX30)
     steps = 0
@@ -93,6 +103,8 @@ def cpu_baseline(mass, pos, vel, budget_s=15.0):
         if el >= budget_s:
             break
     return {"value": N_BODIES * steps / el, "unit": "body-steps/s", "cores": threads,
+            "host_logical_cpus": os.cpu_count(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
             "kind": "port",
             "sample": f"{steps} full step(s) of the same 800k-body workload from its initial "
                       f"state, oracle C restatement, OpenMP {threads} threads, {el:.1f} s"}
@@ -106,39 +118,56 @@ def timed_steps(sim, steps):
     return time.perf_counter() - t0
 
 
-def plain_config(scenario, n, theta, dt, eps, steps, seed, **ic):
-    """body-steps/s of a plain configuration (graph replay, bodies resident)."""
+def plain_config(name, steps, math="fast", phases=False):
+    """body-steps/s of a plain workload (graph replay, bodies resident);
+    phases: also the per-phase device times of eager steps (HIP events)."""
     import nbody_sim
+    from nbody_sim import workloads as W
 
-    mass, pos, vel = nbody_sim.generate(scenario, n, seed=seed, **ic)
-    cfg = nbody_sim.Config(bhTheta=theta, deltaTime=dt, epsilon=eps, softeningMode="constant",
-                           mortonMode="bbox30", mathMode="fast")
-    sim = nbody_sim.Simulation(cfg)
+    w = W.WORKLOADS[name]
+    mass, pos, vel = w.bodies()
+    sim = nbody_sim.Simulation(w.config(math))
     sim.setBodies(mass, pos, vel)
     sim.useGraph(True)
     sim.step(3)
     el = timed_steps(sim, steps)
+    out = {"n": w.n, "steps": steps, "math_mode": math, "value": w.n * steps / el,
+           "ms_per_step": 1e3 * el / steps}
+    if phases:
+        sim.useGraph(False)
+        sim.setProfiling(True)
+        sim.phaseTimes(reset=True)
+        sim.step(20)
+        sim.synchronize()
+        ph, n = sim.phaseTimes(reset=True)
+        out["phases_ms"] = {k: round(v / n, 5) for k, v in ph.items()}
     sim.close()
-    return {"n": n, "steps": steps, "value": n * steps / el, "ms_per_step": 1e3 * el / steps}
+    return out
 
 
-def ensemble_config(steps=200, num_sims=256, seed=4):
+def ensemble_config(steps=200, tune=True):
     """256 independent sims (N log-uniform in [1024, 65536], uniform disk or
-    Plummer) stepped as one ensemble vs a loop of plain per-sim runs."""
-    import numpy as np
-
+    Plummer) stepped as one ensemble, with the launch tunables picked by the
+    autotuner on the ensemble itself, vs a loop of plain per-sim runs."""
     import nbody_sim
+    from nbody_sim import autotune
+    from nbody_sim import workloads as W
 
-    rng = np.random.default_rng(seed)
-    sizes = np.round(np.exp(rng.uniform(np.log(1024), np.log(65536), num_sims))).astype(np.int64)
-    kinds = rng.random(num_sims) < 0.5
-    parts = [nbody_sim.generate("plummer" if k else "uniform-disk", int(nn), seed=1000 + i)
-             for i, (nn, k) in enumerate(zip(sizes, kinds))]
-    cfg = nbody_sim.Config(bhTheta=0.6, deltaTime=1e-4, epsilon=1e-3, softeningMode="constant",
-                           mortonMode="bbox30", mathMode="fast")
+    sizes, _, parts = W.ensemble_members()
+    cfg = W.ensemble_config("fast")
     total = int(sizes.sum())
     ens = nbody_sim.Ensemble(cfg, sizes)
-    ens.setBodies(*[np.concatenate([p[i] for p in parts]) for i in range(3)])
+    ens.setBodies(*W.concat(parts))
+    tuning = None
+    if tune:
+        t0 = time.perf_counter()
+        best, table = autotune.autotune(ens, steps=2, variants=(2, 4, 1), blocks=(128, 256, 512),
+                                        stacks=(64, 128), xcd=(1, 0), lane_bodies=(1, 2),
+                                        radix=(10, 8, 11))
+        tuning = ens.getTuning()
+        tuning["autotune_s"] = round(time.perf_counter() - t0, 2)
+        tuning["candidates"] = len(table)
+        log(f"ensemble autotune: {tuning}")
     ens.useGraph(True)
     ens.step(3)
     el_ens = timed_steps(ens, steps)
@@ -158,24 +187,36 @@ def ensemble_config(steps=200, num_sims=256, seed=4):
     el_loop = time.perf_counter() - t0
     for s in sims:
         s.close()
-    return {"num_sims": num_sims, "total_bodies": total, "steps": steps,
+    return {"num_sims": len(sizes), "total_bodies": total, "key_bits": 38, "steps": steps,
             "value": total * steps / el_ens, "plain_loop_value": total * steps / el_loop,
-            "speedup_vs_plain_loop": el_loop / el_ens}
+            "speedup_vs_plain_loop": el_loop / el_ens, "tuning": tuning}
 
 
-def adjoint_config(n=65536, steps=200, every=20, seed=5):
-    """Adjoint (reverse sweep incl. segment recomputation) vs the forward run."""
+def adjoint_config(steps=200, every=20):
+    """Adjoint (reverse sweep incl. segment recomputation) vs the forward run,
+    both warmed up; the forward is timed eagerly (as the adjoint's own
+    recomputation runs) and replayed from a HIP graph (the production step)."""
     import numpy as np
 
     import nbody_sim
+    from nbody_sim import workloads as W
 
-    mass, pos, vel = nbody_sim.generate("exp-disk", n, seed=seed, scale=0.2, dispersion=0.05,
-                                        mass_sigma=0.5)
-    cfg = nbody_sim.Config(bhTheta=0.6, deltaTime=1e-4, epsilon=1e-3, softeningMode="constant",
-                           mortonMode="bbox30", mathMode="fast")
-    sim = nbody_sim.Simulation(cfg)
+    w = W.WORKLOADS["adjoint_65k"]
+    mass, pos, vel = w.bodies()
+    sim = nbody_sim.Simulation(w.config("fast"))
     sim.setBodies(mass, pos, vel)
-    el_fwd = timed_steps(sim, steps)              # plain forward, eager launches
+    sim.step(5)                                   # warm-up
+    sim.setBodies(mass, pos, vel)
+    el_fwd = timed_steps(sim, steps)              # eager forward
+    sim.setBodies(mass, pos, vel)
+    sim.useGraph(True)
+    sim.step(3)
+    sim.setBodies(mass, pos, vel)
+    el_graph = timed_steps(sim, steps)            # graph-replayed forward
+    sim.useGraph(False)
+    sim.setBodies(mass, pos, vel)
+    sim.adjointForward(10, every)                 # warm-up of the adjoint path
+    sim.adjointBackward(np.ones((w.n, 2), np.float32), None)
     sim.setBodies(mass, pos, vel)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -189,11 +230,51 @@ def adjoint_config(n=65536, steps=200, every=20, seed=5):
     el_bwd = time.perf_counter() - t0
     sim.close()
     ok = bool(np.all(np.isfinite(gx)) and np.all(np.isfinite(gv)) and np.all(np.isfinite(gm)))
-    return {"n": n, "steps": steps, "checkpoint_every": every,
+    return {"n": w.n, "steps": steps, "checkpoint_every": every,
             "forward_ms_per_step": 1e3 * el_fwd / steps,
+            "forward_graph_ms_per_step": 1e3 * el_graph / steps,
             "checkpointed_forward_ms_per_step": 1e3 * el_cp / steps,
             "adjoint_ms_per_step": 1e3 * el_bwd / steps,
-            "adjoint_over_forward": el_bwd / el_fwd, "gradients_finite": ok}
+            "adjoint_over_forward": el_bwd / el_fwd,
+            "adjoint_over_graph_forward": el_bwd / el_graph, "gradients_finite": ok}
+
+
+def force_error(nt=4096):
+    """Median, p99 and max relative force error of the benchmarked walk (fast
+    mode, one step from each workload's initial state) against the library's
+    binary64 direct sum, over `nt` seeded sample bodies per workload (the
+    ensemble: every 16th of its 256 simulations)."""
+    import numpy as np
+
+    import nbody_sim
+    from nbody_sim import workloads as W
+
+    def errs(cfg, mass, pos, vel, seed):
+        n = mass.shape[0]
+        tg = np.sort(np.random.default_rng(seed).choice(n, min(n, nt), replace=False))
+        sim = nbody_sim.Simulation(cfg)
+        sim.setBodies(mass, pos, vel)
+        ref = sim.directAccelF64(tg)
+        sim.setDebug(accel=True)
+        sim.step(1)
+        acc = sim.debugBuffer("accel")[tg]
+        sim.close()
+        return np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+    def summary(e, **kw):
+        return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+                "max": float(e.max()), "samples": int(e.size), **kw}
+
+    out = {}
+    for name in ("plain_16k", "plain_800k", "collision_4M", "adjoint_65k"):
+        w = W.WORKLOADS[name]
+        out[name] = summary(errs(w.config("fast"), *w.bodies(), 0), theta=w.theta, n=w.n)
+    sizes, _, parts = W.ensemble_members()
+    e = np.concatenate([errs(W.ensemble_config("fast"), *parts[s], s)
+                        for s in range(0, len(sizes), 16)])
+    out["ensemble_256"] = summary(e, theta=W.ENSEMBLE["theta"], members=list(range(0, 256, 16)))
+    out["reference"] = "binary64 softened direct sum (nbody_direct_accel_f64)"
+    return out
 
 
 def main():
@@ -290,15 +371,18 @@ def main():
         },
     }
     if not args.no_extras:
+        log("exact-mode 800k")
+        out["plain_800k_exact"] = plain_config("plain_800k", 20, math="exact", phases=True)
         log("small plain config")
-        out["plain_small_16k"] = plain_config("uniform-disk", 16384, 0.5, 1e-4, 1e-3, 100, 1)
+        out["plain_small_16k"] = plain_config("plain_16k", 100, phases=True)
         log("collision config")
-        out["plain_collision_4M"] = plain_config("collision", 4194304, 0.7, 2e-5, 5e-4, 100, 3,
-                                                 scale=0.15)
+        out["plain_collision_4M"] = plain_config("collision_4M", 100)
         log("ensemble")
         out["ensemble_256"] = ensemble_config()
         log("adjoint")
         out["adjoint_65k"] = adjoint_config()
+        log("force error")
+        out["force_error"] = force_error()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mass, pos, vel, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -156,6 +156,8 @@ int nbody_set_debug(nbody_sim *sim, int32_t flags);
 #define NBODY_BUF_ACCEL 7        /* f32[2n] per body (needs NBODY_DEBUG_ACCEL) */
 #define NBODY_BUF_BBOX 8         /* f32[4*num_sims] bounding box of the last step */
 #define NBODY_BUF_SIM_ROOTS 9    /* u32[num_sims] sub-tree root of each sim */
+#define NBODY_BUF_WALK_COUNTS 10 /* u32[2n] per body {accepted, opened} of the last walk with
+                                    NBODY_DEBUG_STATS (pair walks: walk_variant 2 and 4) */
 int nbody_debug_get(nbody_sim *sim, int32_t which, void *dst, int64_t bytes);
 
 /* Walk counters since the last reset: {accepted, opened, overflowed}. */
@@ -214,6 +216,14 @@ int nbody_get_tuning(const nbody_sim *sim, nbody_tuning *out);
 int nbody_adjoint_forward(nbody_sim *sim, int32_t nsteps, int32_t every);
 int nbody_adjoint_backward(nbody_sim *sim, const float *gpos_T, const float *gvel_T, float *gpos0,
                            float *gvel0, float *gmass);
+
+/* Accuracy reference of the walk (no reference counterpart: the reference
+ * never measures its approximation): binary64 softened direct sum over all
+ * bodies of a plain handle, with the handle's G and softening, for
+ * `ntargets` body indices `targets` (NULL: every body, ntargets ignored).
+ * acc_xy receives double[2 * ntargets].  Synchronous; allocates scratch. */
+int nbody_direct_accel_f64(nbody_sim *sim, const int64_t *targets, int64_t ntargets,
+                           double *acc_xy);
 
 /* Message of the last failure on `sim` (NULL: the last failure of a call
  * without a usable handle).  Never NULL. */

@@ -304,7 +304,7 @@ void orc_fill(int64_t n, const uint32_t *idx, const float *mass, const float *po
 void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32_t *left,
               const uint32_t *right, const uint32_t *roots, const float *mass,
               const float *pos_xy, const orc_params *p, int stack_size, float *accel_xy,
-              int64_t *stats, int threads) {
+              int64_t *stats, float *margin, float *term_norm, int threads) {
     if (n <= 0) return;
     const float g = p->grav_constant, dt = p->delta_time, theta = p->bh_theta;
     const float mult = p->epsilon_multiplier, epsc = p->epsilon;
@@ -322,6 +322,8 @@ void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32
         float m1 = mass[bi];
         float ax = 0.0f, ay = 0.0f;
         int64_t acc = 0, opened = 0, dropped = 0;
+        float closest = INFINITY; /* smallest |length*inv - theta| / theta of an internal node */
+        float l1 = 0.0f;          /* sum of the accepted terms' lengths */
         int sp = 0;
         stack[0] = roots ? roots[t] : 0u; /* barnes_hut.wgsl:67-70 */
         while (sp >= 0) {                  /* barnes_hut.wgsl:72-107 */
@@ -342,11 +344,16 @@ void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32
                 eps = epsc;
             }
             float inv = 1.0f / sqrtf(d2 + eps * eps);
+            if (margin && !leaf) {
+                float q = fabsf(C[3] * inv - theta) / theta;
+                if (!(q >= closest)) closest = q;
+            }
             if (leaf || C[3] * inv < theta) {
                 float inv3 = inv * inv * inv;
                 float gm = g * m2;
                 ax = ax + gm * rx * inv3;
                 ay = ay + gm * ry * inv3;
+                if (term_norm) l1 += fabsf(gm * inv3) * sqrtf(d2);
                 acc++;
             } else if (sp + 2 < ssz) {
                 stack[++sp] = left[node];
@@ -363,6 +370,8 @@ void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32
             stats[3 * (size_t)bi + 1] = opened;
             stats[3 * (size_t)bi + 2] = dropped;
         }
+        if (margin) margin[bi] = closest;
+        if (term_norm) term_norm[bi] = l1;
     }
 }
 
@@ -460,7 +469,8 @@ static void one_step(int64_t n, const orc_params *p, const float *mass, float *p
     orc_sort_pairs_u32(n, s->keys, s->idx, 32);
     orc_build_u32(n, s->keys, s->left, s->right, s->parent);
     orc_fill(n, s->idx, mass, pos, s->left, s->right, s->parent, 0, 0, s->cm, s->bb);
-    orc_walk(n, s->idx, s->cm, s->left, s->right, NULL, mass, pos, p, 256, s->acc, stats, threads);
+    orc_walk(n, s->idx, s->cm, s->left, s->right, NULL, mass, pos, p, 256, s->acc, stats, NULL,
+             NULL, threads);
     orc_kick_drift(n, s->acc, p->delta_time, pos, vel);
 }
 

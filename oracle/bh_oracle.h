@@ -86,11 +86,16 @@ void orc_fill(int64_t n, const uint32_t *idx, const float *mass, const float *po
  * per sorted position, the node the walk starts at (ensemble sub-tree
  * roots); NULL = node 0.  stats (may be NULL): per body {accepted, opened}.
  * Leaves are nodes [n-1, 2n-2]; the self leaf of sorted position t is n-1+t.
+ * margin (may be NULL): per body, the smallest |length*inv - theta| / theta
+ * over the internal nodes its walk tests (+inf if none): how close its
+ * closest opening decision came to the other branch of barnes_hut.wgsl:94.
+ * term_norm (may be NULL): per body, the sum of the lengths of the accepted
+ * terms |G m r inv^3| (the scale of the sum's rounding error).
  * Runs on `threads` OpenMP threads (results do not depend on it). */
 void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32_t *left,
               const uint32_t *right, const uint32_t *roots, const float *mass,
               const float *pos_xy, const orc_params *p, int stack_size, float *accel_xy,
-              int64_t *stats, int threads);
+              int64_t *stats, float *margin, float *term_norm, int threads);
 
 /* The nodes each sorted position's walk accepts (barnes_hut.wgsl:94), in
  * walk order, as CSR: offsets[n+1], nodes[offsets[t]..offsets[t+1]).

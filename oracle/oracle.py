@@ -169,7 +169,11 @@ def fill(idx, mass, pos, left, right, parent, offset=0, order_seed=0):
 
 
 def walk(idx, cm, left, right, mass, pos, params: Params, roots=None, stack_size=256,
-         threads=None):
+         threads=None, margin=False):
+    """orc_walk: (accel (n,2), stats (n,3) {accepted, opened, dropped} by body)
+    and, with margin=True, each body's smallest relative distance
+    |length*inv - theta| / theta of an opening test from theta and the sum of
+    its accepted terms' lengths |G m r inv^3|: (accel, stats, margin, term_norm)."""
     idx = np.ascontiguousarray(idx, np.uint32)
     cm = _f32(cm, (-1, 4))
     left = np.ascontiguousarray(left, np.uint32)
@@ -184,10 +188,13 @@ def walk(idx, cm, left, right, mass, pos, params: Params, roots=None, stack_size
     if roots is not None:
         roots = np.ascontiguousarray(roots, np.uint32)
         rp = _p(roots)
+    mg = np.zeros(n, np.float32) if margin else None
+    tn = np.zeros(n, np.float32) if margin else None
     lib().orc_walk(ctypes.c_int64(n), _p(idx), _p(cm), _p(left), _p(right), rp, _p(mass),
                    _p(pos), ctypes.byref(pr), ctypes.c_int(stack_size), _p(acc), _p(stats),
+                   _p(mg) if margin else None, _p(tn) if margin else None,
                    ctypes.c_int(threads or _threads()))
-    return acc, stats
+    return (acc, stats, mg, tn) if margin else (acc, stats)
 
 
 def kick_drift(acc, dt, pos, vel):

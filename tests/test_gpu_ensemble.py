@@ -7,6 +7,7 @@ import pytest
 
 import nbody_sim
 from helpers import params
+from nbody_sim import workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +92,57 @@ def test_ensemble_topology_maps_to_plain_trees():
         for k in range(n - 1):
             e = to_ens(k)
             assert tuple(ch[e]) == (to_ens(r["children"][k, 0]), to_ens(r["children"][k, 1])), (s, k)
+
+
+def _c4_ensemble(math, radix_bits=None, steps=2):
+    sizes, _, parts = W.ensemble_members()
+    ens = nbody_sim.Ensemble(W.ensemble_config(math), sizes)
+    ens.setBodies(*W.concat(parts))
+    if radix_bits is not None:
+        ens.setTuning(radix_bits=radix_bits)
+    ens.setDebug(accel=True)
+    ens.step(steps)
+    ens.synchronize()
+    out = {k: ens.debugBuffer(k) for k in ("keys", "body_indices", "accel", "sim_roots")}
+    _, out["pos"], out["vel"] = ens.getBodies()
+    ens.close()
+    return sizes, parts, out
+
+
+@pytest.mark.parametrize("math", ["fast", "exact"])
+def test_c4_ensemble_256_sims_equals_plain_runs(math):
+    """The benchmarked ensemble itself: 256 sims of 1,024-65,536 bodies
+    (4.65 M in all), 38-bit keys (8 sim-id bits above the 30-bit Morton
+    code; 4 passes of 10 bits), 2 steps: sorted keys, permutation,
+    accelerations and state equal 256 plain runs bit for bit."""
+    sizes, parts, e = _c4_ensemble(math)
+    keys = e["keys"]
+    assert keys.dtype == np.uint64 and int(keys.max() >> 30) == 255  # 38-bit keys
+    assert np.all(np.diff(keys.astype(np.uint64)) >= 0)
+    assert np.array_equal(np.sort(e["body_indices"]), np.arange(keys.shape[0], dtype=np.uint32))
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    cfg = W.ensemble_config(math)
+    for s in range(len(sizes)):
+        r = plain_run(cfg, *parts[s], 2)
+        lo, hi = off[s], off[s + 1]
+        assert np.all((keys[lo:hi] >> 30) == s), s
+        assert np.array_equal((keys[lo:hi] & ((1 << 30) - 1)).astype(np.uint32), r["keys"]), s
+        assert np.array_equal(e["body_indices"][lo:hi] - lo, r["body_indices"]), s
+        assert np.array_equal(e["accel"][lo:hi], r["accel"]), s
+        assert np.array_equal(e["pos"][lo:hi], r["pos"]), s
+        assert np.array_equal(e["vel"][lo:hi], r["vel"]), s
+        assert e["sim_roots"][s] in (lo, hi - 1), s
+
+
+def test_c4_ensemble_radix_widths_identical():
+    """38-bit ensemble keys sorted with 8-, 10- and 11-bit digits (5, 4 and 4
+    passes) give the same keys, permutation and state."""
+    ref = None
+    for rb in (10, 8, 11):
+        _, _, e = _c4_ensemble("fast", radix_bits=rb, steps=1)
+        got = [e["keys"], e["body_indices"], e["pos"], e["vel"]]
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b), rb

@@ -187,7 +187,8 @@ def test_radix_widths_identical(morton):
 
 
 def test_radix_widths_ensemble():
-    """38-bit ensemble keys (sim id above the Morton code): 5, 4 and 4 passes."""
+    """33-bit ensemble keys (5 sims: 3 sim-id bits above the 30-bit Morton code):
+    5, 4 and 3 passes at 8, 10 and 11 bits (38-bit keys: test_gpu_ensemble.py)."""
     sizes = [900, 5000, 1, 3000, 64]
     parts = [nbody_sim.generate("plummer", n, seed=40 + i) for i, n in enumerate(sizes)]
     bodies = [np.concatenate([p[i] for p in parts]) for i in range(3)]

@@ -89,6 +89,7 @@ struct nbody_sim {
     uint32_t *sim_root = nullptr;
     float2 *accel = nullptr;
     unsigned long long *stats = nullptr;
+    uint2 *body_counts = nullptr;  // per-body walk counters (stats mode)
     uint32_t *overflow = nullptr;
     int result_buf = 0;  // which sort buffer holds the last sorted keys/values
 
@@ -209,6 +210,7 @@ void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
     s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
     s->accel = a.take<float2>(cap);
     s->stats = a.take<unsigned long long>(4);
+    s->body_counts = a.take<uint2>(cap);
     s->overflow = a.take<uint32_t>(1);
 }
 
@@ -379,6 +381,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     w.pos = s->pos;
     w.accel = (s->debug & NBODY_DEBUG_ACCEL) ? s->accel : nullptr;
     w.stats = s->stats;
+    w.body_counts = (s->debug & NBODY_DEBUG_STATS) ? s->body_counts : nullptr;
     w.overflow = s->overflow;
     w.n = (uint32_t)n;
     w.p = p;
@@ -775,6 +778,10 @@ int nbody_debug_get(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
             src = s->sim_root;
             need = (int64_t)s->num_sims * 4;
             break;
+        case NBODY_BUF_WALK_COUNTS:
+            src = s->body_counts;
+            need = n * 8;
+            break;
         default:
             return fail(s, NBODY_ERR_INVALID_ARG, "unknown buffer id");
     }
@@ -1102,6 +1109,44 @@ int nbody_step_static(const nbody_config *cfg, int64_t n, const float *mass, flo
     }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(nullptr, e, "nbody_step_static: kernel launch");
+    return NBODY_OK;
+}
+
+int nbody_direct_accel_f64(nbody_sim *s, const int64_t *targets, int64_t ntargets,
+                           double *acc_xy) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (s->ensemble) return fail(s, NBODY_ERR_INVALID_ARG, "the direct sum takes plain handles");
+    const int64_t n = s->n;
+    if (!targets) ntargets = n;
+    if (ntargets < 0 || (ntargets > 0 && !acc_xy))
+        return fail(s, NBODY_ERR_INVALID_ARG, "bad targets / output");
+    std::vector<int64_t> tg((size_t)ntargets);
+    for (int64_t k = 0; k < ntargets; ++k) {
+        tg[(size_t)k] = targets ? targets[k] : k;
+        if (tg[(size_t)k] < 0 || tg[(size_t)k] >= n)
+            return fail(s, NBODY_ERR_INVALID_ARG, "target index out of range");
+    }
+    if (ntargets == 0) return NBODY_OK;
+    HIP_TRY(s, hipSetDevice(s->device));
+    const int chunks = direct_chunks(n, ntargets);
+    const size_t bytes = 8 * (size_t)ntargets + 16 * (size_t)ntargets * ((size_t)chunks + 1);
+    char *buf = nullptr;
+    HIP_TRY(s, hipMalloc(reinterpret_cast<void **>(&buf), bytes));
+    int64_t *d_tg = reinterpret_cast<int64_t *>(buf);
+    double2 *d_out = reinterpret_cast<double2 *>(buf + 8 * (size_t)ntargets);
+    double2 *d_part = d_out + ntargets;
+    hipError_t e = hipMemcpyAsync(d_tg, tg.data(), 8 * (size_t)ntargets, hipMemcpyHostToDevice,
+                                  s->stream);
+    if (e == hipSuccess) {
+        launch_direct_f64(s->pos, s->mass, n, d_tg, ntargets, step_params(s->cfg), d_part, chunks,
+                          d_out, s->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(acc_xy, d_out, 16 * (size_t)ntargets, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(s, e, "nbody_direct_accel_f64");
     return NBODY_OK;
 }
 

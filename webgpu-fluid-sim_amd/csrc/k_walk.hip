@@ -526,6 +526,8 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
     const float eps2c = p.eps * p.eps;
     float ax = 0.0f, ay = 0.0f;
     uint64_t n_acc = 0, n_open = 0, n_drop = 0;
+    uint32_t my_acc = 0, my_open = 0;  // this lane's own counters (stats mode)
+    auto own = [&](uint64_t m) { return (uint32_t)((m >> lane) & 1ull); };
     auto probe = [&](const float4 &c, float &rx, float &ry, float &inv) {
         rx = c.x - px;
         ry = c.y - py;
@@ -557,6 +559,8 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
         if constexpr (kStats) {
             n_acc += __builtin_popcountll(acc);
             n_open += __builtin_popcountll(open);
+            my_acc += own(acc);
+            my_open += own(open);
         }
         if (open) {
             if (sp < kPairStackTop) {
@@ -567,6 +571,7 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
                 if constexpr (kStats) {
                     n_open -= __builtin_popcountll(open);
                     n_drop += __builtin_popcountll(open);
+                    my_open -= own(open);
                 }
             }
         }
@@ -622,6 +627,8 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
             if constexpr (kStats) {
                 n_acc += __builtin_popcountll(accl) + __builtin_popcountll(accr);
                 n_open += __builtin_popcountll(openl) + __builtin_popcountll(openr);
+                my_acc += own(accl) + own(accr);
+                my_open += own(openl) + own(openr);
             }
             // The next entry: the right child first, the left one waits on
             // the stack (written to the free slot every time, kept by
@@ -637,6 +644,7 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
                     if (top >> 6) {
                         n_open -= __builtin_popcountll(openl);
                         n_drop += __builtin_popcountll(openl);
+                        my_open -= own(openl);
                     }
                 }
             }
@@ -680,6 +688,7 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
             atomicAdd(&a.stats[1], (unsigned long long)n_open);
             if (n_drop) atomicAdd(&a.stats[2], (unsigned long long)n_drop);
         }
+        if (valid && a.body_counts) a.body_counts[a.idx[t]] = make_uint2(my_acc, my_open);
     }
 }
 

@@ -164,6 +164,7 @@ struct WalkArgs {
     float2 *pos;             // [n] by body (drift output)
     float2 *accel;           // optional [n] by body
     unsigned long long *stats;  // optional {accepted, opened, overflowed}
+    uint2 *body_counts;         // optional [n] by body {accepted, opened} (pair walks, stats mode)
     uint32_t *overflow;         // set to 1 when a full stack drops nodes
     uint32_t n;
     // 1 when a body's own leaf adds an exact zero to its fast-mode sum:
@@ -204,6 +205,14 @@ void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, fl
 void launch_adjoint_walk(const AdjointArgs &a, bool exact, int stack, int lane_bodies,
                          hipStream_t s);
 void launch_adjoint_fold(const AdjointArgs &a, hipStream_t s);
+
+// ---------------------------------------------------------------- FP64 direct sum
+// Softened direct sum in binary64 for the given target bodies (the walk's
+// accuracy reference): partial[chunks][ntargets], out[ntargets].
+int direct_chunks(int64_t n, int64_t ntargets);
+void launch_direct_f64(const float2 *pos, const float *mass, int64_t n, const int64_t *targets,
+                       int64_t ntargets, const StepParams &p, double2 *partial, int chunks,
+                       double2 *out, hipStream_t s);
 
 // ---------------------------------------------------------------- misc
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s);

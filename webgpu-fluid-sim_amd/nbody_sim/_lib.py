@@ -40,6 +40,7 @@ BUF_NODE_AABB = 6
 BUF_ACCEL = 7
 BUF_BBOX = 8
 BUF_SIM_ROOTS = 9
+BUF_WALK_COUNTS = 10
 
 SCENARIO_DEFAULT = 0
 SCENARIO_TWO_CLUSTERS = 1
@@ -60,7 +61,7 @@ EXPORTS = (
     "nbody_set_profiling", "nbody_phase_times", "nbody_set_tuning", "nbody_get_tuning",
     "nbody_last_error",
     "nbody_generate", "nbody_philox4x32", "nbody_version", "nbody_adjoint_forward",
-    "nbody_adjoint_backward", "nbody_step_static",
+    "nbody_adjoint_backward", "nbody_step_static", "nbody_direct_accel_f64",
 )
 
 # Largest body count of nbody_step_static (its fixed device scratch).
@@ -154,6 +155,7 @@ def lib() -> ctypes.CDLL:
         "nbody_adjoint_forward": (ctypes.c_int, [vp, i32, i32]),
         "nbody_adjoint_backward": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
         "nbody_step_static": (ctypes.c_int, [cfgp, i64, vp, vp, vp, i32, vp]),
+        "nbody_direct_accel_f64": (ctypes.c_int, [vp, vp, i64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -165,6 +165,20 @@ class _Handle:
             _ptr(gp), _ptr(gv), _ptr(gm)))
         return gp, gv, gm
 
+    def directAccelF64(self, targets=None) -> np.ndarray:
+        """Binary64 softened direct-sum accelerations (ntargets, 2) of the
+        current bodies at `targets` (None: all): the walk's accuracy reference."""
+        n = self.getNumBodies()
+        if targets is None:
+            out = np.zeros((n, 2), np.float64)
+            self._check(_lib.lib().nbody_direct_accel_f64(self.handle, None, n, _ptr(out)))
+            return out
+        tg = np.ascontiguousarray(targets, np.int64)
+        out = np.zeros((tg.shape[0], 2), np.float64)
+        self._check(_lib.lib().nbody_direct_accel_f64(self.handle, _ptr(tg), tg.shape[0],
+                                                      _ptr(out)))
+        return out
+
     def debugBuffer(self, which: str) -> np.ndarray:
         """Intermediate buffers of the last step, as numpy arrays."""
         n = self.getNumBodies()
@@ -183,6 +197,7 @@ class _Handle:
             "accel": (_lib.BUF_ACCEL, np.zeros((n, 2), np.float32)),
             "bbox": (_lib.BUF_BBOX, np.zeros((s, 4), np.float32)),
             "sim_roots": (_lib.BUF_SIM_ROOTS, np.zeros(s, np.uint32)),
+            "walk_counts": (_lib.BUF_WALK_COUNTS, np.zeros((n, 2), np.uint32)),
         }
         if which not in spec:
             raise ValueError(f"unknown buffer {which}")
