@@ -103,6 +103,30 @@ int nbody_get_bodies(nbody_sim *sim, float *mass, float *pos_xy, float *vel_xy);
 /* getNumBodies() (simulation.ts:430-432). */
 int64_t nbody_get_num_bodies(const nbody_sim *sim);
 
+/* updateRadiusMultiplierBuffer() (simulation.ts:382-384) and the
+ * radiusMultiplier member of getBuffers() (simulation.ts:17, 87, 427-429):
+ * per-body size factors of the density splat (density.wgsl:57-66, bound by
+ * renderer.ts:147-153).  The step never reads them.  nbody_set_bodies resets
+ * them to 1, as setScenario fills them (simulation.ts:240-241). */
+int nbody_set_radius_multiplier(nbody_sim *sim, const float *radius_multiplier);
+int nbody_get_radius_multiplier(nbody_sim *sim, float *radius_multiplier);
+
+/* Steps run since the bodies were last set (nbody_step, nbody_adjoint_forward;
+ * restored by nbody_load_snapshot).  No reference counterpart. */
+int nbody_get_step_count(const nbody_sim *sim, int64_t *steps);
+
+/* State snapshot (no reference counterpart: the reference never reads its
+ * buffers back).  nbody_save_snapshot writes directory `dir` (created if
+ * needed): mass.npy f4[n], pos.npy f4[n,2], vel.npy f4[n,2],
+ * radius_multiplier.npy f4[n], perm.npy u4[n] (sorted position -> body of the
+ * last step; identity before any step) in NumPy .npy format, and meta.json
+ * (n, simulation sizes, step count, config).  nbody_load_snapshot restores
+ * config, bodies, radius multipliers, permutation and step count into a
+ * handle with the same simulation layout; stepping on from a loaded snapshot
+ * gives the same bits as stepping on from the saved handle. */
+int nbody_save_snapshot(nbody_sim *sim, const char *dir);
+int nbody_load_snapshot(nbody_sim *sim, const char *dir);
+
 /* getCommands() + device.queue.submit() (simulation.ts:386-425, main.ts:38-41):
  * enqueue `nsteps` full steps (Morton encode, sort, tree build, fill, walk
  * with kick, drift) on the handle's stream.  Asynchronous. */
@@ -133,8 +157,12 @@ int nbody_synchronize(nbody_sim *sim);
 /* Run nbody_step's work on `hip_stream` (a hipStream_t; NULL = the handle's own). */
 int nbody_set_stream(nbody_sim *sim, void *hip_stream);
 
-/* Replay steps from a captured HIP graph (1) or launch kernels one by one (0). */
-int nbody_use_graph(nbody_sim *sim, int32_t enable);
+/* Launch kernels one by one (0), or replay steps from a captured HIP graph
+ * holding `steps_per_graph` (1..4096) consecutive steps: nbody_step(k) then
+ * replays the graph k / steps_per_graph times and launches the remaining
+ * k % steps_per_graph steps one by one (the reference encodes `substeps`
+ * steps per command buffer, simulation.ts:393).  Results do not depend on it. */
+int nbody_use_graph(nbody_sim *sim, int32_t steps_per_graph);
 
 /* Debug flags for later steps: NBODY_DEBUG_ACCEL records each body's
  * acceleration; NBODY_DEBUG_STATS counts accepted and opened nodes;
@@ -247,6 +275,11 @@ typedef struct nbody_ic_params {
 } nbody_ic_params;
 int nbody_generate(int32_t scenario, int64_t n, uint64_t seed, const nbody_ic_params *params,
                    float *mass, float *pos_xy, float *vel_xy);
+
+/* setScenario's radiusMultiplierData (simulation.ts:240-241, 321, 330, 339):
+ * 1 for every body; planetary-disk: 12 for the central body, 5 for the two
+ * planets. */
+int nbody_generate_radius(int32_t scenario, int64_t n, float *radius_multiplier);
 
 /* Philox-4x32-10 (Salmon et al., SC'11) used by nbody_generate. */
 void nbody_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

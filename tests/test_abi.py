@@ -127,3 +127,16 @@ def test_static_step_rejects_sizes_beyond_its_scratch():
     assert str(_lib.STATIC_MAX_BODIES) in _lib.last_error(None)
     assert L.nbody_step_static(ctypes.byref(c), -1, None, None, None, 1, None) == _lib.ERR_INVALID_ARG
     assert L.nbody_step_static(ctypes.byref(c), 0, None, None, None, 1, None) == _lib.OK
+
+
+def test_generate_radius_known_answers():
+    """setScenario's radiusMultiplierData (simulation.ts:240-241, 321, 330, 339)."""
+    r = nbody_sim.generate_radius("planetary-disk", 6)
+    assert r.dtype == np.float32 and r.tolist() == [12.0, 5.0, 5.0, 1.0, 1.0, 1.0]
+    assert nbody_sim.generate_radius("planetary-disk", 2).tolist() == [12.0, 5.0]
+    for scen in ("default", "two-clusters", "exp-disk"):
+        assert np.all(nbody_sim.generate_radius(scen, 100) == 1.0)
+    assert nbody_sim.generate_radius("default", 0).shape == (0,)
+    with pytest.raises(ValueError):
+        nbody_sim.generate_radius("nope", 3)
+    assert _lib.lib().nbody_generate_radius(99, 3, None) == _lib.ERR_INVALID_ARG

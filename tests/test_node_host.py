@@ -87,3 +87,38 @@ fs.writeFileSync('{f}', Buffer.concat([Buffer.from(out.pos.buffer), Buffer.from(
     _, pos, vel = sim.getBodies()
     sim.close()
     assert np.array_equal(jpos, pos) and np.array_equal(jvel, vel)
+
+
+@pytest.mark.gpu
+def test_node_radius_and_snapshot_resume_in_python(tmp_path):
+    """The Node host's setScenario fills radiusMultiplier like simulation.ts:
+    321-339; a snapshot saved from Node resumes in Python bit for bit."""
+    snap = str(tmp_path / "snap")
+    f = str(tmp_path / "after.bin")
+    out = run_node(f"""
+const fs = require('fs');
+const s = require('./simulation.js');
+const cfg = Object.assign({{}}, s.getDefaultConfig(), {{epsilon: 1e-2, softeningMode: 1}});
+const sim = new s.Simulation(cfg, 0, 3);
+sim.setNumBodies(2000);
+sim.setScenario('planetary-disk');
+const r = sim.getBuffers().radiusMultiplier;
+sim.step(4);
+sim.saveSnapshot('{snap}');
+sim.step(3);
+sim.synchronize();
+const b = sim.getBuffers();
+fs.writeFileSync('{f}', Buffer.concat([Buffer.from(b.pos.buffer), Buffer.from(b.vel.buffer)]));
+process.stdout.write(JSON.stringify(Array.from(r.slice(0, 4))));
+""")
+    assert json.loads(out) == [12.0, 5.0, 5.0, 1.0]
+    raw = np.fromfile(f, np.float32)
+    sim = nbody_sim.Simulation()
+    sim.loadSnapshot(snap)
+    assert sim.getNumBodies() == 2000 and sim.getStepCount() == 4
+    assert sim.getRadiusMultiplier()[:3].tolist() == [12.0, 5.0, 5.0]
+    sim.step(3)
+    _, pos, vel = sim.getBodies()
+    sim.close()
+    assert np.array_equal(raw[:4000].reshape(-1, 2), pos)
+    assert np.array_equal(raw[4000:].reshape(-1, 2), vel)

@@ -18,10 +18,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <mutex>
+#include <sstream>
+#include <sys/stat.h>
 #include <string>
 #include <vector>
 
@@ -88,6 +93,7 @@ struct nbody_sim {
     uint8_t *body_sim = nullptr;
     uint32_t *sim_root = nullptr;
     float2 *accel = nullptr;
+    float *radius = nullptr;  // per-body radius multipliers (renderer input)
     unsigned long long *stats = nullptr;
     uint2 *body_counts = nullptr;  // per-body walk counters (stats mode)
     uint32_t *overflow = nullptr;
@@ -100,6 +106,8 @@ struct nbody_sim {
     int xcd_order = 1;  // walk workgroup order over the XCDs (nbody_tuning.xcd_order)
 
     bool use_graph = false;
+    int graph_steps = 1;  // steps captured per graph
+    int64_t steps_taken = 0;  // steps run since the bodies were set (snapshots)
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     bool graph_dirty = true;
@@ -209,6 +217,7 @@ void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
     s->sim_root = a.take<uint32_t>(s->num_sims);
     s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
     s->accel = a.take<float2>(cap);
+    s->radius = external_bodies ? nullptr : a.take<float>(cap);
     s->stats = a.take<unsigned long long>(4);
     s->body_counts = a.take<uint2>(cap);
     s->overflow = a.take<uint32_t>(1);
@@ -413,7 +422,8 @@ int accumulate_profile(nbody_sim *s) {
 int capture_graph(nbody_sim *s) {
     release_graph(s);
     HIP_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(s);
+    int rc = NBODY_OK;
+    for (int k = 0; k < s->graph_steps && rc == NBODY_OK; ++k) rc = enqueue_step(s);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (rc != NBODY_OK) {
@@ -650,6 +660,9 @@ int nbody_set_bodies(nbody_sim *s, int64_t n, const float *mass, const float *po
     }
     HIP_TRY(s, hipMemcpyAsync(s->sim_off, s->sim_off_host.data(), 8 * (size_t)(s->num_sims + 1),
                               hipMemcpyHostToDevice, s->stream));
+    if (n > 0)  // setScenario's radiusMultiplierData.fill(1.0) (simulation.ts:240-241)
+        launch_fill_f32(s->radius, 1.0f, n, s->stream);
+    s->steps_taken = 0;
     if (s->ensemble) {
         std::vector<uint8_t> bs((size_t)n);
         for (int i = 0; i < s->num_sims; ++i)
@@ -684,13 +697,19 @@ int nbody_step(nbody_sim *s, int32_t nsteps) {
     if (nsteps < 0) return fail(s, NBODY_ERR_INVALID_ARG, "nsteps must be >= 0");
     HIP_TRY(s, hipSetDevice(s->device));
     if (s->n <= 0 || nsteps == 0) return NBODY_OK;
+    s->steps_taken += nsteps;
     if (s->use_graph) {
         prime_bbox(s);  // eagerly, before the captured steps that expect it
         if (s->graph_dirty || !s->graph_exec) {
             int rc = capture_graph(s);
             if (rc != NBODY_OK) return rc;
         }
-        for (int i = 0; i < nsteps; ++i) HIP_TRY(s, hipGraphLaunch(s->graph_exec, s->stream));
+        const int k = s->graph_steps;
+        for (int i = 0; i < nsteps / k; ++i) HIP_TRY(s, hipGraphLaunch(s->graph_exec, s->stream));
+        for (int i = 0; i < nsteps % k; ++i) {  // the rest, launched one by one
+            int rc = enqueue_step(s);
+            if (rc != NBODY_OK) return rc;
+        }
         return NBODY_OK;
     }
     for (int i = 0; i < nsteps; ++i) {
@@ -727,9 +746,12 @@ int nbody_set_stream(nbody_sim *s, void *hip_stream) {
     return NBODY_OK;
 }
 
-int nbody_use_graph(nbody_sim *s, int32_t enable) {
+int nbody_use_graph(nbody_sim *s, int32_t steps_per_graph) {
     if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
-    s->use_graph = enable != 0;
+    if (steps_per_graph < 0 || steps_per_graph > 4096)
+        return fail(s, NBODY_ERR_INVALID_ARG, "steps_per_graph must be in [0, 4096]");
+    s->use_graph = steps_per_graph != 0;
+    s->graph_steps = std::max(1, (int)steps_per_graph);
     s->graph_dirty = true;
     return NBODY_OK;
 }
@@ -894,6 +916,7 @@ int nbody_adjoint_forward(nbody_sim *s, int32_t nsteps, int32_t every) {
             s->adj_base = p;
         }
     }
+    s->steps_taken += nsteps;
     s->adj_n = n;
     s->adj_steps = nsteps;
     s->adj_every = every;
@@ -1147,6 +1170,232 @@ int nbody_direct_accel_f64(nbody_sim *s, const int64_t *targets, int64_t ntarget
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(s, e, "nbody_direct_accel_f64");
+    return NBODY_OK;
+}
+
+int nbody_set_radius_multiplier(nbody_sim *s, const float *radius_multiplier) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (s->n > 0 && !radius_multiplier) return fail(s, NBODY_ERR_INVALID_ARG, "NULL array");
+    HIP_TRY(s, hipSetDevice(s->device));
+    if (s->n > 0)
+        HIP_TRY(s, hipMemcpyAsync(s->radius, radius_multiplier, 4 * (size_t)s->n,
+                                  hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NBODY_OK;
+}
+
+int nbody_get_radius_multiplier(nbody_sim *s, float *radius_multiplier) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    if (s->n > 0 && !radius_multiplier) return fail(s, NBODY_ERR_INVALID_ARG, "NULL array");
+    HIP_TRY(s, hipSetDevice(s->device));
+    if (s->n > 0)
+        HIP_TRY(s, hipMemcpyAsync(radius_multiplier, s->radius, 4 * (size_t)s->n,
+                                  hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NBODY_OK;
+}
+
+int nbody_get_step_count(const nbody_sim *s, int64_t *steps) {
+    if (!s || !steps) return fail(nullptr, NBODY_ERR_INVALID_ARG, "NULL argument");
+    *steps = s->steps_taken;
+    return NBODY_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- snapshots
+// A snapshot is a directory of .npy arrays (NumPy format 1.0, little endian,
+// C order) plus meta.json:
+//   mass.npy f4[n], pos.npy f4[n,2], vel.npy f4[n,2], radius_multiplier.npy
+//   f4[n], perm.npy u4[n] (sorted position -> body of the last step; the
+//   identity before any step), meta.json {format, n, num_sims, n_per_sim,
+//   steps, config}.
+namespace {
+
+bool npy_write(const std::string &path, const char *descr, const std::vector<int64_t> &shape,
+               const void *data, size_t bytes) {
+    std::string dims;
+    for (size_t i = 0; i < shape.size(); ++i) dims += std::to_string(shape[i]) + ", ";
+    if (shape.size() > 1) dims.erase(dims.size() - 2);
+    std::string h = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': (" +
+                    dims + "), }";
+    const size_t total = (10 + h.size() + 1 + 63) / 64 * 64;
+    h.append(total - 10 - h.size() - 1, ' ');
+    h.push_back('\n');
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    const unsigned char pre[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    f.write(reinterpret_cast<const char *>(pre), 8);
+    const uint16_t len = (uint16_t)h.size();
+    const unsigned char l[2] = {(unsigned char)(len & 0xFF), (unsigned char)(len >> 8)};
+    f.write(reinterpret_cast<const char *>(l), 2);
+    f.write(h.data(), (std::streamsize)h.size());
+    f.write(static_cast<const char *>(data), (std::streamsize)bytes);
+    return (bool)f;
+}
+
+// Reads a C-order array with the given dtype; false on any mismatch.
+bool npy_read(const std::string &path, const char *descr, std::vector<int64_t> &shape,
+              std::vector<char> &data) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    unsigned char pre[8];
+    if (!f.read(reinterpret_cast<char *>(pre), 8) || pre[0] != 0x93 ||
+        std::memcmp(pre + 1, "NUMPY", 5) != 0)
+        return false;
+    uint32_t len = 0;
+    if (pre[6] == 1) {
+        unsigned char l[2];
+        if (!f.read(reinterpret_cast<char *>(l), 2)) return false;
+        len = l[0] | (l[1] << 8);
+    } else if (pre[6] == 2 || pre[6] == 3) {
+        unsigned char l[4];
+        if (!f.read(reinterpret_cast<char *>(l), 4)) return false;
+        len = l[0] | (l[1] << 8) | (l[2] << 16) | ((uint32_t)l[3] << 24);
+    } else {
+        return false;
+    }
+    std::string h(len, '\0');
+    if (!f.read(&h[0], len)) return false;
+    const std::string want = std::string("'descr': '") + descr + "'";
+    if (h.find(want) == std::string::npos || h.find("'fortran_order': False") == std::string::npos)
+        return false;
+    const size_t a = h.find("'shape': (");
+    if (a == std::string::npos) return false;
+    const size_t b = h.find(')', a);
+    std::stringstream dims(h.substr(a + 10, b - a - 10));
+    shape.clear();
+    std::string tok;
+    size_t count = 1;
+    while (std::getline(dims, tok, ',')) {
+        if (tok.find_first_not_of(' ') == std::string::npos) continue;
+        shape.push_back(std::stoll(tok));
+        count *= (size_t)shape.back();
+    }
+    data.resize(count * 4);
+    return (bool)f.read(data.data(), (std::streamsize)data.size());
+}
+
+// Number after "key": in a JSON text written by nbody_save_snapshot.
+bool json_num(const std::string &t, const std::string &key, double &v) {
+    const size_t k = t.find("\"" + key + "\":");
+    if (k == std::string::npos) return false;
+    v = std::strtod(t.c_str() + k + key.size() + 3, nullptr);
+    return true;
+}
+
+std::string fmt9(double v) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "%.9g", v);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbody_save_snapshot(nbody_sim *s, const char *dir) {
+    if (!s || !dir) return fail(s, NBODY_ERR_INVALID_ARG, "NULL argument");
+    const std::string d(dir);
+    if (mkdir(dir, 0755) != 0 && errno != EEXIST)
+        return fail(s, NBODY_ERR_INVALID_ARG, "cannot create snapshot directory " + d);
+    const int64_t n = s->n;
+    std::vector<float> mass((size_t)n), pos(2 * (size_t)n), vel(2 * (size_t)n), rad((size_t)n);
+    std::vector<uint32_t> perm((size_t)n);
+    int rc = nbody_get_bodies(s, mass.data(), pos.data(), vel.data());
+    if (rc == NBODY_OK) rc = nbody_get_radius_multiplier(s, rad.data());
+    if (rc != NBODY_OK) return rc;
+    if (n > 0 && s->steps_taken > 0) {
+        HIP_TRY(s, hipMemcpy(perm.data(), s->vals[s->result_buf], 4 * (size_t)n,
+                             hipMemcpyDeviceToHost));
+    } else {
+        for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = (uint32_t)i;
+    }
+    const bool ok = npy_write(d + "/mass.npy", "<f4", {n}, mass.data(), 4 * mass.size()) &&
+                    npy_write(d + "/pos.npy", "<f4", {n, 2}, pos.data(), 4 * pos.size()) &&
+                    npy_write(d + "/vel.npy", "<f4", {n, 2}, vel.data(), 4 * vel.size()) &&
+                    npy_write(d + "/radius_multiplier.npy", "<f4", {n}, rad.data(), 4 * rad.size()) &&
+                    npy_write(d + "/perm.npy", "<u4", {n}, perm.data(), 4 * perm.size());
+    if (!ok) return fail(s, NBODY_ERR_INVALID_ARG, "cannot write snapshot arrays in " + d);
+    const nbody_config &c = s->cfg;
+    std::string sizes;
+    for (int i = 0; i < s->num_sims; ++i)
+        sizes += (i ? ", " : "") + std::to_string(s->sim_off_host[i + 1] - s->sim_off_host[i]);
+    std::ofstream m(d + "/meta.json");
+    m << "{\"format\": \"nbody_sim snapshot 1\", \"n\": " << n << ", \"num_sims\": " << s->num_sims
+      << ", \"n_per_sim\": [" << sizes << "], \"steps\": " << s->steps_taken
+      << ", \"config\": {\"grav_constant\": " << fmt9(c.grav_constant)
+      << ", \"bh_theta\": " << fmt9(c.bh_theta) << ", \"substeps\": " << c.substeps
+      << ", \"delta_time\": " << fmt9(c.delta_time)
+      << ", \"epsilon_multiplier\": " << fmt9(c.epsilon_multiplier)
+      << ", \"epsilon\": " << fmt9(c.epsilon) << ", \"softening_mode\": " << c.softening_mode
+      << ", \"morton_mode\": " << c.morton_mode << ", \"math_mode\": " << c.math_mode
+      << ", \"offset\": " << c.offset << "}}\n";
+    if (!m) return fail(s, NBODY_ERR_INVALID_ARG, "cannot write " + d + "/meta.json");
+    return NBODY_OK;
+}
+
+int nbody_load_snapshot(nbody_sim *s, const char *dir) {
+    if (!s || !dir) return fail(s, NBODY_ERR_INVALID_ARG, "NULL argument");
+    const std::string d(dir);
+    std::ifstream mf(d + "/meta.json");
+    if (!mf) return fail(s, NBODY_ERR_INVALID_ARG, "no meta.json in " + d);
+    std::stringstream ss;
+    ss << mf.rdbuf();
+    const std::string t = ss.str();
+    if (t.find("\"nbody_sim snapshot 1\"") == std::string::npos)
+        return fail(s, NBODY_ERR_INVALID_ARG, d + "/meta.json is not an nbody_sim snapshot");
+    double n_d = -1, sims_d = 0, steps_d = 0;
+    if (!json_num(t, "n", n_d) || !json_num(t, "num_sims", sims_d) || !json_num(t, "steps", steps_d))
+        return fail(s, NBODY_ERR_INVALID_ARG, "incomplete meta.json in " + d);
+    const int64_t n = (int64_t)n_d;
+    if ((int)sims_d != s->num_sims || (s->ensemble && n != s->sim_off_host[s->num_sims]))
+        return fail(s, NBODY_ERR_INVALID_ARG, "snapshot layout (sims, bodies) differs from the handle");
+    if (s->ensemble) {  // every simulation's size must match
+        const size_t a = t.find("\"n_per_sim\": [");
+        std::stringstream arr(t.substr(a + 14, t.find(']', a) - a - 14));
+        std::string tok;
+        for (int i = 0; i < s->num_sims; ++i) {
+            if (!std::getline(arr, tok, ',') ||
+                std::stoll(tok) != s->sim_off_host[i + 1] - s->sim_off_host[i])
+                return fail(s, NBODY_ERR_INVALID_ARG, "snapshot simulation sizes differ");
+        }
+    }
+    nbody_config c = s->cfg;
+    double v;
+    const std::string cfg = t.substr(t.find("\"config\""));
+    if (json_num(cfg, "grav_constant", v)) c.grav_constant = (float)v;
+    if (json_num(cfg, "bh_theta", v)) c.bh_theta = (float)v;
+    if (json_num(cfg, "substeps", v)) c.substeps = (int32_t)v;
+    if (json_num(cfg, "delta_time", v)) c.delta_time = (float)v;
+    if (json_num(cfg, "epsilon_multiplier", v)) c.epsilon_multiplier = (float)v;
+    if (json_num(cfg, "epsilon", v)) c.epsilon = (float)v;
+    if (json_num(cfg, "softening_mode", v)) c.softening_mode = (int32_t)v;
+    if (json_num(cfg, "morton_mode", v)) c.morton_mode = (int32_t)v;
+    if (json_num(cfg, "math_mode", v)) c.math_mode = (int32_t)v;
+    if (json_num(cfg, "offset", v)) c.offset = (uint32_t)v;
+    std::vector<int64_t> sh;
+    std::vector<char> mass, pos, vel, rad, perm;
+    auto need = [&](const char *name, const char *descr, std::vector<char> &buf, int64_t cols) {
+        return npy_read(d + "/" + name, descr, sh, buf) && !sh.empty() && sh[0] == n &&
+               (cols == 1 ? sh.size() == 1 : (sh.size() == 2 && sh[1] == cols));
+    };
+    if (!need("mass.npy", "<f4", mass, 1) || !need("pos.npy", "<f4", pos, 2) ||
+        !need("vel.npy", "<f4", vel, 2) || !need("radius_multiplier.npy", "<f4", rad, 1) ||
+        !need("perm.npy", "<u4", perm, 1))
+        return fail(s, NBODY_ERR_INVALID_ARG, "missing or malformed arrays in " + d);
+    int rc = nbody_set_config(s, &c);
+    if (rc == NBODY_OK)
+        rc = nbody_set_bodies(s, n, reinterpret_cast<const float *>(mass.data()),
+                              reinterpret_cast<const float *>(pos.data()),
+                              reinterpret_cast<const float *>(vel.data()));
+    if (rc == NBODY_OK) rc = nbody_set_radius_multiplier(s, reinterpret_cast<const float *>(rad.data()));
+    if (rc != NBODY_OK) return rc;
+    if (n > 0) {  // the last step's permutation, for nbody_debug_get
+        HIP_TRY(s, hipMemcpy(s->vals[0], perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
+        s->result_buf = 0;
+    }
+    s->steps_taken = (int64_t)steps_d;
     return NBODY_OK;
 }
 

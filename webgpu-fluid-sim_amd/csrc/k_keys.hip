@@ -303,4 +303,19 @@ void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s) {
     clear_u32_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, words);
 }
 
+namespace {
+__global__ void fill_f32_kernel(float *p, float value, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+}  // namespace
+
+void launch_fill_f32(float *p, float value, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    fill_f32_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, value, count);
+}
+
 }  // namespace nbs

@@ -216,5 +216,6 @@ void launch_direct_f64(const float2 *pos, const float *mass, int64_t n, const in
 
 // ---------------------------------------------------------------- misc
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s);
+void launch_fill_f32(float *p, float value, int64_t count, hipStream_t s);
 
 }  // namespace nbs

@@ -245,4 +245,17 @@ int nbody_generate(int32_t scenario, int64_t n, uint64_t seed, const nbody_ic_pa
     }
 }
 
+int nbody_generate_radius(int32_t scenario, int64_t n, float *radius_multiplier) {
+    if (n < 0 || (n > 0 && !radius_multiplier)) return NBODY_ERR_INVALID_ARG;
+    if (scenario < NBODY_SCENARIO_DEFAULT || scenario > NBODY_SCENARIO_PLUMMER)
+        return NBODY_ERR_INVALID_ARG;
+    for (int64_t i = 0; i < n; ++i) radius_multiplier[i] = 1.0f;  // simulation.ts:240-241
+    if (scenario == NBODY_SCENARIO_PLANETARY_DISK) {
+        if (n > 0) radius_multiplier[0] = 12.0f;  // central body, simulation.ts:321
+        if (n > 1) radius_multiplier[1] = 5.0f;   // planets, simulation.ts:330, 339
+        if (n > 2) radius_multiplier[2] = 5.0f;
+    }
+    return NBODY_OK;
+}
+
 }  // extern "C"

@@ -6,7 +6,8 @@ src/simulation.ts and src/config.ts of the reference.
 """
 from ._lib import NbodyError, lib  # noqa: F401
 from .config import Config, getDefaultConfig  # noqa: F401
-from .simulation import SCENARIOS, Ensemble, Simulation, generate  # noqa: F401
+from .simulation import SCENARIOS, Ensemble, Simulation, generate, generate_radius  # noqa: F401
 
-__all__ = ["Config", "getDefaultConfig", "Simulation", "Ensemble", "generate", "SCENARIOS",
+__all__ = ["Config", "getDefaultConfig", "Simulation", "Ensemble", "generate", "generate_radius",
+           "SCENARIOS",
            "NbodyError", "lib"]

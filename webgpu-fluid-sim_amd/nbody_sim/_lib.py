@@ -62,6 +62,8 @@ EXPORTS = (
     "nbody_last_error",
     "nbody_generate", "nbody_philox4x32", "nbody_version", "nbody_adjoint_forward",
     "nbody_adjoint_backward", "nbody_step_static", "nbody_direct_accel_f64",
+    "nbody_set_radius_multiplier", "nbody_get_radius_multiplier", "nbody_get_step_count",
+    "nbody_save_snapshot", "nbody_load_snapshot", "nbody_generate_radius",
 )
 
 # Largest body count of nbody_step_static (its fixed device scratch).
@@ -156,6 +158,12 @@ def lib() -> ctypes.CDLL:
         "nbody_adjoint_backward": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
         "nbody_step_static": (ctypes.c_int, [cfgp, i64, vp, vp, vp, i32, vp]),
         "nbody_direct_accel_f64": (ctypes.c_int, [vp, vp, i64, vp]),
+        "nbody_set_radius_multiplier": (ctypes.c_int, [vp, vp]),
+        "nbody_get_radius_multiplier": (ctypes.c_int, [vp, vp]),
+        "nbody_get_step_count": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
+        "nbody_save_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "nbody_load_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "nbody_generate_radius": (ctypes.c_int, [i32, i64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

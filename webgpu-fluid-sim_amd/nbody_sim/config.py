@@ -38,6 +38,17 @@ class Config:
         except KeyError as e:
             raise ValueError(f"unknown config mode {e}") from None
 
+    @staticmethod
+    def from_c(c: _lib.NbodyConfig) -> "Config":
+        soft = {_lib.SOFTEN_REFERENCE: "reference", _lib.SOFTEN_CONSTANT: "constant"}
+        mort = {_lib.MORTON_REFERENCE: "reference", _lib.MORTON_BBOX30: "bbox30"}
+        math = {_lib.MATH_FAST: "fast", _lib.MATH_EXACT: "exact"}
+        return Config(gravConstant=c.grav_constant, bhTheta=c.bh_theta, substeps=c.substeps,
+                      deltaTime=c.delta_time, epsilonMultiplier=c.epsilon_multiplier,
+                      epsilon=c.epsilon, softeningMode=soft[c.softening_mode],
+                      mortonMode=mort[c.morton_mode], mathMode=math[c.math_mode],
+                      offset=c.offset)
+
     def with_(self, **kw) -> "Config":
         return replace(self, **kw)
 

@@ -38,6 +38,15 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def generate_radius(scenario: str, n: int) -> np.ndarray:
+    """setScenario's radiusMultiplierData (simulation.ts:240-241, 321, 330, 339)."""
+    if scenario not in SCENARIOS:
+        raise ValueError(f"Unknown scenario: {scenario}")
+    r = np.zeros(n, np.float32)
+    _lib.check(_lib.lib().nbody_generate_radius(SCENARIOS[scenario], n, _ptr(r)))
+    return r
+
+
 def generate(scenario: str, n: int, seed: int = 0, grav_constant: float = 1.0,
              scale: float = 0.0, dispersion: float = 0.0, mass_sigma: float = 0.0):
     """Seeded initial conditions: (mass (n,), pos (n,2), vel (n,2)) float32."""
@@ -105,8 +114,38 @@ class _Handle:
     def synchronize(self):
         self._check(_lib.lib().nbody_synchronize(self.handle))
 
-    def useGraph(self, enable: bool = True):
-        self._check(_lib.lib().nbody_use_graph(self.handle, int(bool(enable))))
+    def useGraph(self, enable: bool = True, steps: int = 1):
+        """Replay `steps` steps per captured HIP graph (enable False: no graph)."""
+        self._check(_lib.lib().nbody_use_graph(self.handle, int(steps) if enable else 0))
+
+    def setRadiusMultiplier(self, r):
+        """updateRadiusMultiplierBuffer (simulation.ts:382-384)."""
+        r = np.ascontiguousarray(r, np.float32)
+        if r.shape != (self.getNumBodies(),):
+            raise ValueError("one radius multiplier per body")
+        self._check(_lib.lib().nbody_set_radius_multiplier(self.handle, _ptr(r)))
+
+    def getRadiusMultiplier(self) -> np.ndarray:
+        r = np.zeros(self.getNumBodies(), np.float32)
+        self._check(_lib.lib().nbody_get_radius_multiplier(self.handle, _ptr(r)))
+        return r
+
+    def getStepCount(self) -> int:
+        k = ctypes.c_int64(0)
+        self._check(_lib.lib().nbody_get_step_count(self.handle, ctypes.byref(k)))
+        return int(k.value)
+
+    def saveSnapshot(self, path):
+        """Write mass/pos/vel/radius_multiplier/perm .npy files and meta.json."""
+        self._check(_lib.lib().nbody_save_snapshot(self.handle, str(path).encode()))
+
+    def loadSnapshot(self, path):
+        """Restore config, bodies, radius multipliers and step count."""
+        self._check(_lib.lib().nbody_load_snapshot(self.handle, str(path).encode()))
+        self._after_load()
+
+    def _after_load(self):
+        pass
 
     def setDebug(self, accel: bool = False, stats: bool = False, fill_climb: bool = False):
         """Record accelerations / walk statistics; fill_climb forces the tree
@@ -250,6 +289,7 @@ class Simulation(_Handle):
                                   grav_constant=self.config.gravConstant, **ic)
         self.updateMetadataBuffer()
         self.setBodies(mass, pos, vel)
+        self.setRadiusMultiplier(generate_radius(scenario, self.numBodies))  # simulation.ts:373
 
     def getCommands(self):
         """simulation.ts:386-425 + queue.submit (main.ts:38-41): substeps steps."""
@@ -258,7 +298,8 @@ class Simulation(_Handle):
     def getBuffers(self):
         """simulation.ts:427-429, read back to host."""
         mass, pos, vel = self.getBodies()
-        return {"mass": mass, "pos": pos, "vel": vel}
+        return {"mass": mass, "pos": pos, "vel": vel,
+                "radiusMultiplier": self.getRadiusMultiplier()}
 
     def getNumBodies(self) -> int:
         return super().getNumBodies()
@@ -266,6 +307,12 @@ class Simulation(_Handle):
     def setBodies(self, mass, pos, vel):
         super().setBodies(mass, pos, vel)
         self.numBodies = int(np.asarray(mass).shape[0])
+
+    def _after_load(self):
+        self.numBodies = self.getNumBodies()
+        c = _lib.NbodyConfig()
+        self._check(_lib.lib().nbody_get_config(self.handle, ctypes.byref(c)))
+        self.config = Config.from_c(c)
 
 
 class Ensemble(_Handle):

@@ -23,6 +23,7 @@ export type SimBuffers = {
     mass: Float32Array;   // [n]
     pos: Float32Array;    // [2n] x0,y0,x1,y1,...
     vel: Float32Array;    // [2n]
+    radiusMultiplier?: Float32Array;  // [n] (getBuffers; simulation.ts:17)
 };
 
 export declare function generate(scenario: SimScenario, n: number, seed?: number,
@@ -37,7 +38,10 @@ export declare class Simulation {
     getCommands(): void;
     step(nsteps?: number): void;
     synchronize(): void;
-    useGraph(enable?: boolean): void;
+    useGraph(enable?: boolean, stepsPerGraph?: number): void;
+    setRadiusMultiplier(r: Float32Array): void;
+    saveSnapshot(dir: string): void;
+    loadSnapshot(dir: string): void;
     getBuffers(): SimBuffers;
     getNumBodies(): number;
 }

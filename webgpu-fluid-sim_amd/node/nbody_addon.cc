@@ -223,13 +223,87 @@ napi_value Synchronize(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
+// useGraph(handle, stepsPerGraph): 0 = launch kernels one by one.
 napi_value UseGraph(napi_env env, napi_callback_info info) {
     napi_value argv[2];
     if (!get_args(env, info, 2, argv)) return nullptr;
     nbody_sim *h = unwrap(env, argv[0]);
-    bool b = false;
-    napi_get_value_bool(env, argv[1], &b);
-    int rc = nbody_use_graph(h, b ? 1 : 0);
+    int rc = nbody_use_graph(h, (int32_t)num(env, argv[1], 1));
+    if (rc) return throw_nbody(env, rc, h);
+    return nullptr;
+}
+
+// updateRadiusMultiplierBuffer (simulation.ts:382-384).
+napi_value SetRadius(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    nbody_sim *h = unwrap(env, argv[0]);
+    size_t nr = 0;
+    float *r = f32(env, argv[1], &nr);
+    if (!r || (int64_t)nr != nbody_get_num_bodies(h)) {
+        napi_throw_type_error(env, nullptr, "setRadiusMultiplier(r: Float32Array(n))");
+        return nullptr;
+    }
+    int rc = nbody_set_radius_multiplier(h, r);
+    if (rc) return throw_nbody(env, rc, h);
+    return nullptr;
+}
+
+napi_value GetRadius(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    if (!get_args(env, info, 1, argv)) return nullptr;
+    nbody_sim *h = unwrap(env, argv[0]);
+    const int64_t n = nbody_get_num_bodies(h);
+    if (n < 0) return throw_nbody(env, NBODY_ERR_INVALID_ARG, h);
+    float *r;
+    napi_value ar = new_f32(env, (size_t)n, &r);
+    if (!ar) return nullptr;
+    int rc = nbody_get_radius_multiplier(h, r);
+    if (rc) return throw_nbody(env, rc, h);
+    return ar;
+}
+
+// setScenario's radiusMultiplierData (simulation.ts:240-241, 321, 330, 339).
+napi_value GenerateRadius(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    const int64_t n = (int64_t)num(env, argv[1], 0);
+    if (n < 0) {
+        napi_throw_range_error(env, nullptr, "n must be >= 0");
+        return nullptr;
+    }
+    float *r;
+    napi_value ar = new_f32(env, (size_t)n, &r);
+    if (!ar) return nullptr;
+    if (nbody_generate_radius((int32_t)num(env, argv[0], -1), n, r)) {
+        napi_throw_error(env, nullptr, "Unknown scenario");
+        return nullptr;
+    }
+    return ar;
+}
+
+std::string str(napi_env env, napi_value v) {
+    size_t len = 0;
+    napi_get_value_string_utf8(env, v, nullptr, 0, &len);
+    std::string out(len, '\0');
+    napi_get_value_string_utf8(env, v, &out[0], len + 1, &len);
+    return out;
+}
+
+napi_value SaveSnapshot(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    nbody_sim *h = unwrap(env, argv[0]);
+    int rc = nbody_save_snapshot(h, str(env, argv[1]).c_str());
+    if (rc) return throw_nbody(env, rc, h);
+    return nullptr;
+}
+
+napi_value LoadSnapshot(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    nbody_sim *h = unwrap(env, argv[0]);
+    int rc = nbody_load_snapshot(h, str(env, argv[1]).c_str());
     if (rc) return throw_nbody(env, rc, h);
     return nullptr;
 }
@@ -288,6 +362,11 @@ napi_value Init(napi_env env, napi_value exports) {
         {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"synchronize", nullptr, Synchronize, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"useGraph", nullptr, UseGraph, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"setRadius", nullptr, SetRadius, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"getRadius", nullptr, GetRadius, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"generateRadius", nullptr, GenerateRadius, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"saveSnapshot", nullptr, SaveSnapshot, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"loadSnapshot", nullptr, LoadSnapshot, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"getNumBodies", nullptr, GetNumBodies, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"generate", nullptr, Generate, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},

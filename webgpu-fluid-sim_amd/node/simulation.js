@@ -8,6 +8,7 @@
 //   getCommands()                simulation.ts:386-425 + queue.submit (main.ts:38-41):
 //                                enqueues config.substeps steps on the GPU
 //   getBuffers()                 simulation.ts:427-429, read back as Float32Arrays
+//                                (mass, pos, vel, radiusMultiplier)
 //   getNumBodies()               simulation.ts:430-432
 "use strict";
 
@@ -71,6 +72,21 @@ class Simulation {
                            { gravConstant: this.config.gravConstant });
         this.updateMetadataBuffer();
         addon.setBodies(this.handle, b.mass, b.pos, b.vel);
+        // updateRadiusMultiplierBuffer (simulation.ts:373, 382-384)
+        addon.setRadius(this.handle, addon.generateRadius(SCENARIOS[scenario], this.numBodies));
+    }
+
+    setRadiusMultiplier(r) {
+        addon.setRadius(this.handle, r);
+    }
+
+    saveSnapshot(dir) {
+        addon.saveSnapshot(this.handle, dir);
+    }
+
+    loadSnapshot(dir) {
+        addon.loadSnapshot(this.handle, dir);
+        this.numBodies = addon.getNumBodies(this.handle);
     }
 
     setBodies(mass, pos, vel) {
@@ -90,12 +106,15 @@ class Simulation {
         addon.synchronize(this.handle);
     }
 
-    useGraph(enable = true) {
-        addon.useGraph(this.handle, !!enable);
+    // Replay `stepsPerGraph` steps per captured HIP graph (false/0: no graph).
+    useGraph(enable = true, stepsPerGraph = 1) {
+        addon.useGraph(this.handle, enable ? stepsPerGraph : 0);
     }
 
     getBuffers() {
-        return addon.getBodies(this.handle);
+        const b = addon.getBodies(this.handle);
+        b.radiusMultiplier = addon.getRadius(this.handle);
+        return b;
     }
 
     getNumBodies() {
