@@ -186,6 +186,8 @@ int nbody_set_debug(nbody_sim *sim, int32_t flags);
 #define NBODY_BUF_SIM_ROOTS 9    /* u32[num_sims] sub-tree root of each sim */
 #define NBODY_BUF_WALK_COUNTS 10 /* u32[2n] per body {accepted, opened} of the last walk with
                                     NBODY_DEBUG_STATS (pair walks: walk_variant 2 and 4) */
+#define NBODY_BUF_TONE_TABLE 11  /* u32[65536] RGBA8 (R in the low byte) of every fp16
+                                    density value, after the first nbody_render */
 int nbody_debug_get(nbody_sim *sim, int32_t which, void *dst, int64_t bytes);
 
 /* Walk counters since the last reset: {accepted, opened, overflowed}. */
@@ -252,6 +254,35 @@ int nbody_adjoint_backward(nbody_sim *sim, const float *gpos_T, const float *gve
  * acc_xy receives double[2 * ntargets].  Synchronous; allocates scratch. */
 int nbody_direct_accel_f64(nbody_sim *sim, const int64_t *targets, int64_t ntargets,
                            double *acc_xy);
+
+/* The reference's Renderer (renderer.ts:22-329) drawing the simulation's
+ * positions and radius multipliers: camera and viewport of its Metadata
+ * uniform (renderer.ts:158-172, density.wgsl:3-10). */
+typedef struct nbody_camera {
+    float center_x, center_y; /* camCenter (renderer.ts:49)                       */
+    float half_x, half_y;     /* camHalfSize (renderer.ts:50; x = y * aspect, 242-243) */
+    int32_t width, height;    /* viewport and density texture size in pixels      */
+} nbody_camera;
+
+/* The Renderer's initial camera for a width x height canvas (renderer.ts:
+ * 49-50, 240-243): centre (0, 0), half height 10, half width 10 * aspect. */
+int nbody_default_camera(int32_t width, int32_t height, nbody_camera *out);
+
+/* Renderer.getCommands() (renderer.ts:293-329) as compute kernels: every
+ * body's quad of 2R x 2R pixels, R = 2 * radiusMultiplier, adds
+ * base * exp(-10 r^2) * zoom_scale at each pixel centre with r = |uv| <= 1
+ * (density.wgsl:41-96), summed exactly in fixed point and rounded once to
+ * fp16 (the reference blends into r16float, rounding every add); then the
+ * tone map (tone_map.wgsl:46-84: exposure 8, Reinhard, gamma 1/2.2, the
+ * four-stop colour ramp) to 8-bit RGBA.  density_f16 (may be NULL) receives
+ * the fp16 texture, uint16 bits [height][width], row 0 at the top (NDC y =
+ * +1); rgba (may be NULL) the image, uint8 [height][width][4], row 0 at the
+ * top of the canvas.  The full-screen pass flips rows, as the reference's
+ * does: canvas row y shows texture row height-1-y.  Synchronous. */
+int nbody_render(nbody_sim *sim, const nbody_camera *cam, uint16_t *density_f16, uint8_t *rgba);
+
+/* 8-bit RGBA image -> PNG file (zlib-compressed, no filter). */
+int nbody_write_png(const char *path, int32_t width, int32_t height, const uint8_t *rgba);
 
 /* Message of the last failure on `sim` (NULL: the last failure of a call
  * without a usable handle).  Never NULL. */

@@ -547,3 +547,153 @@ void orc_direct_sum_f64(int64_t n, const float *pos_xy, const float *mass, const
         acc_xy[2 * k + 1] = ay;
     }
 }
+
+/* ---------------------------------------------------------------- render */
+
+/* exp(x), x <= 0, in binary32: x log2(e) = n + f (n = rint), 2^f by the
+ * degree-7 Taylor polynomial of exp(f ln 2) in Horner form, times 2^n. */
+float orc_render_exp(float x) {
+    float t = x * 1.44269504f;
+    float n = rintf(t);
+    float f = t - n;
+    float p = 1.5252734e-05f;
+    p = p * f + 1.5403530e-04f;
+    p = p * f + 1.3333558e-03f;
+    p = p * f + 9.6181291e-03f;
+    p = p * f + 5.5504109e-02f;
+    p = p * f + 2.4022651e-01f;
+    p = p * f + 6.9314718e-01f;
+    p = p * f + 1.0f;
+    return ldexpf(p, (int)n);
+}
+
+uint16_t orc_fixed40_to_f16(uint64_t v) {
+    if (v == 0) return 0;
+    int msb = 63 - clz64(v);
+    int e = msb - 40;
+    uint64_t m, rem, half;
+    if (e >= -14) {
+        int shift = msb - 10;
+        m = v >> shift;
+        rem = v & ((1ull << shift) - 1ull);
+        half = 1ull << (shift - 1);
+    } else {
+        m = v >> 16;
+        rem = v & 0xFFFFull;
+        half = 0x8000ull;
+        e = -15;
+    }
+    if (rem > half || (rem == half && (m & 1ull))) ++m;
+    if (e == -15) return (uint16_t)m;
+    if (m == 2048ull) {
+        m = 1024ull;
+        ++e;
+    }
+    if (e > 15) return 0x7C00u;
+    return (uint16_t)(((uint32_t)(e + 15) << 10) | (uint32_t)(m & 0x3FFull));
+}
+
+void orc_splat(int64_t n, const float *pos_xy, const float *radius, const float *cam4,
+               int32_t width, int32_t height, uint64_t *fixed) {
+    /* density.wgsl:41-96, each fragment's value added exactly in units of 2^-40 */
+    memset(fixed, 0, 8 * (size_t)width * (size_t)height);
+    const float cx = cam4[0], cy = cam4[1], hx = cam4[2], hy = cam4[3];
+    const float zoom = fmaxf(hx, hy);
+    const float zoom_scale = 1.0f / (1.0f + 0.05f * zoom);
+    const float sx = 2.0f / (float)width, sy = 2.0f / (float)height;
+    const float W = (float)width, H = (float)height;
+    for (int64_t i = 0; i < n; ++i) {
+        const float px = pos_xy[2 * i], py = pos_xy[2 * i + 1], rm = radius[i];
+        const float cnx = (px - cx) / hx, cny = (py - cy) / hy;
+        const float rpx = 2.0f * rm;
+        const float rnx = rpx * sx, rny = rpx * sy;
+        if (!(fabsf(cnx) < 1e30f) || !(fabsf(cny) < 1e30f) || !(rnx > 0.0f) || !(rny > 0.0f) ||
+            !(rnx < 1e30f))
+            continue;
+        const float base = rm > 1.0f ? 0.1f : 0.02f;
+        const float fx0 = floorf((cnx - rnx + 1.0f) / sx - 0.5f) - 1.0f;
+        const float fx1 = ceilf((cnx + rnx + 1.0f) / sx - 0.5f) + 1.0f;
+        const float fy0 = floorf((1.0f - cny - rny) / sy - 0.5f) - 1.0f;
+        const float fy1 = ceilf((1.0f - cny + rny) / sy - 0.5f) + 1.0f;
+        const int x0 = (int)fminf(fmaxf(fx0, 0.0f), W), x1 = (int)fmaxf(fminf(fx1, W - 1.0f), -1.0f);
+        const int y0 = (int)fminf(fmaxf(fy0, 0.0f), H), y1 = (int)fmaxf(fminf(fy1, H - 1.0f), -1.0f);
+        for (int y = y0; y <= y1; ++y) {
+            const float yn = 1.0f - ((float)y + 0.5f) * sy;
+            const float v = (yn - cny) / rny;
+            for (int x = x0; x <= x1; ++x) {
+                const float xn = ((float)x + 0.5f) * sx - 1.0f;
+                const float u = (xn - cnx) / rnx;
+                const float r = sqrtf(u * u + v * v);
+                if (r > 1.0f) continue;
+                const float w = orc_render_exp(-10.0f * r * r);
+                const float d = base * w * zoom_scale;
+                fixed[(size_t)y * width + x] += (uint64_t)rint((double)d * 1099511627776.0);
+            }
+        }
+    }
+}
+
+static double f16_value(uint32_t h) {
+    uint32_t ex = (h >> 10) & 0x1Fu, mant = h & 0x3FFu;
+    double sgn = (h & 0x8000u) ? -1.0 : 1.0;
+    if (ex == 0) return sgn * ldexp((double)mant, -24);
+    if (ex == 31) return mant ? NAN : sgn * INFINITY;
+    return sgn * ldexp((double)(mant | 0x400u), (int)ex - 25);
+}
+
+static uint32_t unorm8(double c) {
+    double q = rint(c * 255.0);
+    return (uint32_t)(q < 0.0 ? 0.0 : (q > 255.0 ? 255.0 : q));
+}
+
+static double mixd(double a, double b, double t) { return a * (1.0 - t) + b * t; }
+
+void orc_tone_table(uint32_t *table) {
+    /* tone_map.wgsl:66-84 in binary64 with the shader's binary32 constants */
+    const double s1 = (double)0.33f, s2 = (double)0.66f, w3 = (double)0.34f;
+    const double c1[3] = {(double)0.10f, (double)0.20f, (double)0.90f};
+    const double c2[3] = {(double)0.20f, (double)0.90f, (double)0.90f};
+    const double gamma = (double)(float)(1.0 / 2.2);
+    for (uint32_t h = 0; h < 65536u; ++h) {
+        double x = (double)8.0f * f16_value(h);
+        double mapped = x / (1.0 + x);
+        if (!(mapped == mapped)) mapped = x > 0.0 ? 1.0 : 0.0;
+        mapped = fmin(fmax(mapped, 0.0), 1.0);
+        double t = pow(mapped, gamma);
+        uint32_t out = 255u << 24;
+        for (int k = 0; k < 3; ++k) {
+            double c;
+            if (t < s1)
+                c = mixd(0.0, c1[k], t / s1);
+            else if (t < s2)
+                c = mixd(c1[k], c2[k], (t - s1) / s1);
+            else
+                c = mixd(c2[k], 1.0, (t - s2) / w3);
+            out |= unorm8(c) << (8 * k);
+        }
+        table[h] = out;
+    }
+}
+
+void orc_render(int64_t n, const float *pos_xy, const float *radius, const float *cam4,
+                int32_t width, int32_t height, uint16_t *d16, uint8_t *rgba) {
+    size_t px = (size_t)width * (size_t)height;
+    uint64_t *fixed = (uint64_t *)malloc(8 * px);
+    uint32_t *table = (uint32_t *)malloc(4 * 65536);
+    orc_splat(n, pos_xy, radius, cam4, width, height, fixed);
+    orc_tone_table(table);
+    for (int32_t y = 0; y < height; ++y)
+        for (int32_t x = 0; x < width; ++x) {
+            size_t texel = (size_t)(height - 1 - y) * width + x; /* tone_map.wgsl:58-70 uv flip */
+            uint16_t h = orc_fixed40_to_f16(fixed[texel]);
+            if (d16) d16[texel] = h;
+            uint32_t c = table[h];
+            uint8_t *o = rgba + 4 * ((size_t)y * width + x);
+            o[0] = (uint8_t)c;
+            o[1] = (uint8_t)(c >> 8);
+            o[2] = (uint8_t)(c >> 16);
+            o[3] = (uint8_t)(c >> 24);
+        }
+    free(fixed);
+    free(table);
+}

@@ -128,6 +128,22 @@ void orc_direct_sum_f64(int64_t n, const float *pos_xy, const float *mass,
                         const orc_params *p, const int64_t *targets, int64_t ntargets,
                         double *acc_xy, int threads);
 
+/* ---- renderer (renderer.ts:293-329, density.wgsl, tone_map.wgsl) ----
+ * cam4 = {center.x, center.y, half.x, half.y}; images are [height][width].
+ * orc_splat: every body's quad of 2R x 2R pixels (R = 2 radius) adds
+ * base * exp(-10 r^2) * zoom_scale (density.wgsl:77-96) at each pixel centre
+ * with r <= 1, in fixed point (units of 2^-40), texture rows (row 0 = NDC
+ * y +1).  orc_tone_table: RGBA8 (R in the low byte) of every fp16 density.
+ * orc_render: fp16 density (texture rows) and RGBA (canvas rows: row y
+ * shows texture row height-1-y, tone_map.wgsl:58-70). */
+float orc_render_exp(float x);
+uint16_t orc_fixed40_to_f16(uint64_t v);
+void orc_splat(int64_t n, const float *pos_xy, const float *radius, const float *cam4,
+               int32_t width, int32_t height, uint64_t *fixed);
+void orc_tone_table(uint32_t *table);
+void orc_render(int64_t n, const float *pos_xy, const float *radius, const float *cam4,
+                int32_t width, int32_t height, uint16_t *d16, uint8_t *rgba);
+
 #ifdef __cplusplus
 }
 #endif

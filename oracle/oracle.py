@@ -289,3 +289,50 @@ def walk_lists(idx, cm, left, right, mass, pos, params: Params):
             return off, nodes[:total]
         cap = int(total)
         nodes = np.zeros(cap, np.uint32)
+
+
+# ---------------------------------------------------------------- renderer
+
+def render_exp(x):
+    """orc_render_exp over an array of float32 values."""
+    f = lib().orc_render_exp
+    f.restype = ctypes.c_float
+    f.argtypes = [ctypes.c_float]
+    x = np.asarray(x, np.float32).ravel()
+    return np.array([f(float(v)) for v in x], np.float32)
+
+
+def fixed40_to_f16(v):
+    """fp16 bits of v * 2^-40 (round to nearest even), for an array of uint64."""
+    f = lib().orc_fixed40_to_f16
+    f.restype = ctypes.c_uint16
+    f.argtypes = [ctypes.c_uint64]
+    return np.array([f(int(u)) for u in np.asarray(v, np.uint64).ravel()], np.uint16)
+
+
+def splat(pos, radius, cam4, width, height):
+    pos = _f32(pos, (-1, 2))
+    radius = _f32(radius)
+    cam4 = _f32(cam4)
+    out = np.zeros((height, width), np.uint64)
+    lib().orc_splat(ctypes.c_int64(pos.shape[0]), _p(pos), _p(radius), _p(cam4),
+                    ctypes.c_int32(width), ctypes.c_int32(height), _p(out))
+    return out
+
+
+def tone_table():
+    t = np.zeros(65536, np.uint32)
+    lib().orc_tone_table(_p(t))
+    return t
+
+
+def render(pos, radius, cam4, width, height):
+    """(fp16 density bits [h][w] in texture rows, RGBA uint8 [h][w][4])."""
+    pos = _f32(pos, (-1, 2))
+    radius = _f32(radius)
+    cam4 = _f32(cam4)
+    d16 = np.zeros((height, width), np.uint16)
+    rgba = np.zeros((height, width, 4), np.uint8)
+    lib().orc_render(ctypes.c_int64(pos.shape[0]), _p(pos), _p(radius), _p(cam4),
+                     ctypes.c_int32(width), ctypes.c_int32(height), _p(d16), _p(rgba))
+    return d16, rgba

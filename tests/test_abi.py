@@ -140,3 +140,43 @@ def test_generate_radius_known_answers():
     with pytest.raises(ValueError):
         nbody_sim.generate_radius("nope", 3)
     assert _lib.lib().nbody_generate_radius(99, 3, None) == _lib.ERR_INVALID_ARG
+
+
+def test_write_png_round_trip(tmp_path):
+    """nbody_write_png: PNG signature, IHDR (8-bit RGBA), chunk CRCs, and the
+    decompressed scanlines (filter byte 0) equal the image."""
+    import struct
+    import zlib
+
+    from nbody_sim.render import write_png
+
+    rgba = np.random.default_rng(0).integers(0, 256, (37, 53, 4), dtype=np.uint8)
+    path = tmp_path / "x.png"
+    write_png(path, rgba)
+    data = path.read_bytes()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, {}
+    while pos < len(data):
+        (length,) = struct.unpack(">I", data[pos:pos + 4])
+        ctype = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + length]
+        (crc,) = struct.unpack(">I", data[pos + 8 + length:pos + 12 + length])
+        assert crc == zlib.crc32(ctype + body) & 0xFFFFFFFF
+        chunks.setdefault(ctype, b"")
+        chunks[ctype] += body
+        pos += 12 + length
+    w, h, depth, ctype, comp, filt, inter = struct.unpack(">IIBBBBB", chunks[b"IHDR"])
+    assert (w, h, depth, ctype, comp, filt, inter) == (53, 37, 8, 6, 0, 0, 0)
+    raw = np.frombuffer(zlib.decompress(chunks[b"IDAT"]), np.uint8).reshape(37, 1 + 53 * 4)
+    assert np.all(raw[:, 0] == 0)
+    assert np.array_equal(raw[:, 1:].reshape(37, 53, 4), rgba)
+    assert b"IEND" in chunks
+    assert _lib.lib().nbody_write_png(str(tmp_path / "y.png").encode(), 0, 5, None) == _lib.ERR_INVALID_ARG
+
+
+def test_default_camera():
+    """renderer.ts:49-50, 240-243: centre 0, half height 10, half width 10 * aspect."""
+    c = _lib.NbodyCamera()
+    assert _lib.lib().nbody_default_camera(1920, 1080, ctypes.byref(c)) == 0
+    assert (c.center_x, c.center_y, c.half_y, c.width, c.height) == (0, 0, 10, 1920, 1080)
+    assert c.half_x == np.float32(10 * np.float32(1920 / 1080)) or abs(c.half_x - 10 * 1920 / 1080) < 1e-5

@@ -122,3 +122,34 @@ process.stdout.write(JSON.stringify(Array.from(r.slice(0, 4))));
     sim.close()
     assert np.array_equal(raw[:4000].reshape(-1, 2), pos)
     assert np.array_equal(raw[4000:].reshape(-1, 2), vel)
+
+
+@pytest.mark.gpu
+def test_node_renderer_matches_python(tmp_path):
+    """The Node Renderer (renderer.js) draws the same frame as the Python one."""
+    f = str(tmp_path / "frame.bin")
+    png = str(tmp_path / "frame.png")
+    run_node(f"""
+const fs = require('fs');
+const s = require('./simulation.js');
+const {{ Renderer }} = require('./renderer.js');
+const sim = new s.Simulation(s.getDefaultConfig(), 0, 2);
+sim.setNumBodies(5000);
+sim.setScenario('planetary-disk');
+const r = new Renderer(sim, 160, 90);
+r.zoomCamera(2.0, 80, 45);
+const fr = r.getCommands();
+r.writePng('{png}');
+fs.writeFileSync('{f}', Buffer.from(fr.rgba.buffer));
+""")
+    from nbody_sim import render
+
+    sim = nbody_sim.Simulation(nbody_sim.getDefaultConfig(), seed=2)
+    sim.setNumBodies(5000)
+    sim.setScenario("planetary-disk")
+    r = render.Renderer(sim, 160, 90)
+    r.zoomCamera(2.0, 80, 45)
+    rgba = r.getCommands()
+    sim.close()
+    assert np.array_equal(np.fromfile(f, np.uint8).reshape(90, 160, 4), rgba)
+    assert os.path.getsize(png) > 100

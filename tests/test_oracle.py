@@ -379,3 +379,109 @@ def test_force_error_shrinks_with_theta():
         e = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
         med.append(np.median(e))
     assert all(a > b for a, b in zip(med, med[1:]))
+
+
+# ---------------------------------------------------------------- renderer
+
+def test_render_exp_against_exp():
+    """orc_render_exp: within 1e-6 relative of exp on the splat's [-10, 0],
+    exactly 1 at 0, and monotone."""
+    x = np.linspace(-10.0, 0.0, 20001).astype(np.float32)
+    got = O.render_exp(x).astype(np.float64)
+    want = np.exp(x.astype(np.float64))
+    assert np.max(np.abs(got - want) / want) < 1e-6
+    assert O.render_exp([0.0])[0] == 1.0
+    assert np.all(np.diff(got) >= 0)
+
+
+def test_fixed_to_f16_matches_numpy_rounding():
+    """Fixed point (2^-40 units) -> fp16 bits, round to nearest even, against
+    numpy's float64 -> float16 conversion (exact values below 2^53):
+    subnormals, the normal range, ties and overflow to inf."""
+    rng = np.random.default_rng(3)
+    v = np.concatenate([
+        np.array([0, 1, 2 ** 15, 2 ** 16, 2 ** 16 + 2 ** 15, 3 * 2 ** 15, 2 ** 26, 2 ** 40,
+                  65504 * 2 ** 40, 65519 * 2 ** 40, 65520 * 2 ** 40, 2 ** 52], np.uint64),
+        rng.integers(0, 2 ** 30, 2000, dtype=np.uint64),
+        rng.integers(0, 2 ** 52, 2000, dtype=np.uint64),
+        (rng.integers(1, 2 ** 11, 500, dtype=np.uint64) << np.uint64(30)) + np.uint64(2 ** 29),
+    ])
+    got = O.fixed40_to_f16(v)
+    with np.errstate(over="ignore"):
+        want = (v.astype(np.float64) * 2.0 ** -40).astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+
+
+def _cam(w, h, cx=0.0, cy=0.0, half_y=10.0):
+    return np.array([cx, cy, half_y * w / h, half_y], np.float32)
+
+
+def test_splat_single_body_footprint():
+    """A body at a pixel centre: R = 2 px covers the 13 pixel centres with
+    dx^2 + dy^2 <= 4; the centre holds base * zoom_scale exactly, the
+    footprint is symmetric, and every pixel holds base * exp(-10 r^2) *
+    zoom_scale to within the exp's 1e-6 (density.wgsl:77-96)."""
+    w = h = 64
+    cam = _cam(w, h)
+    # pixel (20, 40) centre in world coordinates (texture rows: y_ndc = 1 - (row + .5) 2/h)
+    xn = (20 + 0.5) * 2 / w - 1
+    yn = 1 - (40 + 0.5) * 2 / h
+    pos = np.array([[xn * cam[2], yn * cam[3]]], np.float32)
+    zoom_scale = np.float32(1) / (np.float32(1) + np.float32(0.05) * np.float32(max(cam[2], cam[3])))
+    for rm, base in ((1.0, 0.02), (3.0, 0.1)):
+        fx = O.splat(pos, [rm], cam, w, h).astype(np.float64) * 2.0 ** -40
+        R = 2 * rm
+        ys, xs = np.nonzero(fx)
+        dx, dy = xs - 20, ys - 40
+        assert np.all(dx * dx + dy * dy <= R * R)
+        assert len(xs) == np.sum([(a * a + b * b <= R * R) for a in range(-8, 9) for b in range(-8, 9)])
+        assert fx[40, 20] == np.float64(np.float32(base) * zoom_scale)
+        sym = fx[40 - 8:40 + 9, 20 - 8:20 + 9]
+        assert np.array_equal(sym, sym[::-1, :]) and np.array_equal(sym, sym[:, ::-1])
+        r2 = (dx * dx + dy * dy) / R ** 2
+        want = base * np.exp(-10 * r2) * float(zoom_scale)
+        assert np.max(np.abs(fx[ys, xs] - want) / want) < 2e-6
+
+
+def test_splat_is_additive_and_rows_flip():
+    """Fixed-point sums: splat(A + B) = splat(A) + splat(B) exactly (order of
+    bodies irrelevant); in the RGBA frame a body above the camera centre
+    (world y > 0, texture rows near the top) appears in the lower half."""
+    rng = np.random.default_rng(5)
+    w, h = 96, 48
+    cam = _cam(w, h)
+    pa = rng.uniform(-9, 9, (300, 2)).astype(np.float32)
+    pb = rng.uniform(-9, 9, (200, 2)).astype(np.float32)
+    ra, rb = np.ones(300, np.float32), rng.uniform(0.5, 4, 200).astype(np.float32)
+    sa, sb = O.splat(pa, ra, cam, w, h), O.splat(pb, rb, cam, w, h)
+    both = O.splat(np.concatenate([pb, pa]), np.concatenate([rb, ra]), cam, w, h)
+    assert np.array_equal(both, sa + sb)
+    top = np.array([[0.0, 6.0]], np.float32)
+    d16, rgba = O.render(top, [2.0], cam, w, h)
+    rows = np.nonzero(d16.any(axis=1))[0]
+    assert rows.max() < h // 2
+    lit = np.nonzero(rgba[:, :, :3].any(axis=(1, 2)))[0]
+    assert lit.min() >= h // 2 and np.array_equal(np.sort(h - 1 - rows), lit)
+
+
+def test_tone_table_known_answers():
+    """tone_map.wgsl:66-84: black at 0, white at the largest densities and
+    inf, opaque, every channel non-decreasing in the density (the ramp's
+    stops are channel-wise ordered), and the ramp's stops where the gamma-
+    mapped value crosses 0.33 and 0.66."""
+    t = O.tone_table()
+    rgba = np.stack([(t >> s) & 0xFF for s in (0, 8, 16, 24)], axis=1)
+    assert tuple(rgba[0]) == (0, 0, 0, 255)
+    assert tuple(rgba[0x7BFF]) == (255, 255, 255, 255)  # 65504
+    assert tuple(rgba[0x7C00]) == (255, 255, 255, 255)  # +inf
+    assert np.all(rgba[:, 3] == 255)
+    pos = np.arange(0, 0x7C01)  # 0 .. +inf in increasing order of value
+    assert np.all(np.diff(rgba[pos, :3].astype(int), axis=0) >= 0)
+    vals = np.arange(0x7C00, dtype=np.uint16).view(np.float16).astype(np.float64)
+    x = 8.0 * vals
+    g = (x / (1 + x)) ** float(np.float32(1 / 2.2))
+    c1 = np.rint(255 * np.float32([0.1, 0.2, 0.9]).astype(np.float64))
+    c2 = np.rint(255 * np.float32([0.2, 0.9, 0.9]).astype(np.float64))
+    k1 = np.argmax(g >= float(np.float32(0.33)))
+    k2 = np.argmax(g >= float(np.float32(0.66)))
+    assert np.all(np.abs(rgba[k1, :3] - c1) <= 1) and np.all(np.abs(rgba[k2, :3] - c2) <= 1)

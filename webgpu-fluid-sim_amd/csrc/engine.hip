@@ -27,6 +27,7 @@
 #include <mutex>
 #include <sstream>
 #include <sys/stat.h>
+#include <zlib.h>
 #include <string>
 #include <vector>
 
@@ -94,6 +95,13 @@ struct nbody_sim {
     uint32_t *sim_root = nullptr;
     float2 *accel = nullptr;
     float *radius = nullptr;  // per-body radius multipliers (renderer input)
+    // renderer: fixed-point density, fp16 density, RGBA, tone table (one allocation)
+    void *render_base = nullptr;
+    int render_w = 0, render_h = 0;
+    unsigned long long *render_dens = nullptr;
+    uint16_t *render_d16 = nullptr;
+    uint32_t *render_rgba = nullptr, *render_lut = nullptr;
+    bool render_lut_ready = false;
     unsigned long long *stats = nullptr;
     uint2 *body_counts = nullptr;  // per-body walk counters (stats mode)
     uint32_t *overflow = nullptr;
@@ -597,6 +605,7 @@ int nbody_destroy(nbody_sim *s) {
     release_graph(s);
     if (s->arena.base) (void)hipFree(s->arena.base);
     if (s->adj_base) (void)hipFree(s->adj_base);
+    if (s->render_base) (void)hipFree(s->render_base);
     for (auto &ev : s->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
@@ -803,6 +812,12 @@ int nbody_debug_get(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
         case NBODY_BUF_WALK_COUNTS:
             src = s->body_counts;
             need = n * 8;
+            break;
+        case NBODY_BUF_TONE_TABLE:
+            if (!s->render_lut_ready)
+                return fail(s, NBODY_ERR_STATE, "no tone table before the first nbody_render");
+            src = s->render_lut;
+            need = 65536 * 4;
             break;
         default:
             return fail(s, NBODY_ERR_INVALID_ARG, "unknown buffer id");
@@ -1396,6 +1411,124 @@ int nbody_load_snapshot(nbody_sim *s, const char *dir) {
         s->result_buf = 0;
     }
     s->steps_taken = (int64_t)steps_d;
+    return NBODY_OK;
+}
+
+int nbody_default_camera(int32_t width, int32_t height, nbody_camera *out) {
+    if (!out || width < 1 || height < 1)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "camera needs width, height >= 1");
+    // renderer.ts:49-50, 240-243: centre (0, 0), half height 10, half width
+    // 10 * aspect
+    out->center_x = 0.0f;
+    out->center_y = 0.0f;
+    out->half_y = 10.0f;
+    out->half_x = 10.0f * ((float)width / (float)height);
+    out->width = width;
+    out->height = height;
+    return NBODY_OK;
+}
+
+int nbody_render(nbody_sim *s, const nbody_camera *cam, uint16_t *density_f16, uint8_t *rgba) {
+    if (!s || !cam) return fail(s, NBODY_ERR_INVALID_ARG, "NULL argument");
+    if (cam->width < 1 || cam->height < 1 || cam->width > 16384 || cam->height > 16384)
+        return fail(s, NBODY_ERR_INVALID_ARG, "viewport must be 1..16384 pixels per side");
+    if (!(cam->half_x > 0.0f) || !(cam->half_y > 0.0f))
+        return fail(s, NBODY_ERR_INVALID_ARG, "camera half sizes must be > 0");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const int64_t px = (int64_t)cam->width * cam->height;
+    if (cam->width != s->render_w || cam->height != s->render_h) {
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        if (s->render_base) HIP_TRY(s, hipFree(s->render_base));
+        s->render_base = nullptr;
+        Arena a;
+        for (int pass = 0; pass < 2; ++pass) {
+            a.used = 0;
+            s->render_dens = a.take<unsigned long long>(px);
+            s->render_d16 = a.take<uint16_t>(px);
+            s->render_rgba = a.take<uint32_t>(px);
+            s->render_lut = a.take<uint32_t>(65536);
+            if (pass == 0) {
+                void *p = nullptr;
+                HIP_TRY(s, hipMalloc(&p, a.used));
+                a.base = static_cast<char *>(p);
+                s->render_base = p;
+            }
+        }
+        s->render_w = cam->width;
+        s->render_h = cam->height;
+        s->render_lut_ready = false;
+    }
+    RenderParams rp;
+    rp.cam_x = cam->center_x;
+    rp.cam_y = cam->center_y;
+    rp.half_x = cam->half_x;
+    rp.half_y = cam->half_y;
+    rp.width = cam->width;
+    rp.height = cam->height;
+    const float zoom = std::max(cam->half_x, cam->half_y);  // density.wgsl:86-88
+    rp.zoom_scale = 1.0f / (1.0f + 0.05f * zoom);
+    launch_render(s->pos, s->radius, (uint32_t)s->n, rp, s->render_dens, s->render_lut,
+                  !s->render_lut_ready, s->render_d16, s->render_rgba, s->stream);
+    HIP_TRY(s, hipGetLastError());
+    s->render_lut_ready = true;
+    if (density_f16)
+        HIP_TRY(s, hipMemcpyAsync(density_f16, s->render_d16, 2 * (size_t)px,
+                                  hipMemcpyDeviceToHost, s->stream));
+    if (rgba)
+        HIP_TRY(s, hipMemcpyAsync(rgba, s->render_rgba, 4 * (size_t)px, hipMemcpyDeviceToHost,
+                                  s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NBODY_OK;
+}
+
+int nbody_write_png(const char *path, int32_t width, int32_t height, const uint8_t *rgba) {
+    if (!path || !rgba || width < 1 || height < 1 || width > 65535 || height > 65535)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "nbody_write_png: bad arguments");
+    // scanlines with filter type 0 (none) in front of each row
+    const size_t row = 4 * (size_t)width;
+    std::vector<unsigned char> raw((row + 1) * (size_t)height);
+    for (int32_t y = 0; y < height; ++y) {
+        raw[(row + 1) * (size_t)y] = 0;
+        std::memcpy(&raw[(row + 1) * (size_t)y + 1], rgba + row * (size_t)y, row);
+    }
+    uLongf zlen = compressBound((uLong)raw.size());
+    std::vector<unsigned char> z(zlen);
+    if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 6) != Z_OK)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "nbody_write_png: zlib failed");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return fail(nullptr, NBODY_ERR_INVALID_ARG, std::string("cannot write ") + path);
+    auto be32 = [](unsigned char *p, uint32_t v) {
+        p[0] = (unsigned char)(v >> 24);
+        p[1] = (unsigned char)(v >> 16);
+        p[2] = (unsigned char)(v >> 8);
+        p[3] = (unsigned char)v;
+    };
+    auto chunk = [&](const char *type, const unsigned char *data, uint32_t len) {
+        unsigned char hdr[8];
+        be32(hdr, len);
+        std::memcpy(hdr + 4, type, 4);
+        f.write(reinterpret_cast<const char *>(hdr), 8);
+        if (len) f.write(reinterpret_cast<const char *>(data), len);
+        uLong crc = crc32(0L, reinterpret_cast<const Bytef *>(type), 4);
+        if (len) crc = crc32(crc, data, len);
+        unsigned char c[4];
+        be32(c, (uint32_t)crc);
+        f.write(reinterpret_cast<const char *>(c), 4);
+    };
+    const unsigned char sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    f.write(reinterpret_cast<const char *>(sig), 8);
+    unsigned char ihdr[13];
+    be32(ihdr, (uint32_t)width);
+    be32(ihdr + 4, (uint32_t)height);
+    ihdr[8] = 8;   // bits per channel
+    ihdr[9] = 6;   // RGBA
+    ihdr[10] = 0;  // deflate
+    ihdr[11] = 0;  // adaptive filtering (each row says none)
+    ihdr[12] = 0;  // no interlace
+    chunk("IHDR", ihdr, 13);
+    chunk("IDAT", z.data(), (uint32_t)zlen);
+    chunk("IEND", nullptr, 0);
+    if (!f) return fail(nullptr, NBODY_ERR_INVALID_ARG, std::string("cannot write ") + path);
     return NBODY_OK;
 }
 

@@ -214,6 +214,22 @@ void launch_direct_f64(const float2 *pos, const float *mass, int64_t n, const in
                        int64_t ntargets, const StepParams &p, double2 *partial, int chunks,
                        double2 *out, hipStream_t s);
 
+// ---------------------------------------------------------------- render
+// Camera and viewport of the density splat (the Renderer's Metadata,
+// density.wgsl:3-10) and the splat's zoom factor 1 / (1 + 0.05 max(half)).
+struct RenderParams {
+    float cam_x, cam_y, half_x, half_y;
+    int width, height;
+    float zoom_scale;
+};
+// Clears dens (u64 fixed point, 2^-40 units, [height][width] texture rows),
+// builds the tone table lut[65536] when build_lut, splats n bodies, then
+// writes the fp16 density (d16, texture rows; may be null) and the RGBA8
+// image (rgba, output rows: row y shows texture row height-1-y).
+void launch_render(const float2 *pos, const float *radius, uint32_t n, const RenderParams &rp,
+                   unsigned long long *dens, uint32_t *lut, bool build_lut, uint16_t *d16,
+                   uint32_t *rgba, hipStream_t s);
+
 // ---------------------------------------------------------------- misc
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s);
 void launch_fill_f32(float *p, float value, int64_t count, hipStream_t s);

@@ -41,6 +41,7 @@ BUF_ACCEL = 7
 BUF_BBOX = 8
 BUF_SIM_ROOTS = 9
 BUF_WALK_COUNTS = 10
+BUF_TONE_TABLE = 11
 
 SCENARIO_DEFAULT = 0
 SCENARIO_TWO_CLUSTERS = 1
@@ -64,6 +65,7 @@ EXPORTS = (
     "nbody_adjoint_backward", "nbody_step_static", "nbody_direct_accel_f64",
     "nbody_set_radius_multiplier", "nbody_get_radius_multiplier", "nbody_get_step_count",
     "nbody_save_snapshot", "nbody_load_snapshot", "nbody_generate_radius",
+    "nbody_default_camera", "nbody_render", "nbody_write_png",
 )
 
 # Largest body count of nbody_step_static (its fixed device scratch).
@@ -93,6 +95,17 @@ class NbodyTuning(ctypes.Structure):
         ("xcd_order", ctypes.c_int32),
         ("lane_bodies", ctypes.c_int32),
         ("radix_bits", ctypes.c_int32),
+    ]
+
+
+class NbodyCamera(ctypes.Structure):
+    _fields_ = [
+        ("center_x", ctypes.c_float),
+        ("center_y", ctypes.c_float),
+        ("half_x", ctypes.c_float),
+        ("half_y", ctypes.c_float),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
     ]
 
 
@@ -164,6 +177,9 @@ def lib() -> ctypes.CDLL:
         "nbody_save_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "nbody_load_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "nbody_generate_radius": (ctypes.c_int, [i32, i64, vp]),
+        "nbody_default_camera": (ctypes.c_int, [i32, i32, ctypes.POINTER(NbodyCamera)]),
+        "nbody_render": (ctypes.c_int, [vp, ctypes.POINTER(NbodyCamera), vp, vp]),
+        "nbody_write_png": (ctypes.c_int, [ctypes.c_char_p, i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

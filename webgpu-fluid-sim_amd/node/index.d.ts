@@ -47,3 +47,20 @@ export declare class Simulation {
 }
 
 export declare function version(): string;
+
+// renderer.js: the reference's Renderer (src/renderer.ts) over nbody_render.
+export declare class Renderer {
+    constructor(sim: Simulation, width?: number, height?: number);
+    camCenter: [number, number];
+    camHalfSize: [number, number];
+    viewPort: [number, number];
+    rebindSimBuffers(sim: Simulation): void;
+    setNumBodies(numBodies: number): void;
+    resize(width: number, height: number): void;
+    deltaPxToDeltaWorld(deltaPx: number, deltaPy: number): [number, number];
+    canvasPxToWorld(px: number, py: number): [number, number];
+    panCamera(deltaPx: number, deltaPy: number): void;
+    zoomCamera(zoomFactor: number, zoomCenterX: number, zoomCenterY: number): void;
+    getCommands(): { rgba: Uint8Array; density: Uint16Array };
+    writePng(file: string): void;
+}

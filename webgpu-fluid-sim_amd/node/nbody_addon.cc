@@ -290,6 +290,58 @@ std::string str(napi_env env, napi_value v) {
     return out;
 }
 
+// render(handle, {centerX, centerY, halfX, halfY, width, height}) ->
+// {rgba: Uint8Array(4wh), density: Uint16Array(wh)} (renderer.ts:293-329).
+napi_value Render(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    nbody_sim *h = unwrap(env, argv[0]);
+    nbody_camera c;
+    c.center_x = (float)prop(env, argv[1], "centerX", 0.0);
+    c.center_y = (float)prop(env, argv[1], "centerY", 0.0);
+    c.half_x = (float)prop(env, argv[1], "halfX", 10.0);
+    c.half_y = (float)prop(env, argv[1], "halfY", 10.0);
+    c.width = (int32_t)prop(env, argv[1], "width", 0);
+    c.height = (int32_t)prop(env, argv[1], "height", 0);
+    if (c.width < 1 || c.height < 1 || c.width > 16384 || c.height > 16384) {
+        napi_throw_range_error(env, nullptr, "width and height must be 1..16384");
+        return nullptr;
+    }
+    const size_t px = (size_t)c.width * (size_t)c.height;
+    napi_value ab8, ab16, rgba, dens;
+    void *p8 = nullptr, *p16 = nullptr;
+    NAPI_CALL(env, napi_create_arraybuffer(env, 4 * px, &p8, &ab8));
+    NAPI_CALL(env, napi_create_arraybuffer(env, 2 * px, &p16, &ab16));
+    NAPI_CALL(env, napi_create_typedarray(env, napi_uint8_array, 4 * px, ab8, 0, &rgba));
+    NAPI_CALL(env, napi_create_typedarray(env, napi_uint16_array, px, ab16, 0, &dens));
+    int rc = nbody_render(h, &c, static_cast<uint16_t *>(p16), static_cast<uint8_t *>(p8));
+    if (rc) return throw_nbody(env, rc, h);
+    napi_value o;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    napi_set_named_property(env, o, "rgba", rgba);
+    napi_set_named_property(env, o, "density", dens);
+    return o;
+}
+
+// writePng(path, width, height, rgba: Uint8Array)
+napi_value WritePng(napi_env env, napi_callback_info info) {
+    napi_value argv[4];
+    if (!get_args(env, info, 4, argv)) return nullptr;
+    const int32_t w = (int32_t)num(env, argv[1], 0), hgt = (int32_t)num(env, argv[2], 0);
+    napi_typedarray_type type;
+    size_t length = 0, offset = 0;
+    void *data = nullptr;
+    napi_value ab;
+    if (napi_get_typedarray_info(env, argv[3], &type, &length, &data, &ab, &offset) != napi_ok ||
+        type != napi_uint8_array || w < 1 || hgt < 1 || length != 4 * (size_t)w * (size_t)hgt) {
+        napi_throw_type_error(env, nullptr, "writePng(path, width, height, rgba: Uint8Array(4wh))");
+        return nullptr;
+    }
+    int rc = nbody_write_png(str(env, argv[0]).c_str(), w, hgt, static_cast<const uint8_t *>(data));
+    if (rc) return throw_nbody(env, rc, nullptr);
+    return nullptr;
+}
+
 napi_value SaveSnapshot(napi_env env, napi_callback_info info) {
     napi_value argv[2];
     if (!get_args(env, info, 2, argv)) return nullptr;
@@ -366,6 +418,8 @@ napi_value Init(napi_env env, napi_value exports) {
         {"getRadius", nullptr, GetRadius, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"generateRadius", nullptr, GenerateRadius, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"saveSnapshot", nullptr, SaveSnapshot, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"render", nullptr, Render, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"writePng", nullptr, WritePng, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"loadSnapshot", nullptr, LoadSnapshot, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"getNumBodies", nullptr, GetNumBodies, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"generate", nullptr, Generate, nullptr, nullptr, nullptr, napi_default, nullptr},
