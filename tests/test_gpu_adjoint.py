@@ -210,3 +210,55 @@ def test_autograd_function_wrapper():
     assert np.allclose(x0.grad.numpy(), gx, rtol=1e-4, atol=1e-6 * np.abs(gx).max())
     assert np.allclose(m0.grad.numpy(), gm, rtol=1e-4, atol=1e-6 * np.abs(gm).max())
     assert np.all(np.isfinite(gv))
+
+
+def test_adjoint_full_size_c5_segment():
+    """The C5 configuration at its full size (65,536-body exponential disk,
+    theta 0.6, fast mode, the bench's bodies) over a 5-step segment with
+    checkpoints every 2 steps, against torch autograd of the FP64
+    frozen-decision trajectory."""
+    from nbody_sim import workloads as W
+    from oracle import oracle as O
+
+    w = W.WORKLOADS["adjoint_65k"]
+    mass, pos, vel = w.bodies()
+    steps, every = 5, 2
+    cfg = w.config("fast")
+    P = O.Params(grav_constant=1.0, delta_time=w.dt, bh_theta=w.theta, epsilon=w.eps,
+                 softening_mode=O.SOFTEN_CONSTANT, morton_mode=O.MORTON_BBOX30)
+    rng = np.random.default_rng(7)
+    wx = rng.normal(size=(w.n, 2)).astype(np.float32)
+    wv = (0.3 * rng.normal(size=(w.n, 2))).astype(np.float32)
+    (gx, gv, gm), _, _ = gpu_grads(cfg, mass, pos, vel, steps, every, wx, wv)
+    F = FrozenTrajectory(mass, pos, vel, P, steps)
+    x0 = torch.from_numpy(pos.astype(np.float64)).requires_grad_()
+    v0 = torch.from_numpy(vel.astype(np.float64)).requires_grad_()
+    m0 = torch.from_numpy(mass.astype(np.float64)).requires_grad_()
+    xt, vt = F.run(x0, v0, m0)
+    L = (torch.from_numpy(wx.astype(np.float64)) * xt).sum() + \
+        (torch.from_numpy(wv.astype(np.float64)) * vt).sum()
+    rx, rv, rm = (t.numpy() for t in torch.autograd.grad(L, (x0, v0, m0)))
+    ex, ev, em = rel(gx, rx), rel(gv, rv), rel(gm, rm)
+    print(f"adjoint 65k x 5 steps vs FP64 autograd: pos {ex:.2e} vel {ev:.2e} mass {em:.2e}")
+    assert ex < 1e-3 and ev < 1e-3 and em < 1e-3
+
+
+def test_adjoint_200_steps_checkpoint_spacing():
+    """The bench's 200-step, 65,536-body run: gradients with a checkpoint every
+    20 steps equal those with a checkpoint every step, within the spread of
+    two runs with the same spacing (the node cotangents are summed with float
+    atomics, whose order varies)."""
+    from nbody_sim import workloads as W
+
+    w = W.WORKLOADS["adjoint_65k"]
+    mass, pos, vel = w.bodies()
+    cfg = w.config("fast")
+    g = np.random.default_rng(0).normal(size=(w.n, 2)).astype(np.float32)
+    runs = [gpu_grads(cfg, mass, pos, vel, 200, every, g, None)[0] for every in (20, 20, 1)]
+    for k in range(3):
+        assert np.all(np.isfinite(runs[0][k]))
+        floor = rel(runs[1][k], runs[0][k])
+        diff = rel(runs[2][k], runs[0][k])
+        print(f"gradient {('pos', 'vel', 'mass')[k]}: every 20 twice {floor:.2e}, "
+              f"every 20 vs every 1 {diff:.2e}")
+        assert diff < 1e-4 and diff <= 10 * floor + 1e-7

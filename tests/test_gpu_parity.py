@@ -215,6 +215,35 @@ def test_step_bit_exact_800k():
     assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
 
 
+def test_step_bit_exact_4M_collision():
+    """The 4M-body collision (128 super-blocks of 32768 leaves, the top list
+    resolved beside 128 phase-2 workgroups): tree, accelerations and state
+    equal the oracle bit for bit on the second step."""
+    from nbody_sim import workloads as W
+
+    w = W.WORKLOADS["collision_4M"]
+    mass, pos, vel = w.bodies()
+    cfg, P = params(theta=w.theta, dt=w.dt, eps=w.eps)
+    g = gpu_step(cfg, mass, pos, vel, nsteps=2)
+    r = oracle_step(P, mass, pos, vel, nsteps=2)
+    _compare_tree(g, r)
+    assert np.array_equal(g["accel"], r["accel"])
+    assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
+
+
+def test_own_leaf_term_with_tiny_softening():
+    """Constant softening so small that eps^-3 overflows binary32 (eps = 1e-14)
+    with masses small enough that m * eps^-3 would not: the fast walk must
+    leave the own leaf out instead of adding m * inf * 0 = NaN."""
+    mass, pos, vel = nbody_sim.generate("uniform-disk", 3000, seed=12)
+    mass = np.full_like(mass, 1e-13)
+    cfg, P = params(theta=0.5, dt=1e-4, eps=1e-14)
+    g = gpu_step(cfg, mass, pos, vel, math="fast")
+    r = oracle_step(P, mass, pos, vel)
+    assert np.all(np.isfinite(g["accel"])) and np.all(np.isfinite(g["pos"]))
+    assert np.median(rel_err(g["accel"], r["accel"])) < 1e-6
+
+
 def test_coincident_bodies():
     n = 500
     pos = np.zeros((n, 2), np.float32)

@@ -266,15 +266,17 @@ int allocate(nbody_sim *s, int64_t cap) {
 }
 
 // True when a body's own leaf adds an exact zero to its fast-mode sum: the
-// offset is exactly zero, and with constant softening 1/sqrt(eps^2) and
-// |m| * eps^-3 stay finite (far from overflow), so the term m * inv^3 * 0 is
-// a zero, never a NaN.
+// offset is exactly zero, and with constant softening 1/sqrt(eps^2), the
+// factor inv^3 = eps^-3 (formed before the mass multiply) and |m| * eps^-3
+// all stay finite (far from overflow), so the term m * inv^3 * 0 is a zero,
+// never a NaN.
 bool self_term_zero(const nbody_sim *s) {
     if (s->cfg.softening_mode != NBODY_SOFTEN_CONSTANT) return false;
     const double eps2 = (double)s->cfg.epsilon * (double)s->cfg.epsilon;
     const double m = (double)s->mass_absmax;
     if (!(eps2 >= 1e-30) || !(m <= 1e30)) return false;  // also false for NaN
-    return m * std::pow(eps2, -1.5) <= 1e30;
+    const double inv3 = std::pow(eps2, -1.5);
+    return inv3 <= 1e37 && m * inv3 <= 1e30;
 }
 
 // The queue walk's chunk order for an ensemble: the chunks of 64 sorted

@@ -541,7 +541,9 @@ __device__ __forceinline__ void wait_input(const TreeBufs &tb, uint32_t sb1, uin
     const uint32_t *flag = tb.counts + done_flag(sb1 - 1u);
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
         __builtin_amdgcn_s_sleep(2);
-    asm volatile("" ::: "memory");  // the record reads stay behind the flag
+    // acquire (agent scope): pairs with the producer's release fence before
+    // its flag store, so the records below are the producer's
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     V = load_device_f4(tb.pcm + kPairStride * (size_t)node + side);
     VB = load_device_f4(tb.paabb + 2 * (size_t)node + side);
 }
