@@ -50,6 +50,14 @@ extern "C" {
 #define NBODY_MATH_FAST 0
 #define NBODY_MATH_EXACT 1
 
+/* Expansion of an accepted node's pull: MONOPOLE is the reference's
+ * G m r / (|r|^2 + eps^2)^{3/2} (barnes_hut.wgsl:92-97); QUADRUPOLE adds the
+ * next term of the softened potential's Taylor series about the node's
+ * centre of mass, from its second mass moments (no reference counterpart;
+ * constant softening and the fast arithmetic only).  Same opening test. */
+#define NBODY_EXPANSION_MONOPOLE 0
+#define NBODY_EXPANSION_QUADRUPOLE 1
+
 /* Replaces `Config` (config.ts:1-7) plus the `Metadata` uniform that
  * updateMetadataBuffer() packs (simulation.ts:212-225, lbvh.wgsl:3-10). */
 typedef struct nbody_config {
@@ -64,6 +72,7 @@ typedef struct nbody_config {
     int32_t math_mode;        /* NBODY_MATH_*    */
     uint32_t offset;          /* Metadata.offset (simulation.ts:222, lbvh.wgsl:171); the
                                  race-free fill gives the same tree for every value */
+    int32_t expansion;        /* NBODY_EXPANSION_* (engine extension)                 */
 } nbody_config;
 
 typedef struct nbody_sim nbody_sim;
@@ -188,6 +197,8 @@ int nbody_set_debug(nbody_sim *sim, int32_t flags);
                                     NBODY_DEBUG_STATS (pair walks: walk_variant 2 and 4) */
 #define NBODY_BUF_TONE_TABLE 11  /* u32[65536] RGBA8 (R in the low byte) of every fp16
                                     density value, after the first nbody_render */
+#define NBODY_BUF_NODE_QUAD 12   /* f32[3*(2n-1)] {xx, xy, yy} central second mass moments of
+                                    every node (quadrupole expansion; leaves 0) */
 int nbody_debug_get(nbody_sim *sim, int32_t which, void *dst, int64_t bytes);
 
 /* Walk counters since the last reset: {accepted, opened, overflowed}. */

@@ -304,7 +304,7 @@ void orc_fill(int64_t n, const uint32_t *idx, const float *mass, const float *po
 void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32_t *left,
               const uint32_t *right, const uint32_t *roots, const float *mass,
               const float *pos_xy, const orc_params *p, int stack_size, float *accel_xy,
-              int64_t *stats, float *margin, float *term_norm, int threads) {
+              int64_t *stats, const float *quad, float *margin, float *term_norm, int threads) {
     if (n <= 0) return;
     const float g = p->grav_constant, dt = p->delta_time, theta = p->bh_theta;
     const float mult = p->epsilon_multiplier, epsc = p->epsilon;
@@ -351,8 +351,21 @@ void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32
             if (leaf || C[3] * inv < theta) {
                 float inv3 = inv * inv * inv;
                 float gm = g * m2;
-                ax = ax + gm * rx * inv3;
-                ay = ay + gm * ry * inv3;
+                if (quad) {
+                    /* quadrupole term of the softened potential about the
+                     * node's centre: -G inv^5 [(1.5 tr Q - 7.5 inv^2 r.Q r) r + 3 Q r] */
+                    const float *Q = quad + 3 * (size_t)node;
+                    float inv2 = inv * inv;
+                    float inv5 = (inv2 * inv2) * inv;
+                    float qrx = Q[0] * rx + Q[1] * ry, qry = Q[1] * rx + Q[2] * ry;
+                    float rqr = rx * qrx + ry * qry;
+                    float k = inv5 * (1.5f * (Q[0] + Q[2]) - 7.5f * inv2 * rqr);
+                    ax = ax + (gm * rx * inv3 - g * (k * rx + 3.0f * inv5 * qrx));
+                    ay = ay + (gm * ry * inv3 - g * (k * ry + 3.0f * inv5 * qry));
+                } else {
+                    ax = ax + gm * rx * inv3;
+                    ay = ay + gm * ry * inv3;
+                }
                 if (term_norm) l1 += fabsf(gm * inv3) * sqrtf(d2);
                 acc++;
             } else if (sp + 2 < ssz) {
@@ -470,7 +483,7 @@ static void one_step(int64_t n, const orc_params *p, const float *mass, float *p
     orc_build_u32(n, s->keys, s->left, s->right, s->parent);
     orc_fill(n, s->idx, mass, pos, s->left, s->right, s->parent, 0, 0, s->cm, s->bb);
     orc_walk(n, s->idx, s->cm, s->left, s->right, NULL, mass, pos, p, 256, s->acc, stats, NULL,
-             NULL, threads);
+             NULL, NULL, threads);
     orc_kick_drift(n, s->acc, p->delta_time, pos, vel);
 }
 
@@ -696,4 +709,61 @@ void orc_render(int64_t n, const float *pos_xy, const float *radius, const float
         }
     free(fixed);
     free(table);
+}
+
+/* ------------------------------------------------------------ quadrupole */
+
+void orc_node_quad(int64_t n, const uint32_t *idx, const float *mass, const float *pos_xy,
+                   const uint32_t *left, const uint32_t *right, float *quad) {
+    /* Central second mass moments {xx, xy, yy} of every node's leaf range,
+     * in binary64: centre of mass first, then the moments about it, each a
+     * sum over the range in sorted order.  Leaves: 0. */
+    int64_t nodes = 2 * n - 1;
+    if (n <= 0) return;
+    int64_t *lo = (int64_t *)malloc(8 * (size_t)nodes), *hi = (int64_t *)malloc(8 * (size_t)nodes);
+    int64_t *order = (int64_t *)malloc(8 * (size_t)nodes);
+    for (int64_t t = 0; t < n; ++t) lo[n - 1 + t] = hi[n - 1 + t] = t;
+    int64_t cnt = 0, sp = 0;
+    int64_t *stack = (int64_t *)malloc(8 * (size_t)nodes);
+    if (n > 1) stack[sp++] = 0;
+    while (sp) { /* pre-order of internal nodes; reversed = children first */
+        int64_t v = stack[--sp];
+        order[cnt++] = v;
+        if (left[v] < (uint32_t)(n - 1)) stack[sp++] = left[v];
+        if (right[v] < (uint32_t)(n - 1)) stack[sp++] = right[v];
+    }
+    for (int64_t k = cnt - 1; k >= 0; --k) {
+        int64_t v = order[k];
+        lo[v] = lo[left[v]];
+        hi[v] = hi[right[v]];
+    }
+    for (int64_t v = 0; v < nodes; ++v) {
+        float *Q = quad + 3 * (size_t)v;
+        Q[0] = Q[1] = Q[2] = 0.0f;
+        if (v >= n - 1 && n > 1) continue;
+        double m = 0.0, sx = 0.0, sy = 0.0;
+        for (int64_t t = lo[v]; t <= hi[v]; ++t) {
+            uint32_t b = idx[t];
+            double mb = mass[b], xb = pos_xy[2 * (size_t)b], yb = pos_xy[2 * (size_t)b + 1];
+            m += mb;
+            sx += mb * xb;
+            sy += mb * yb;
+        }
+        if (!(m > 0.0)) continue;
+        double cx = sx / m, cy = sy / m, qxx = 0.0, qxy = 0.0, qyy = 0.0;
+        for (int64_t t = lo[v]; t <= hi[v]; ++t) {
+            uint32_t b = idx[t];
+            double mb = mass[b], dx = pos_xy[2 * (size_t)b] - cx, dy = pos_xy[2 * (size_t)b + 1] - cy;
+            qxx += mb * dx * dx;
+            qxy += mb * dx * dy;
+            qyy += mb * dy * dy;
+        }
+        Q[0] = (float)qxx;
+        Q[1] = (float)qxy;
+        Q[2] = (float)qyy;
+    }
+    free(stack);
+    free(order);
+    free(lo);
+    free(hi);
 }

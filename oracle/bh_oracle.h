@@ -91,11 +91,21 @@ void orc_fill(int64_t n, const uint32_t *idx, const float *mass, const float *po
  * closest opening decision came to the other branch of barnes_hut.wgsl:94.
  * term_norm (may be NULL): per body, the sum of the lengths of the accepted
  * terms |G m r inv^3| (the scale of the sum's rounding error).
+ * quad (may be NULL): per node {xx, xy, yy} central second mass moments
+ * (orc_node_quad); each accepted node then adds the quadrupole term of the
+ * softened potential, -G inv^5 [(1.5 tr Q - 7.5 inv^2 r.Q r) r + 3 Q r]
+ * (an extension: the reference adds the monopole only).
  * Runs on `threads` OpenMP threads (results do not depend on it). */
 void orc_walk(int64_t n, const uint32_t *idx, const float *node_cm, const uint32_t *left,
               const uint32_t *right, const uint32_t *roots, const float *mass,
               const float *pos_xy, const orc_params *p, int stack_size, float *accel_xy,
-              int64_t *stats, float *margin, float *term_norm, int threads);
+              int64_t *stats, const float *quad, float *margin, float *term_norm, int threads);
+
+/* Central second mass moments {xx, xy, yy} (float[3 * (2n-1)]) of every
+ * node's leaf range, summed in binary64 in sorted order about the range's
+ * binary64 centre of mass; leaves 0 (a single body: the root is its leaf). */
+void orc_node_quad(int64_t n, const uint32_t *idx, const float *mass, const float *pos_xy,
+                   const uint32_t *left, const uint32_t *right, float *quad);
 
 /* The nodes each sorted position's walk accepts (barnes_hut.wgsl:94), in
  * walk order, as CSR: offsets[n+1], nodes[offsets[t]..offsets[t+1]).

@@ -169,11 +169,12 @@ def fill(idx, mass, pos, left, right, parent, offset=0, order_seed=0):
 
 
 def walk(idx, cm, left, right, mass, pos, params: Params, roots=None, stack_size=256,
-         threads=None, margin=False):
+         threads=None, margin=False, quad=None):
     """orc_walk: (accel (n,2), stats (n,3) {accepted, opened, dropped} by body)
     and, with margin=True, each body's smallest relative distance
     |length*inv - theta| / theta of an opening test from theta and the sum of
-    its accepted terms' lengths |G m r inv^3|: (accel, stats, margin, term_norm)."""
+    its accepted terms' lengths |G m r inv^3|: (accel, stats, margin, term_norm).
+    quad: per-node second moments (node_quad) for the quadrupole expansion."""
     idx = np.ascontiguousarray(idx, np.uint32)
     cm = _f32(cm, (-1, 4))
     left = np.ascontiguousarray(left, np.uint32)
@@ -190,8 +191,12 @@ def walk(idx, cm, left, right, mass, pos, params: Params, roots=None, stack_size
         rp = _p(roots)
     mg = np.zeros(n, np.float32) if margin else None
     tn = np.zeros(n, np.float32) if margin else None
+    qp = None
+    if quad is not None:
+        quad = _f32(quad, (-1, 3))
+        qp = _p(quad)
     lib().orc_walk(ctypes.c_int64(n), _p(idx), _p(cm), _p(left), _p(right), rp, _p(mass),
-                   _p(pos), ctypes.byref(pr), ctypes.c_int(stack_size), _p(acc), _p(stats),
+                   _p(pos), ctypes.byref(pr), ctypes.c_int(stack_size), _p(acc), _p(stats), qp,
                    _p(mg) if margin else None, _p(tn) if margin else None,
                    ctypes.c_int(threads or _threads()))
     return (acc, stats, mg, tn) if margin else (acc, stats)
@@ -336,3 +341,16 @@ def render(pos, radius, cam4, width, height):
     lib().orc_render(ctypes.c_int64(pos.shape[0]), _p(pos), _p(radius), _p(cam4),
                      ctypes.c_int32(width), ctypes.c_int32(height), _p(d16), _p(rgba))
     return d16, rgba
+
+
+def node_quad(idx, mass, pos, left, right):
+    """Central second mass moments {xx, xy, yy} of every node (2n-1, 3)."""
+    idx = np.ascontiguousarray(idx, np.uint32)
+    mass = _f32(mass)
+    pos = _f32(pos, (-1, 2))
+    left = np.ascontiguousarray(left, np.uint32)
+    right = np.ascontiguousarray(right, np.uint32)
+    n = idx.shape[0]
+    q = np.zeros((max(2 * n - 1, 1), 3), np.float32)
+    lib().orc_node_quad(ctypes.c_int64(n), _p(idx), _p(mass), _p(pos), _p(left), _p(right), _p(q))
+    return q

@@ -95,6 +95,7 @@ struct nbody_sim {
     uint32_t *sim_root = nullptr;
     float2 *accel = nullptr;
     float *radius = nullptr;  // per-body radius multipliers (renderer input)
+    double *quad_prefix = nullptr;  // leaf-moment prefix sums (quadrupole expansion)
     // renderer: fixed-point density, fp16 density, RGBA, tone table (one allocation)
     void *render_base = nullptr;
     int render_w = 0, render_h = 0;
@@ -166,6 +167,7 @@ StepParams step_params(const nbody_config &c) {
     p.soft_mode = c.softening_mode;
     p.morton_mode = c.morton_mode;
     p.math_mode = c.math_mode;
+    p.expansion = c.expansion;
     return p;
 }
 
@@ -178,6 +180,12 @@ int validate_config(const nbody_config *c) {
     if (c->math_mode != NBODY_MATH_FAST && c->math_mode != NBODY_MATH_EXACT)
         return fail(nullptr, NBODY_ERR_INVALID_ARG, "unknown math_mode");
     if (c->substeps < 1) return fail(nullptr, NBODY_ERR_INVALID_ARG, "substeps must be >= 1");
+    if (c->expansion != NBODY_EXPANSION_MONOPOLE && c->expansion != NBODY_EXPANSION_QUADRUPOLE)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "unknown expansion");
+    if (c->expansion == NBODY_EXPANSION_QUADRUPOLE &&
+        (c->softening_mode != NBODY_SOFTEN_CONSTANT || c->math_mode != NBODY_MATH_FAST))
+        return fail(nullptr, NBODY_ERR_INVALID_ARG,
+                    "the quadrupole expansion needs constant softening and the fast arithmetic");
     return NBODY_OK;
 }
 
@@ -208,7 +216,7 @@ void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
     s->tree.leaf = a.take<float4>(cap);
     s->tree.pcm = a.take<float4>(kPairStride * cap);
     s->tree.paabb = a.take<float4>(2 * cap);
-    s->tree.root = a.take<float4>(2);
+    s->tree.root = a.take<float4>(4);
     s->tree.child = a.take<uint2>(cap);
     s->tree.range = a.take<uint2>(cap);
     s->tree.parent = a.take<uint32_t>(nodes);
@@ -226,6 +234,8 @@ void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
     s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
     s->accel = a.take<float2>(cap);
     s->radius = external_bodies ? nullptr : a.take<float>(cap);
+    // the quadrupole pass's prefix sums (the allocation-free step is monopole only)
+    s->quad_prefix = external_bodies ? nullptr : a.take<double>(quad_prefix_words(cap));
     s->stats = a.take<unsigned long long>(4);
     s->body_counts = a.take<uint2>(cap);
     s->overflow = a.take<uint32_t>(1);
@@ -276,6 +286,9 @@ bool self_term_zero(const nbody_sim *s) {
     const double m = (double)s->mass_absmax;
     if (!(eps2 >= 1e-30) || !(m <= 1e30)) return false;  // also false for NaN
     const double inv3 = std::pow(eps2, -1.5);
+    // the quadrupole term also forms inv^5 (times zero moments for a leaf)
+    if (s->cfg.expansion == NBODY_EXPANSION_QUADRUPOLE && !(std::pow(eps2, -2.5) <= 1e37))
+        return false;
     return inv3 <= 1e37 && m * inv3 <= 1e30;
 }
 
@@ -378,8 +391,10 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
         launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, lpl, s->sort.hist,
                          clear, slots ? s->bbox_slots : nullptr, st);
     mark(4);
-    // phase 4: bottom-up fill (lbvh.wgsl:168-233)
+    // phase 4: bottom-up fill (lbvh.wgsl:168-233), then the second moments
+    // of the quadrupole expansion
     launch_fill(idx, s->mass, s->pos, n, s->tree, (s->debug & NBODY_DEBUG_FILL_CLIMB) != 0, st);
+    if (s->cfg.expansion == NBODY_EXPANSION_QUADRUPOLE) launch_quad(s->tree, n, s->quad_prefix, st);
     mark(5);
     if (tree_only) {
         // the build has emptied the accumulators and no walk refills them
@@ -518,7 +533,8 @@ nbody_sim *create_common(const nbody_config *cfg, int32_t device, int *err, bool
 int debug_node_buffer(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
     const int64_t n = s->n;
     const int64_t nodes = std::max<int64_t>(2 * n - 1, 0);
-    const int64_t need = nodes * (which == NBODY_BUF_PARENT ? 4 : 16);
+    const int64_t need =
+        nodes * (which == NBODY_BUF_PARENT ? 4 : (which == NBODY_BUF_NODE_QUAD ? 12 : 16));
     if (bytes < need)
         return fail(s, NBODY_ERR_INVALID_ARG,
                     "destination too small: need " + std::to_string(need) + " bytes");
@@ -530,6 +546,28 @@ int debug_node_buffer(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
         for (auto &v : parent)
             if (v != kNoNode) v &= kParentMask;
         std::memcpy(dst, parent.data(), 4 * (size_t)nodes);
+        return NBODY_OK;
+    }
+    if (which == NBODY_BUF_NODE_QUAD) {
+        // moments of child `side` of p: pcm[4p+2].zw, pcm[4p+3].x (left),
+        // pcm[4p+3].yzw (right); the root's in root[2]
+        std::vector<float> pr(16 * (size_t)std::max<int64_t>(n - 1, 1));
+        float4 r2;
+        if (n > 1)
+            HIP_TRY(s, hipMemcpy(pr.data(), s->tree.pcm, 64 * (size_t)(n - 1),
+                                 hipMemcpyDeviceToHost));
+        HIP_TRY(s, hipMemcpy(&r2, s->tree.root + 2, 16, hipMemcpyDeviceToHost));
+        float *out = static_cast<float *>(dst);
+        out[0] = r2.x;
+        out[1] = r2.y;
+        out[2] = r2.z;
+        for (int64_t k = 1; k < nodes; ++k) {
+            const uint32_t pw = parent[(size_t)k];
+            const float *rec = &pr[16 * (size_t)(pw & kParentMask)];
+            const float *q = (pw >> 31) ? rec + 13 : rec + 10;
+            if (k >= n - 1) q = nullptr;  // leaves have no moments
+            for (int c = 0; c < 3; ++c) out[3 * k + c] = q ? q[c] : 0.0f;
+        }
         return NBODY_OK;
     }
     const bool cm = which == NBODY_BUF_NODE_CM;
@@ -798,6 +836,7 @@ int nbody_debug_get(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
         case NBODY_BUF_PARENT:
         case NBODY_BUF_NODE_CM:
         case NBODY_BUF_NODE_AABB:
+        case NBODY_BUF_NODE_QUAD:
             return debug_node_buffer(s, which, dst, bytes);
         case NBODY_BUF_ACCEL:
             src = s->accel;
@@ -905,6 +944,8 @@ int nbody_adjoint_forward(nbody_sim *s, int32_t nsteps, int32_t every) {
     if (s->cfg.softening_mode != NBODY_SOFTEN_CONSTANT)
         return fail(s, NBODY_ERR_INVALID_ARG, "the adjoint needs constant softening");
     if (s->n < 2) return fail(s, NBODY_ERR_INVALID_ARG, "the adjoint needs at least two bodies");
+    if (s->cfg.expansion != NBODY_EXPANSION_MONOPOLE)
+        return fail(s, NBODY_ERR_INVALID_ARG, "the adjoint differentiates the monopole walk");
     HIP_TRY(s, hipSetDevice(s->device));
     const int64_t n = s->n;
     const int32_t cps = (nsteps + every - 1) / every;
@@ -1103,6 +1144,8 @@ int nbody_step_static(const nbody_config *cfg, int64_t n, const float *mass, flo
     if (n > 0 && (!mass || !pos_xy || !vel_xy))
         return fail(nullptr, NBODY_ERR_INVALID_ARG,
                     "nbody_step_static: mass/pos/vel must be device pointers");
+    if (cfg->expansion != NBODY_EXPANSION_MONOPOLE)
+        return fail(nullptr, NBODY_ERR_INVALID_ARG, "nbody_step_static: monopole expansion only");
     if (n == 0 || nsteps == 0) return NBODY_OK;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);

@@ -502,9 +502,44 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
 // own leaf is accepted: its offset is exactly zero, so with a finite pull
 // factor m * inv^3 (the host checks that eps^2 and the largest mass keep it
 // finite) its term is an exact +0 and the sum is unchanged.
+// Quadrupole expansion of a node's pull (central second moments Q = {xx, xy,
+// yy} about its centre of mass, r = centre - body, inv = 1/sqrt(|r|^2 +
+// eps^2)): the softened potential's Taylor series to second order gives
+//   a = G [ (m inv^3 - k) r - 3 inv^5 Q r ],  k = inv^5 (1.5 tr Q - 7.5 inv^2 r.Q r),
+// returned as the factor k1 = m inv^3 - k and the vector g = 3 inv^5 Q r
+// (the sums take a += k1 r - g; G is applied once at the end).  A leaf has
+// Q = 0: k1 = m inv^3 and g = 0 exactly.
+struct QuadTerm {
+    float k1, gx, gy;
+};
+__device__ __forceinline__ QuadTerm quad_term(float f0, float qxx, float qxy, float qyy, float rx,
+                                              float ry, float inv) {
+    const float inv2 = inv * inv;
+    const float inv5 = (inv2 * inv2) * inv;
+    const float qrx = __builtin_fmaf(qxx, rx, qxy * ry);
+    const float qry = __builtin_fmaf(qxy, rx, qyy * ry);
+    const float rqr = __builtin_fmaf(rx, qrx, ry * qry);
+    const float k = inv5 * __builtin_fmaf(-7.5f * inv2, rqr, 1.5f * (qxx + qyy));
+    const float g5 = 3.0f * inv5;
+    return QuadTerm{f0 - k, g5 * qrx, g5 * qry};
+}
+
+// Second moments of node `node` (the root's, or from its parent's pair record).
+__device__ __forceinline__ float3 node_quad(const WalkArgs &a, uint32_t node) {
+    if (node == 0u) {
+        const float4 q = a.root[2];
+        return make_float3(q.x, q.y, q.z);
+    }
+    const uint32_t pw = a.parent[node];
+    const float *rec = reinterpret_cast<const float *>(a.pcm + kPairStride * (size_t)(pw & kParentMask));
+    const float *q = (pw >> 31) ? rec + 13 : rec + 10;
+    return make_float3(q[0], q[1], q[2]);
+}
+
 // The pair walk of one chunk: the 64 bodies at sorted positions [64c, 64c + 64),
-// one per lane of the calling wavefront.
-template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32>
+// one per lane of the calling wavefront.  kQuad: quadrupole expansion (the
+// children's moments are the record's padding words).
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32, bool kQuad>
 __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chunk) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = a.n;
@@ -543,6 +578,14 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
         ax = __builtin_fmaf(fs, rx, ax);
         ay = __builtin_fmaf(fs, ry, ay);
     };
+    // quadrupole form: a + k1 r - g for the lanes of `acc` (the others add
+    // 0 r - 0, an exact zero)
+    auto accumulate_q = [&](const QuadTerm &q, float rx, float ry, uint64_t acc) {
+        const bool in = __builtin_amdgcn_inverse_ballot_w64(acc);
+        const float ks = in ? q.k1 : 0.0f, gxs = in ? q.gx : 0.0f, gys = in ? q.gy : 0.0f;
+        ax = __builtin_fmaf(ks, rx, ax) - gxs;
+        ay = __builtin_fmaf(ks, ry, ay) - gys;
+    };
     PairStack st;
     st.put(0, 0u, 0ull);  // the sentinel
     int sp = 1;
@@ -555,7 +598,15 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
         // a start node that is a leaf is the root of a one-body (sub-)tree
         if (r >= leaf0) acc &= __ballot(my_leaf != r);
         const uint64_t open = m & ~pass;
-        if (acc) accumulate(c.z * (inv * inv * inv), rx, ry, acc);
+        if (acc) {
+            if constexpr (kQuad) {
+                const float3 q = node_quad(a, r);
+                accumulate_q(quad_term(c.z * (inv * inv * inv), q.x, q.y, q.z, rx, ry, inv), rx, ry,
+                             acc);
+            } else {
+                accumulate(c.z * (inv * inv * inv), rx, ry, acc);
+            }
+        }
         if constexpr (kStats) {
             n_acc += __builtin_popcountll(acc);
             n_open += __builtin_popcountll(open);
@@ -611,12 +662,25 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
                                 uniform_f32(pcm_s[rb + 6]), uniform_f32(pcm_s[rb + 7]));
         uint2 ch = make_uint2(uniform_u32(pcmu_s[rb + 8]), uniform_u32(pcmu_s[rb + 9]));
         PairPad pad = {};
+        if constexpr (kQuad) {  // the first record's moments
+#pragma unroll
+            for (int k = 0; k < 6; ++k) pad.w[k] = uniform_u32(pcmu_s[rb + 10 + k]);
+        }
         for (;;) {
             keep_live(pad);  // one 64-byte load per record
             float rxl, ryl, invl, rxr, ryr, invr;
             const uint64_t bl = __ballot(probe(cl, rxl, ryl, invl));
             const uint64_t br = __ballot(probe(cr, rxr, ryr, invr));
             const float fl = cl.z * (invl * invl * invl), fr = cr.z * (invr * invr * invr);
+            // the moments' terms, from this record's padding words (before the
+            // next record's loads overwrite them)
+            QuadTerm tl{}, tr{};
+            if constexpr (kQuad) {
+                tl = quad_term(fl, __uint_as_float(pad.w[0]), __uint_as_float(pad.w[1]),
+                               __uint_as_float(pad.w[2]), rxl, ryl, invl);
+                tr = quad_term(fr, __uint_as_float(pad.w[3]), __uint_as_float(pad.w[4]),
+                               __uint_as_float(pad.w[5]), rxr, ryr, invr);
+            }
             uint64_t accl = mask & bl, accr = mask & br;
             if constexpr (kSelf) {
                 accl &= __ballot(my_leaf != ch.x);
@@ -668,8 +732,13 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
             }
             // keep the pulls behind the loads in the schedule
             __builtin_amdgcn_sched_barrier(0);
-            accumulate(fr, rxr, ryr, accr);
-            accumulate(fl, rxl, ryl, accl);
+            if constexpr (kQuad) {
+                accumulate_q(tr, rxr, ryr, accr);
+                accumulate_q(tl, rxl, ryl, accl);
+            } else {
+                accumulate(fr, rxr, ryr, accr);
+                accumulate(fl, rxl, ryl, accl);
+            }
             mask = nmask;
             if (mask == 0) break;
         }
@@ -692,10 +761,10 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
     }
 }
 
-template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32>
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32, bool kQuad>
 __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint32_t xcd_run) {
     const uint32_t block = logical_block(blockIdx.x, xcd_run);
-    pair_walk_chunk<kRefSoft, kEnsemble, kStats, kSelf, kOff32>(
+    pair_walk_chunk<kRefSoft, kEnsemble, kStats, kSelf, kOff32, kQuad>(
         a, (block * blockDim.x + threadIdx.x) >> 6);
 }
 
@@ -706,13 +775,13 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
 // in a.chunk_order (null: sorted order).  Ensembles list their largest
 // simulations' chunks first, so the longest work items start early and the
 // short ones fill the end of the launch.
-template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32>
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32, bool kQuad>
 __global__ __launch_bounds__(256) void walk_pair_queue_kernel(const WalkArgs a, uint32_t resident) {
     const uint32_t chunks = (a.n + 63u) / 64u;
     uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     while (item < chunks) {
         const uint32_t c = a.chunk_order ? a.chunk_order[item] : item;
-        pair_walk_chunk<kRefSoft, kEnsemble, kStats, kSelf, kOff32>(a, uniform_u32(c));
+        pair_walk_chunk<kRefSoft, kEnsemble, kStats, kSelf, kOff32, kQuad>(a, uniform_u32(c));
         uint32_t next = 0;
         if ((threadIdx.x & 63) == 0) next = atomicAdd(a.queue, 1u);
         item = resident + uniform_u32(next);  // lane 0's value
@@ -722,7 +791,7 @@ __global__ __launch_bounds__(256) void walk_pair_queue_kernel(const WalkArgs a, 
 // Register-stack capacity for a requested stack size: one or two rows of 64.
 int rs_stack_cap(int stack) { return stack < 64 ? 64 : (stack > 128 ? 128 : stack); }
 
-template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf>
+template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kQuad = false>
 void launch_pair(const WalkArgs &a, int block, int xcd, bool queue, hipStream_t s) {
     if (queue && a.queue && a.resident) {
         const uint32_t chunks = (a.n + 63u) / 64u;
@@ -730,19 +799,21 @@ void launch_pair(const WalkArgs &a, int block, int xcd, bool queue, hipStream_t 
         const uint32_t grid = (waves + 3u) / 4u;  // 4 waves per workgroup
         const uint32_t resident = grid * 4u;
         if (a.n <= (1u << 26))
-            walk_pair_queue_kernel<kRefSoft, kEnsemble, kStats, kSelf, true>
+            walk_pair_queue_kernel<kRefSoft, kEnsemble, kStats, kSelf, true, kQuad>
                 <<<grid, 256, 0, s>>>(a, resident);
         else
-            walk_pair_queue_kernel<kRefSoft, kEnsemble, kStats, kSelf, false>
+            walk_pair_queue_kernel<kRefSoft, kEnsemble, kStats, kSelf, false, kQuad>
                 <<<grid, 256, 0, s>>>(a, resident);
         return;
     }
     uint32_t grid, run;
     xcd_grid((a.n + (unsigned)block - 1u) / (unsigned)block, xcd, grid, run);
     if (a.n <= (1u << 26))
-        walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf, true><<<grid, block, 0, s>>>(a, run);
+        walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf, true, kQuad>
+            <<<grid, block, 0, s>>>(a, run);
     else
-        walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf, false><<<grid, block, 0, s>>>(a, run);
+        walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf, false, kQuad>
+            <<<grid, block, 0, s>>>(a, run);
 }
 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats, int kB>
@@ -757,6 +828,22 @@ void launch_rs(const WalkArgs &a, int block, int stack, int xcd, hipStream_t s) 
 template <bool kExact, bool kRefSoft, bool kEnsemble, bool kStats>
 void launch_t(const WalkArgs &a, int block, int stack, int xcd, int variant, int lane_bodies,
               hipStream_t s) {
+    if constexpr (!kExact && !kRefSoft) {
+        // the quadrupole expansion (constant softening, fast arithmetic) runs
+        // a pair walk whatever the variant
+        if (a.p.expansion == 1) {
+            const bool queue = variant == 4;
+            if constexpr (kStats) {
+                launch_pair<false, kEnsemble, true, true, true>(a, block, xcd, queue, s);
+            } else {
+                if (a.self_zero)
+                    launch_pair<false, kEnsemble, false, false, true>(a, block, xcd, queue, s);
+                else
+                    launch_pair<false, kEnsemble, false, true, true>(a, block, xcd, queue, s);
+            }
+            return;
+        }
+    }
     if constexpr (!kExact) {
         if (variant == 2 || variant == 4) {
             // the own-leaf compare is needed for the counters and whenever

@@ -37,6 +37,7 @@ struct StepParams {
     int soft_mode;
     int morton_mode;
     int math_mode;
+    int expansion;  // 0 monopole, 1 quadrupole (second moments in the pair records)
 };
 
 // ---------------------------------------------------------------- sort scratch
@@ -125,7 +126,7 @@ struct TreeBufs {
     float4 *leaf;        // [n] sorted bodies {x, y, mass, 0}
     float4 *pcm;         // [4(n-1)] pair records (children's {com, mass, length}, indices)
     float4 *paabb;       // [2(n-1)] children's {min.x, min.y, max.x, max.y}
-    float4 *root;        // [2] root {cm, aabb}
+    float4 *root;        // [3] root {cm, aabb, second moments {xx, xy, yy, 0}}
     uint32_t *counts;    // fill counters (fill_count_words)
     uint32_t *lists;     // the fill's top node list (fill_list_words)
     uint32_t *slot;      // [n-1] position of a top node in the top list
@@ -213,6 +214,12 @@ int direct_chunks(int64_t n, int64_t ntargets);
 void launch_direct_f64(const float2 *pos, const float *mass, int64_t n, const int64_t *targets,
                        int64_t ntargets, const StepParams &p, double2 *partial, int chunks,
                        double2 *out, hipStream_t s);
+
+// ---------------------------------------------------------------- quadrupole
+// Second mass moments of every node (k_quad.hip) into the pair records'
+// padding and root[2]; `prefix` holds quad_prefix_words(n) doubles.
+int64_t quad_prefix_words(int64_t cap);
+void launch_quad(const TreeBufs &tb, int64_t n, double *prefix, hipStream_t s);
 
 // ---------------------------------------------------------------- render
 // Camera and viewport of the density splat (the Renderer's Metadata,

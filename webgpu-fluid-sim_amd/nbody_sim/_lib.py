@@ -26,6 +26,8 @@ MORTON_REFERENCE = 0
 MORTON_BBOX30 = 1
 MATH_FAST = 0
 MATH_EXACT = 1
+EXPANSION_MONOPOLE = 0
+EXPANSION_QUADRUPOLE = 1
 
 DEBUG_ACCEL = 1
 DEBUG_STATS = 2
@@ -42,6 +44,7 @@ BUF_BBOX = 8
 BUF_SIM_ROOTS = 9
 BUF_WALK_COUNTS = 10
 BUF_TONE_TABLE = 11
+BUF_NODE_QUAD = 12
 
 SCENARIO_DEFAULT = 0
 SCENARIO_TWO_CLUSTERS = 1
@@ -84,6 +87,7 @@ class NbodyConfig(ctypes.Structure):
         ("morton_mode", ctypes.c_int32),
         ("math_mode", ctypes.c_int32),
         ("offset", ctypes.c_uint32),
+        ("expansion", ctypes.c_int32),
     ]
 
 

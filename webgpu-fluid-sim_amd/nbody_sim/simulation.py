@@ -238,6 +238,7 @@ class _Handle:
             "sim_roots": (_lib.BUF_SIM_ROOTS, np.zeros(s, np.uint32)),
             "walk_counts": (_lib.BUF_WALK_COUNTS, np.zeros((n, 2), np.uint32)),
             "tone_table": (_lib.BUF_TONE_TABLE, np.zeros(65536, np.uint32)),
+            "node_quad": (_lib.BUF_NODE_QUAD, np.zeros((nn, 3), np.float32)),
         }
         if which not in spec:
             raise ValueError(f"unknown buffer {which}")

@@ -12,6 +12,7 @@ export type Config = Readonly<{
     softeningMode?: 0 | 1;   // 0 reference (barnes_hut.wgsl:90-91), 1 constant
     mortonMode?: 0 | 1;      // 0 reference box (lbvh.wgsl:48-65), 1 bbox30
     mathMode?: 0 | 1;        // 0 fast, 1 exact
+    expansion?: 0 | 1;       // 0 monopole (barnes_hut.wgsl:92-97), 1 quadrupole
 }>;
 
 export declare function getDefaultConfig(): Config;

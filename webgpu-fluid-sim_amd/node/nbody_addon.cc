@@ -74,6 +74,7 @@ nbody_config to_config(napi_env env, napi_value obj) {
     c.morton_mode = (int32_t)prop(env, obj, "mortonMode", c.morton_mode);
     c.math_mode = (int32_t)prop(env, obj, "mathMode", c.math_mode);
     c.offset = (uint32_t)prop(env, obj, "offset", c.offset);
+    c.expansion = (int32_t)prop(env, obj, "expansion", c.expansion);
     return c;
 }
 
