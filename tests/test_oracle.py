@@ -491,7 +491,7 @@ def test_tone_table_known_answers():
 
 import nbody_sim  # noqa: E402  (host-side generator only: no device call)
 
-def _tree(mass, pos, P):
+def _quad_tree(mass, pos, P):
     r = O.step_debug(mass, pos, np.zeros_like(pos), P)
     return r
 
@@ -504,7 +504,7 @@ def test_node_quad_matches_brute_force():
     pos = rng.normal(0, 1, (n, 2)).astype(np.float32)
     mass = rng.uniform(0.2, 2.0, n).astype(np.float32)
     P = O.Params(softening_mode=O.SOFTEN_CONSTANT, morton_mode=O.MORTON_BBOX30, bh_theta=0.5)
-    r = _tree(mass, pos, P)
+    r = _quad_tree(mass, pos, P)
     q = O.node_quad(r["idx"], mass, pos, r["left"], r["right"])
     from frozen_bh import node_ranges
 
@@ -526,7 +526,7 @@ def test_quad_walk_theta_zero_is_the_monopole_walk():
     quadrupole walk gives the monopole walk's bits."""
     mass, pos, _ = nbody_sim.generate("plummer", 2000, seed=3)
     P = O.Params(softening_mode=O.SOFTEN_CONSTANT, morton_mode=O.MORTON_BBOX30, bh_theta=0.0)
-    r = _tree(mass, pos, P)
+    r = _quad_tree(mass, pos, P)
     q = O.node_quad(r["idx"], mass, pos, r["left"], r["right"])
     a0, _ = O.walk(r["idx"], r["node_cm"], r["left"], r["right"], mass, pos, P)
     a2, _ = O.walk(r["idx"], r["node_cm"], r["left"], r["right"], mass, pos, P, quad=q)
@@ -548,7 +548,7 @@ def test_quad_far_field_order():
             tgt[axis] = d
             pos = np.array([[-a, 0], [a, 0], tgt], np.float32)
             mass = np.array([1, 1, 1e-30], np.float32)
-            r = _tree(mass, pos, P)
+            r = _quad_tree(mass, pos, P)
             q = O.node_quad(r["idx"], mass, pos, r["left"], r["right"])
             exact = O.direct_sum_f64(pos, mass, P, targets=[2])[0]
             out = []
@@ -573,7 +573,7 @@ def test_quad_walk_is_more_accurate():
     mass, pos, _ = nbody_sim.generate("uniform-disk", 16384, seed=1)
     P = O.Params(softening_mode=O.SOFTEN_CONSTANT, morton_mode=O.MORTON_BBOX30, bh_theta=0.5,
                  epsilon=1e-3)
-    r = _tree(mass, pos, P)
+    r = _quad_tree(mass, pos, P)
     q = O.node_quad(r["idx"], mass, pos, r["left"], r["right"])
     tg = np.arange(0, 16384, 8)
     ref = O.direct_sum_f64(pos, mass, P, targets=tg)
