@@ -233,7 +233,12 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *                Morton-consecutive bodies together (the LDS and pair walks
  *                ignore it, walking one body per lane)
  *   radix_bits   digit width of the radix sort: 8, 10 (default) or 11 bits
- *                per pass (30-bit keys: 4, 3 or 3 passes) */
+ *                per pass (30-bit keys: 4, 3 or 3 passes)
+ *   wave_bodies  bodies per wavefront of the pair walks (plain handles): 64,
+ *                32, 16 or 8 Morton-consecutive bodies share one traversal;
+ *                fewer give more, shorter traversals (small simulations,
+ *                where the walk is bound by latency); 0 (default) = the
+ *                widest that leaves at least 2048 wavefronts */
 typedef struct nbody_tuning {
     int32_t walk_block;
     int32_t walk_stack;
@@ -241,6 +246,7 @@ typedef struct nbody_tuning {
     int32_t xcd_order;
     int32_t lane_bodies;
     int32_t radix_bits;
+    int32_t wave_bodies;
 } nbody_tuning;
 int nbody_set_tuning(nbody_sim *sim, const nbody_tuning *t);
 int nbody_get_tuning(const nbody_sim *sim, nbody_tuning *out);

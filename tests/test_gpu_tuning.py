@@ -152,10 +152,11 @@ def test_autotune_picks_a_working_configuration():
     sim.setBodies(*bodies)
     best, table = autotune(sim, steps=2, blocks=(128, 256), stacks=(128,), xcd=(1,),
                            lane_bodies=(1, 2), radix=(10, 8))
-    assert best[:5] in table and ("radix_bits", best[5]) in table
+    assert best[:6] in table and ("radix_bits", best[6]) in table
     assert all(v > 0 for v in table.values())
     t = sim.getTuning()
-    assert (t["walk_variant"], t["walk_block"], t["radix_bits"]) == (best[0], best[1], best[5])
+    assert (t["walk_variant"], t["walk_block"], t["wave_bodies"], t["radix_bits"]) == \
+        (best[0], best[1], best[5], best[6])
     _, p0, _ = sim.getBodies()
     assert np.array_equal(p0, bodies[1])  # state restored
     sim.step(2)
@@ -244,3 +245,36 @@ def test_bbox_carried_between_steps():
     got, want = box_of_step_after(lambda: None)
     assert np.array_equal(got, np.asarray(want, np.float32))
     sim.close()
+
+
+def test_wave_bodies_identical():
+    """The pair walks with 64, 32, 16 or 8 bodies per wavefront (and the
+    automatic choice) take the same decisions and give the same bits: each
+    lane's terms come in its own fixed order."""
+    bodies = nbody_sim.generate("uniform-disk", 16384, seed=31)
+    cfg, _ = params(theta=0.5)
+    cfg = cfg.with_(mathMode="fast")
+    ref, ref_counts = None, None
+    for variant in (2, 4):
+        for wb in (64, 32, 16, 8, 0):
+            sim = nbody_sim.Simulation(cfg)
+            sim.setTuning(walk_variant=variant, wave_bodies=wb)
+            for stats in (False, True):
+                sim.setBodies(*bodies)
+                sim.setDebug(accel=True, stats=stats)
+                sim.walkStats(reset=True)
+                sim.step(2)
+                sim.synchronize()
+                got = [sim.debugBuffer("accel"), sim.getBodies()[1]]
+                if ref is None:
+                    ref = got
+                for a, b in zip(ref, got):
+                    assert np.array_equal(a, b), (variant, wb, stats)
+                if stats:
+                    st = sim.walkStats()
+                    counts = (st["accepted"], st["opened"], sim.debugBuffer("walk_counts").copy())
+                    if ref_counts is None:
+                        ref_counts = counts
+                    assert counts[:2] == ref_counts[:2], (variant, wb)
+                    assert np.array_equal(counts[2], ref_counts[2]), (variant, wb)
+            sim.close()

@@ -112,6 +112,7 @@ struct nbody_sim {
     int walk_block = 256, walk_stack = 128;
     int walk_variant = 2;  // 0: LDS-stack walk, 1: register-stack walk, 2: pair walk (fast mode)
     int lane_bodies = 1;   // bodies per lane in the register-stack walks (1, 2, 4)
+    int wave_bodies = 0;   // bodies per wavefront in the pair walks (64, 32, 16, 8; 0 = auto)
     int xcd_order = 1;  // walk workgroup order over the XCDs (nbody_tuning.xcd_order)
 
     bool use_graph = false;
@@ -423,6 +424,18 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     w.queue = s->sort.counters + kWalkQueueWord;  // zeroed by the build
     w.chunk_order = s->chunk_order;
     w.resident = s->resident_waves;
+    // the ensembles' queue order lists 64-body chunks: they keep 64 bodies per
+    // wavefront; plain handles by default take the widest that leaves at
+    // least 2048 wavefronts (two per SIMD), so a small simulation's
+    // latency-bound traversals run side by side
+    uint32_t wb = (uint32_t)s->wave_bodies;
+    if (s->ensemble) {
+        wb = 64u;
+    } else if (wb == 0u) {
+        wb = 64u;
+        while (wb > 8u && (uint64_t)n / wb < 2048u) wb >>= 1;
+    }
+    w.wave_bodies = wb;
     w.bbox_slots = slots ? s->bbox_slots : nullptr;
     launch_walk(w, s->ensemble, (s->debug & NBODY_DEBUG_STATS) != 0, s->walk_block, s->walk_stack,
                 s->xcd_order, s->walk_variant, s->lane_bodies, st);
@@ -912,6 +925,9 @@ int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
         return fail(s, NBODY_ERR_INVALID_ARG, "walk_variant must be 0, 1, 2 or 4");
     if (t->lane_bodies != 1 && t->lane_bodies != 2 && t->lane_bodies != 4)
         return fail(s, NBODY_ERR_INVALID_ARG, "lane_bodies must be 1, 2 or 4");
+    if (t->wave_bodies != 0 && t->wave_bodies != 64 && t->wave_bodies != 32 &&
+        t->wave_bodies != 16 && t->wave_bodies != 8)
+        return fail(s, NBODY_ERR_INVALID_ARG, "wave_bodies must be 0 (auto), 64, 32, 16 or 8");
     if (t->xcd_order < 0 || t->xcd_order > 65536)
         return fail(s, NBODY_ERR_INVALID_ARG, "xcd_order must be in [0, 65536]");
     if (t->radix_bits != 8 && t->radix_bits != 10 && t->radix_bits != 11)
@@ -919,6 +935,7 @@ int nbody_set_tuning(nbody_sim *s, const nbody_tuning *t) {
     s->radix_bits = t->radix_bits;
     s->walk_block = t->walk_block;
     s->lane_bodies = t->lane_bodies;
+    s->wave_bodies = t->wave_bodies;
     s->walk_stack = t->walk_stack;
     s->walk_variant = t->walk_variant;
     s->xcd_order = t->xcd_order;
@@ -933,6 +950,7 @@ int nbody_get_tuning(const nbody_sim *s, nbody_tuning *out) {
     out->walk_variant = s->walk_variant;
     out->xcd_order = s->xcd_order;
     out->lane_bodies = s->lane_bodies;
+    out->wave_bodies = s->wave_bodies;
     out->radix_bits = s->radix_bits;
     return NBODY_OK;
 }

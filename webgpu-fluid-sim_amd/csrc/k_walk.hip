@@ -543,8 +543,12 @@ template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32, b
 __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chunk) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = a.n;
-    const uint32_t t = chunk * 64u + (uint32_t)lane;  // sorted position
-    const bool valid = t < n;
+    // the chunk's bodies: wave_bodies (64, 32, 16 or 8) Morton-consecutive
+    // bodies on the first lanes (fewer bodies per wavefront: more, shorter
+    // traversals, for latency-bound small simulations; no sum changes)
+    const uint32_t wb = a.wave_bodies;
+    const uint32_t t = chunk * wb + (uint32_t)lane;  // sorted position
+    const bool valid = (uint32_t)lane < wb && t < n;
     const uint32_t leaf0 = n - 1u;
     const uint32_t my_leaf = leaf0 + t;
     float px = 0.0f, py = 0.0f, m1 = 0.0f;
@@ -777,7 +781,7 @@ __global__ __launch_bounds__(1024) void walk_pair_kernel(const WalkArgs a, uint3
 // short ones fill the end of the launch.
 template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kOff32, bool kQuad>
 __global__ __launch_bounds__(256) void walk_pair_queue_kernel(const WalkArgs a, uint32_t resident) {
-    const uint32_t chunks = (a.n + 63u) / 64u;
+    const uint32_t chunks = (a.n + a.wave_bodies - 1u) / a.wave_bodies;
     uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     while (item < chunks) {
         const uint32_t c = a.chunk_order ? a.chunk_order[item] : item;
@@ -793,8 +797,8 @@ int rs_stack_cap(int stack) { return stack < 64 ? 64 : (stack > 128 ? 128 : stac
 
 template <bool kRefSoft, bool kEnsemble, bool kStats, bool kSelf, bool kQuad = false>
 void launch_pair(const WalkArgs &a, int block, int xcd, bool queue, hipStream_t s) {
+    const uint32_t chunks = (a.n + a.wave_bodies - 1u) / a.wave_bodies;
     if (queue && a.queue && a.resident) {
-        const uint32_t chunks = (a.n + 63u) / 64u;
         const uint32_t waves = chunks < a.resident ? chunks : a.resident;
         const uint32_t grid = (waves + 3u) / 4u;  // 4 waves per workgroup
         const uint32_t resident = grid * 4u;
@@ -807,7 +811,8 @@ void launch_pair(const WalkArgs &a, int block, int xcd, bool queue, hipStream_t 
         return;
     }
     uint32_t grid, run;
-    xcd_grid((a.n + (unsigned)block - 1u) / (unsigned)block, xcd, grid, run);
+    const uint32_t waves_per_block = (uint32_t)block / 64u;
+    xcd_grid((chunks + waves_per_block - 1u) / waves_per_block, xcd, grid, run);
     if (a.n <= (1u << 26))
         walk_pair_kernel<kRefSoft, kEnsemble, kStats, kSelf, true, kQuad>
             <<<grid, block, 0, s>>>(a, run);

@@ -180,6 +180,7 @@ struct WalkArgs {
     uint32_t *queue;
     const uint32_t *chunk_order;
     uint32_t resident;
+    uint32_t wave_bodies;  // pair walks: bodies per wavefront (64, 32, 16, 8)
     StepParams p;
 };
 void launch_walk(const WalkArgs &a, bool ensemble, bool stats, int block, int stack,

@@ -99,6 +99,7 @@ class NbodyTuning(ctypes.Structure):
         ("xcd_order", ctypes.c_int32),
         ("lane_bodies", ctypes.c_int32),
         ("radix_bits", ctypes.c_int32),
+        ("wave_bodies", ctypes.c_int32),
     ]
 
 

@@ -160,13 +160,14 @@ class _Handle:
         return {f: getattr(t, f) for f, _ in _lib.NbodyTuning._fields_}
 
     def setTuning(self, walk_block=None, walk_stack=None, walk_variant=None, xcd_order=None,
-                  lane_bodies=None, radix_bits=None):
+                  lane_bodies=None, radix_bits=None, wave_bodies=None):
         """Launch tunables (nbody_tuning); None keeps a value.  Every setting takes
         the same decisions; only walk_variant 2 (fast mode) sums in another order."""
         cur = self.getTuning()
         for k, v in (("walk_block", walk_block), ("walk_stack", walk_stack),
                      ("walk_variant", walk_variant), ("xcd_order", xcd_order),
-                     ("lane_bodies", lane_bodies), ("radix_bits", radix_bits)):
+                     ("lane_bodies", lane_bodies), ("radix_bits", radix_bits),
+                     ("wave_bodies", wave_bodies)):
             if v is not None:
                 cur[k] = int(v)
         t = _lib.NbodyTuning(**cur)
