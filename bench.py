@@ -40,10 +40,15 @@ PMC_PROFILE = os.path.join(ROOT, "profiles", "step_800k_pmc.md")
 PEAK_HBM_GBS = 8000.0
 FLOPS_ACCEPT = 18          # per accepted interaction (constant softening, rsqrt = 1)
 FLOPS_OPEN = 9             # per opened node
+QUAD_EXTRA = 24            # quadrupole term per accepted interaction: inv^2 1, inv^5 2, Q r 6, r.Q r 3,
+                           # k 6, g 3, k1 1, a -= g 2
+DESCRIPTIONS = {"plain_800k": "plain BH step, 2D exponential disk N=800000, theta=0.7, "
+                              "eps=1e-3, dt=5e-5, bbox30 keys, seed 1"}
 # The walk kernel this workload dispatches: pair walk, plain (one simulation),
 # no counters, reference softening off, own-leaf term exactly zero, 32-bit
-# record offsets.
-WALK_KERNEL = "walk_pair_kernel<false, false, false, false, true>"
+# record offsets, monopole (the quadrupole expansion's has true last).
+WALK_KERNEL = "walk_pair_kernel<false, false, false, false, true, false>"
+WALK_KERNEL_QUAD = "walk_pair_kernel<false, false, false, false, true, true>"
 
 
 def walk_traffic():
@@ -64,13 +69,16 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def workload():
+def workload(name="plain_800k", expansion="monopole", theta=None):
+    """(workload, config, mass, pos, vel) of the timed leg."""
     from nbody_sim import workloads as W
 
-    w = W.WORKLOADS["plain_800k"]
-    assert (w.n, w.theta, w.dt, w.eps, w.seed) == (N_BODIES, THETA, DT, EPS, SEED)
+    w = W.WORKLOADS[name]
+    if name == "plain_800k":
+        assert (w.n, w.theta, w.dt, w.eps, w.seed) == (N_BODIES, THETA, DT, EPS, SEED)
     mass, pos, vel = w.bodies()
-    return w.config("fast"), mass, pos, vel
+    cfg = w.config("fast").with_(expansion=expansion, bhTheta=w.theta if theta is None else theta)
+    return w, cfg, mass, pos, vel
 
 
 def cpu_threads():
@@ -84,14 +92,15 @@ def cpu_threads():
     return max(1, avail)
 
 
-def cpu_baseline(mass, pos, vel, budget_s=15.0):
+def cpu_baseline(cfg, mass, pos, vel, budget_s=15.0):
     """The oracle's restatement of one full step, OpenMP over the host cores,
     repeated until ~budget_s of work (at least one step)."""
     from oracle import oracle as O
 
     threads = cpu_threads()
-    P = O.Params(grav_constant=1.0, delta_time=DT, bh_theta=THETA, epsilon=EPS,
-                 softening_mode=O.SOFTEN_CONSTANT, morton_mode=O.MORTON_BBOX30)
+    P = O.Params(grav_constant=1.0, delta_time=cfg.deltaTime, bh_theta=cfg.bhTheta,
+                 epsilon=cfg.epsilon, softening_mode=O.SOFTEN_CONSTANT,
+                 morton_mode=O.MORTON_BBOX30)
     steps = 0
     t0 = time.perf_counter()
     p, v = pos, vel
@@ -102,11 +111,12 @@ def cpu_baseline(mass, pos, vel, budget_s=15.0):
         log(f"cpu baseline step {steps}: {el:.1f}s")
         if el >= budget_s:
             break
-    return {"value": N_BODIES * steps / el, "unit": "body-steps/s", "cores": threads,
+    n = mass.shape[0]
+    return {"value": n * steps / el, "unit": "body-steps/s", "cores": threads,
             "host_logical_cpus": os.cpu_count(),
             "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
             "kind": "port",
-            "sample": f"{steps} full step(s) of the same 800k-body workload from its initial "
+            "sample": f"{steps} full step(s) of the same {n}-body workload from its initial "
                       f"state, oracle C restatement, OpenMP {threads} threads, {el:.1f} s"}
 
 
@@ -323,6 +333,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other configurations, the ensemble, adjoint and error legs")
+    ap.add_argument("--workload", default="plain_800k",
+                    help="timed leg: plain_800k (default), plain_16k, collision_4M, adjoint_65k")
+    ap.add_argument("--expansion", default="monopole", choices=("monopole", "quadrupole"))
+    ap.add_argument("--theta", type=float, default=None, help="override the workload's theta")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -331,8 +345,9 @@ def main():
 
     import nbody_sim
 
-    cfg, mass, pos, vel = workload()
-    log(f"workload generated: n={N_BODIES}")
+    w, cfg, mass, pos, vel = workload(args.workload, args.expansion, args.theta)
+    default_leg = (args.workload, args.expansion, args.theta) == ("plain_800k", "monopole", None)
+    log(f"workload generated: {w.name}, n={w.n}, {args.expansion}, theta={cfg.bhTheta}")
     sim = nbody_sim.Simulation(cfg, device=0)
     sim.setBodies(mass, pos, vel)
     sim.useGraph(True)
@@ -346,7 +361,7 @@ def main():
     sim.synchronize()
     el = time.perf_counter() - t0
     ms_per_step = 1e3 * el / args.steps
-    value = N_BODIES * args.steps / el
+    value = w.n * args.steps / el
     log(f"timed {args.steps} steps: {el:.3f}s, {ms_per_step:.4f} ms/step")
 
     # Untimed: per-phase event timing and walk counters over the same steps.
@@ -372,7 +387,14 @@ def main():
     achieved = flops / (walk_ms * 1e-3) / 1e12
     log(f"phases ms/step: {phase_avg}; walk flops/launch {flops:.3e}")
 
-    traffic, traffic_src = walk_traffic()
+    kernel = WALK_KERNEL_QUAD if args.expansion == "quadrupole" else WALK_KERNEL
+    # the committed PMC summary covers the default leg only
+    traffic, traffic_src = walk_traffic() if default_leg else (None, None)
+    flop_model = "18 per accepted interaction + 9 per opened node"
+    if args.expansion == "quadrupole":
+        flops += QUAD_EXTRA * st["accepted"] / args.profile_steps
+        achieved = flops / (walk_ms * 1e-3) / 1e12
+        flop_model += f" + {QUAD_EXTRA} per accepted interaction (quadrupole term)"
     out = {
         "metric": METRIC,
         "value": value,
@@ -386,13 +408,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain BH step, 2D exponential disk N=800000, theta=0.7, "
-                               "eps=1e-3, dt=5e-5, bbox30 keys, seed 1",
-                   "n_bodies": N_BODIES, "theta": THETA, "dt": DT, "eps": EPS,
-                   "math_mode": "fast", "graph": True},
+        "config": {"workload": DESCRIPTIONS.get(w.name, w.name) if default_leg else
+                   f"{w.name}: {w.scenario} N={w.n}, {args.expansion}, theta={cfg.bhTheta}",
+                   "n_bodies": w.n, "theta": cfg.bhTheta, "dt": w.dt, "eps": w.eps,
+                   "math_mode": "fast", "expansion": args.expansion, "graph": True},
         "phases_ms": {k: round(v, 5) for k, v in phase_avg.items()},
         "roofline": {
-            "kernel": WALK_KERNEL + " (tree walk + kick + drift)",
+            "kernel": kernel + " (tree walk + kick + drift)",
             "bound": "valu-fp32",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
@@ -401,12 +423,13 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_src,
             "flops_per_launch": flops,
+            "flop_model": flop_model,
             "walk_ms": walk_ms,
             "interactions_per_step": st["accepted"] / args.profile_steps,
             "opened_per_step": st["opened"] / args.profile_steps,
         },
     }
-    if not args.no_extras:
+    if not args.no_extras and default_leg:
         log("exact-mode 800k")
         out["plain_800k_exact"] = plain_config("plain_800k", 20, math="exact", phases=True)
         log("small plain config")
@@ -422,7 +445,7 @@ def main():
         log("accuracy target")
         out["accuracy_target"] = accuracy_target()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mass, pos, vel, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(cfg, mass, pos, vel, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 
