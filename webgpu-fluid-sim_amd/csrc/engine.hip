@@ -169,6 +169,7 @@ StepParams step_params(const nbody_config &c) {
     p.morton_mode = c.morton_mode;
     p.math_mode = c.math_mode;
     p.expansion = c.expansion;
+    open_params(c.bh_theta, p.open_scale, p.open_bias);
     return p;
 }
 
@@ -394,6 +395,11 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     mark(4);
     // phase 4: bottom-up fill (lbvh.wgsl:168-233), then the second moments
     // of the quadrupole expansion
+    // monopole pair records carry their children's opening keys (the
+    // quadrupole pass writes moments into the same words)
+    s->tree.pair_keys = p.expansion == NBODY_EXPANSION_MONOPOLE ? 1u : 0u;
+    s->tree.open_scale = p.open_scale;
+    s->tree.open_bias = p.open_bias;
     launch_fill(idx, s->mass, s->pos, n, s->tree, (s->debug & NBODY_DEBUG_FILL_CLIMB) != 0, st);
     if (s->cfg.expansion == NBODY_EXPANSION_QUADRUPOLE) launch_quad(s->tree, n, s->quad_prefix, st);
     mark(5);
@@ -562,6 +568,12 @@ int debug_node_buffer(nbody_sim *s, int32_t which, void *dst, int64_t bytes) {
         return NBODY_OK;
     }
     if (which == NBODY_BUF_NODE_QUAD) {
+        // the monopole expansion computes no moments (its pair records' padding
+        // holds opening keys)
+        if (s->cfg.expansion != NBODY_EXPANSION_QUADRUPOLE) {
+            std::memset(dst, 0, (size_t)need);
+            return NBODY_OK;
+        }
         // moments of child `side` of p: pcm[4p+2].zw, pcm[4p+3].x (left),
         // pcm[4p+3].yzw (right); the root's in root[2]
         std::vector<float> pr(16 * (size_t)std::max<int64_t>(n - 1, 1));
@@ -1045,6 +1057,7 @@ int nbody_adjoint_backward(nbody_sim *s, const float *gpos_T, const float *gvel_
     a.n = (uint32_t)n;
     a.grav = s->cfg.grav_constant;
     a.theta = s->cfg.bh_theta;
+    open_params(a.theta, a.open_scale, a.open_bias);
     a.eps = s->cfg.epsilon;
     const bool exact = s->cfg.math_mode == NBODY_MATH_EXACT;
 

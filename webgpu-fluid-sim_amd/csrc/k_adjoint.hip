@@ -103,8 +103,8 @@ __global__ __launch_bounds__(256) void adjoint_walk_kernel(const AdjointArgs a, 
         [&](const float4 &c, int b) {
             rx[b] = c.x - px[b];
             ry[b] = c.y - py[b];
-            inv[b] = inv_dist<kExact>(rx[b], ry[b], eps2);
-            return passes(c.w, inv[b], theta);
+            return probe_node<kExact>(c, rx[b], ry[b], eps2, theta, a.open_scale, a.open_bias,
+                                      inv[b]);
         },
         [&](const float4 &c, uint32_t node, int b, uint64_t acc_mask) {
             const bool acc = __builtin_amdgcn_inverse_ballot_w64(acc_mask);
@@ -184,8 +184,7 @@ __global__ __launch_bounds__(256) void adjoint_pair_kernel(const AdjointArgs a) 
     auto probe = [&](const float4 &c, float &rx, float &ry, float &inv) {
         rx = c.x - px;
         ry = c.y - py;
-        inv = inv_dist<false>(rx, ry, eps2);
-        return passes(c.w, inv, theta);
+        return probe_node<false>(c, rx, ry, eps2, theta, a.open_scale, a.open_bias, inv);
     };
     // The VJP of the pull of a node (record c) for the lanes of acc: the
     // lane's terms (hx, hy, dm), added to its own sums by the caller.

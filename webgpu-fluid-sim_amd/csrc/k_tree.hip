@@ -378,6 +378,9 @@ __device__ __forceinline__ void publish(const TreeBufs &tb, uint32_t node, uint3
     }
     tb.pcm[pcm_slot(pw)] = C;
     tb.paabb[paabb_slot(pw)] = B;
+    if (tb.pair_keys)
+        reinterpret_cast<float *>(tb.pcm)[16 * (size_t)(pw & kParentMask) + 10 + (pw >> 31)] =
+            open_key(C.w, tb.open_scale, tb.open_bias);
 }
 
 

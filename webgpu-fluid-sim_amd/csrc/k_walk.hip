@@ -156,8 +156,9 @@ __device__ __forceinline__ bool interact(const float4 c, bool leaf, float px, fl
     const float rx = c.x - px;
     const float ry = c.y - py;
     const float eps2 = kRefSoft ? soft_eps2(m1, c.z, p) : eps2c;
-    const float inv = inv_dist<kExact>(rx, ry, eps2);
-    if (leaf || passes(c.w, inv, p.theta)) {
+    float inv;
+    const bool pass = probe_node<kExact>(c, rx, ry, eps2, p.theta, p.open_scale, p.open_bias, inv);
+    if (leaf || pass) {
         add_pull<kExact>(c, rx, ry, inv, p.grav, ax, ay);
         return false;
     }
@@ -440,8 +441,8 @@ __global__ __launch_bounds__(1024) void walk_rs_kernel(const WalkArgs a, int sta
             rx[b] = cc.x - px[b];
             ry[b] = cc.y - py[b];
             const float eps2 = kRefSoft ? soft_eps2(m1[b], cc.z, p) : eps2c;
-            inv[b] = inv_dist<kExact>(rx[b], ry[b], eps2);
-            return passes(cc.w, inv[b], p.theta);
+            return probe_node<kExact>(cc, rx[b], ry[b], eps2, p.theta, p.open_scale, p.open_bias,
+                                      inv[b]);
         },
         [&](const float4 &cc, uint32_t, int b, uint64_t acc) {
             float nx = ax[b], ny = ay[b];
@@ -571,8 +572,7 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
         rx = c.x - px;
         ry = c.y - py;
         const float eps2 = kRefSoft ? soft_eps2(m1, c.z, p) : eps2c;
-        inv = inv_dist<false>(rx, ry, eps2);
-        return passes(c.w, inv, p.theta);
+        return probe_node<false>(c, rx, ry, eps2, p.theta, p.open_scale, p.open_bias, inv);
     };
     // a + f r for the lanes of `acc` (f = m inv^3: add_pull's fast form);
     // the other lanes add 0 * r, an exact zero for a finite offset (the sum
@@ -665,84 +665,111 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
         float4 cr = make_float4(uniform_f32(pcm_s[rb + 4]), uniform_f32(pcm_s[rb + 5]),
                                 uniform_f32(pcm_s[rb + 6]), uniform_f32(pcm_s[rb + 7]));
         uint2 ch = make_uint2(uniform_u32(pcmu_s[rb + 8]), uniform_u32(pcmu_s[rb + 9]));
+        // the first record's padding: the children's opening keys (monopole)
+        // or moments (quadrupole)
         PairPad pad = {};
-        if constexpr (kQuad) {  // the first record's moments
 #pragma unroll
-            for (int k = 0; k < 6; ++k) pad.w[k] = uniform_u32(pcmu_s[rb + 10 + k]);
-        }
+        for (int k = 0; k < (kQuad ? 6 : 2); ++k) pad.w[k] = uniform_u32(pcmu_s[rb + 10 + k]);
+        // The pull of one child for the lanes of `acc` (non-empty), from its
+        // mass, moments, offset and squared softened distance d2: the
+        // reciprocal square root is taken only here, for the children some
+        // lane accepts (about a quarter of the children no lane accepts).
+        auto pull = [&](float m, uint32_t q0, uint32_t q1, uint32_t q2, float rx, float ry,
+                        float d2, uint64_t acc) {
+            const float inv = __builtin_amdgcn_rsqf(d2);
+            const float f = m * (inv * inv * inv);
+            if constexpr (kQuad)
+                accumulate_q(quad_term(f, __uint_as_float(q0), __uint_as_float(q1),
+                                       __uint_as_float(q2), rx, ry, inv),
+                             rx, ry, acc);
+            else
+                accumulate(f, rx, ry, acc);
+        };
         for (;;) {
             keep_live(pad);  // one 64-byte load per record
-            float rxl, ryl, invl, rxr, ryr, invr;
-            const uint64_t bl = __ballot(probe(cl, rxl, ryl, invl));
-            const uint64_t br = __ballot(probe(cr, rxr, ryr, invr));
-            const float fl = cl.z * (invl * invl * invl), fr = cr.z * (invr * invr * invr);
-            // the moments' terms, from this record's padding words (before the
-            // next record's loads overwrite them)
-            QuadTerm tl{}, tr{};
+            const float eps2l = kRefSoft ? soft_eps2(m1, cl.z, p) : eps2c;
+            const float eps2r = kRefSoft ? soft_eps2(m1, cr.z, p) : eps2c;
+            const float rxl = cl.x - px, ryl = cl.y - py, rxr = cr.x - px, ryr = cr.y - py;
+            const float d2l = soft_d2(rxl, ryl, eps2l), d2r = soft_d2(rxr, ryr, eps2r);
+            // the opening keys: the record's own (monopole), or computed from
+            // the lengths with the same expression (quadrupole: the padding
+            // holds moments)
+            float kl, kr;
             if constexpr (kQuad) {
-                tl = quad_term(fl, __uint_as_float(pad.w[0]), __uint_as_float(pad.w[1]),
-                               __uint_as_float(pad.w[2]), rxl, ryl, invl);
-                tr = quad_term(fr, __uint_as_float(pad.w[3]), __uint_as_float(pad.w[4]),
-                               __uint_as_float(pad.w[5]), rxr, ryr, invr);
+                kl = open_key(cl.w, p.open_scale, p.open_bias);
+                kr = open_key(cr.w, p.open_scale, p.open_bias);
+            } else {
+                kl = __uint_as_float(pad.w[0]);
+                kr = __uint_as_float(pad.w[1]);
             }
+            const uint64_t bl = __ballot(passes_key(kl, d2l));
+            const uint64_t br = __ballot(passes_key(kr, d2r));
             uint64_t accl = mask & bl, accr = mask & br;
             if constexpr (kSelf) {
                 accl &= __ballot(my_leaf != ch.x);
                 accr &= __ballot(my_leaf != ch.y);
             }
             // leaves always pass, so an opened child is an internal node
-            const uint64_t openl = mask & ~bl, openr = mask & ~br;
+            // The next entry: the right child first, the left one waits on
+            // the stack (written to the free slot every time, kept by
+            // advancing the top); with neither child opened, the top entry
+            // (the sentinel's empty mask ends the walk).  A plain tree of up
+            // to 2^26 bodies is at most 32 + 26 levels deep (30-bit keys tie-
+            // broken on the sorted position), so its stack never reaches the
+            // top; ensembles' 38-bit keys and larger trees keep the clamp.
+            constexpr bool kClamp = kEnsemble || kStats || !kOff32;
+            uint32_t next, sp_next, both;
+            uint64_t openl, openr, nmask;
+            pair_next<kClamp>(mask, bl, br, ch.x, ch.y, (uint32_t)sp, openl, openr, nmask, next,
+                              sp_next, both);
             if constexpr (kStats) {
                 n_acc += __builtin_popcountll(accl) + __builtin_popcountll(accr);
                 n_open += __builtin_popcountll(openl) + __builtin_popcountll(openr);
                 my_acc += own(accl) + own(accr);
                 my_open += own(openl) + own(openr);
             }
-            // The next entry: the right child first, the left one waits on
-            // the stack (written to the free slot every time, kept by
-            // advancing the top); with neither child opened, the top entry.
-            // Flags are 0/1 integers: booleans here would be materialised as
-            // lane masks.
-            const uint32_t hl = nonzero_flag(openl), hr = nonzero_flag(openr);
             st.put(sp, ch.x, openl);
-            const uint32_t top = (uint32_t)sp + (hl & hr);
-            if constexpr (kEnsemble || kStats) {
-                n_over += top >> 6;  // top == 64: the left child is dropped
+            if constexpr (kClamp) {
+                // both children opened with the top at kPairStackTop: the
+                // left child is dropped
+                const uint32_t drop = both & (uint32_t)(sp == kPairStackTop);
+                n_over += drop;
                 if constexpr (kStats) {
-                    if (top >> 6) {
+                    if (drop) {
                         n_open -= __builtin_popcountll(openl);
                         n_drop += __builtin_popcountll(openl);
                         my_open -= own(openl);
                     }
                 }
             }
-            const uint32_t any = hl | hr;
-            uint32_t next = hr ? ch.y : ch.x;
-            uint64_t nmask = hr ? openr : openl;
-            sp = any ? (int)min(top, (uint32_t)kPairStackTop) : sp - 1;
-            if (!any) st.get(sp, next, nmask);  // the sentinel's empty mask ends the walk
-            // the next record's loads, issued before this node's pulls
-            {
-                const_as<float> *rf;
-                if constexpr (kOff32)  // byte offset below 2^32 (n <= 2^26): an SGPR offset
-                    rf = (const_as<float> *)((const_as<char> *)pcm_s + (uint32_t)(next * 64u));
-                else
-                    rf = pcm_s + 4 * kPairStride * (size_t)next;
-                const_as<uint32_t> *ru = (const_as<uint32_t> *)rf;
-                cl = make_float4(rf[0], rf[1], rf[2], rf[3]);
-                cr = make_float4(rf[4], rf[5], rf[6], rf[7]);
-                ch = make_uint2(ru[8], ru[9]);
-                load_pad(pad, ru);
-            }
-            // keep the pulls behind the loads in the schedule
-            __builtin_amdgcn_sched_barrier(0);
+            sp = (int)sp_next;
+            if (nmask == 0) st.get(sp, next, nmask);
+            // what the pulls take from this record, copied out of the
+            // record's registers so that the next record lands in them whole
+            const float ml = scalar_copy(cl.z), mr = scalar_copy(cr.z);
+            PairPad q;
             if constexpr (kQuad) {
-                accumulate_q(tr, rxr, ryr, accr);
-                accumulate_q(tl, rxl, ryl, accl);
-            } else {
-                accumulate(fr, rxr, ryr, accr);
-                accumulate(fl, rxl, ryl, accl);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) q.w[k] = scalar_copy_u32(pad.w[k]);
             }
+            // the next record, one 64-byte load, issued before this node's pulls
+            {
+                const_as<RecordWords> *rf;
+                if constexpr (kOff32)  // byte offset below 2^32 (n <= 2^26): an SGPR offset
+                    rf = (const_as<RecordWords> *)((const_as<char> *)pcm_s + (uint32_t)(next * 64u));
+                else
+                    rf = (const_as<RecordWords> *)(pcm_s + 4 * kPairStride * (size_t)next);
+                const RecordWords w = *rf;
+                cl = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
+                                 __uint_as_float(w[2]), __uint_as_float(w[3]));
+                cr = make_float4(__uint_as_float(w[4]), __uint_as_float(w[5]),
+                                 __uint_as_float(w[6]), __uint_as_float(w[7]));
+                ch = make_uint2(w[8], w[9]);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) pad.w[k] = w[10 + k];
+            }
+            if (accr) pull(mr, q.w[3], q.w[4], q.w[5], rxr, ryr, d2r, accr);
+            if (accl) pull(ml, q.w[0], q.w[1], q.w[2], rxl, ryl, d2l, accl);
             mask = nmask;
             if (mask == 0) break;
         }

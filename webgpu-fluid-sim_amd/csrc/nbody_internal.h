@@ -38,7 +38,36 @@ struct StepParams {
     int morton_mode;
     int math_mode;
     int expansion;  // 0 monopole, 1 quadrupole (second moments in the pair records)
+    float open_scale, open_bias;  // the fast arithmetic's opening keys (open_key)
 };
+
+// The fast arithmetic's opening test.  A body passes a node of length len
+// (the reference's length * inv < theta, barnes_hut.wgsl:89) when
+// !(open_key(len) >= d2), d2 = rx^2 + ry^2 + eps^2 the squared softened
+// distance: key = len |len| / theta^2, so the test needs no square root and
+// the walks take the reciprocal square root only for the nodes some body
+// accepts.  In exact arithmetic both tests agree; in binary32 they part only
+// where len / d lies within a few units in the last place of theta.  Leaves
+// (length -inf) get the key -inf, or NaN when theta <= 0: both pass for every
+// d2.  theta <= 0 (or NaN) makes the bias +inf, so every internal node opens,
+// as the reference's test does for such a theta.  The pair records carry the
+// children's keys in their first two padding words (monopole), written by the
+// fill with this same expression.
+__host__ __device__ inline float open_key(float len, float scale, float bias) {
+    return len * fabsf(len) * scale + bias;
+}
+inline void open_params(float theta, float &scale, float &bias) {
+    if (theta > 0.0f) {
+        double s = 1.0 / ((double)theta * (double)theta);
+        s = s < 1.1754943508222875e-38 ? 1.1754943508222875e-38 : s;  // FLT_MIN: never 0
+        s = s > 3.4028234663852886e38 ? 3.4028234663852886e38 : s;    // FLT_MAX: never inf
+        scale = (float)s;
+        bias = 0.0f;
+    } else {
+        scale = 3.4028234663852886e38f;
+        bias = __builtin_huge_valf();
+    }
+}
 
 // ---------------------------------------------------------------- sort scratch
 constexpr int kSortTile = 4096;       // keys per one-sweep tile
@@ -130,6 +159,11 @@ struct TreeBufs {
     uint32_t *counts;    // fill counters (fill_count_words)
     uint32_t *lists;     // the fill's top node list (fill_list_words)
     uint32_t *slot;      // [n-1] position of a top node in the top list
+    // with pair_keys set, the fill also writes each child's opening key
+    // open_key(length, open_scale, open_bias) into its pair record's padding
+    // (word 10 left, 11 right)
+    uint32_t pair_keys;
+    float open_scale, open_bias;
 };
 // The builds also write the sorted leaf records t.leaf[rank[b]] = {pos[b],
 // mass[b], 0} (body order in, scattered out), zero `clear_words` words at
@@ -201,6 +235,7 @@ struct AdjointArgs {
     uint32_t *overflow;
     uint32_t n;
     float grav, theta, eps;
+    float open_scale, open_bias;  // open_key's parameters for theta
 };
 void launch_adjoint_pre(const float2 *lx, float2 *lv, float2 *la, uint32_t n, float dt,
                         hipStream_t s);

@@ -54,6 +54,61 @@ __device__ __forceinline__ uint32_t nonzero_flag(uint64_t m) {
     asm volatile("s_cmp_lg_u64 %1, 0\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(m) : "scc");
     return r;
 }
+// The pair walk's step on the scalar unit: from the visiting lanes `mask`
+// and the lanes that accept the left and right child (bl, br), the lanes
+// that open each child, and the next entry: the right child (ch_r, with its
+// open mask) when some lane opens it, else the left one, else the stack's top
+// entry (nmask is then 0 and the caller pops entry sp_next).  The caller
+// writes (ch_l, openl) to slot sp, which becomes an entry when both children
+// are opened: sp_next = sp + 1 (push), sp (one child) or sp - 1 (pop).  With
+// kClamp the top stays at kPairStackTop (a push past it drops the left child:
+// both = 1 reports it); without, the caller guarantees that the tree is less
+// than kPairStackTop levels deep.  Straight-line scalar ALU code on the flags
+// the mask operations set (written in C++, each select becomes a lane-mask
+// materialisation of several instructions).
+template <bool kClamp>
+__device__ __forceinline__ void pair_next(uint64_t mask, uint64_t bl, uint64_t br, uint32_t ch_l,
+                                          uint32_t ch_r, uint32_t sp, uint64_t &openl,
+                                          uint64_t &openr, uint64_t &nmask, uint32_t &next,
+                                          uint32_t &sp_next, uint32_t &both) {
+    // delta = R ? L : L - 1 with L, R the two flags (1: some lane opens)
+    uint32_t hl, hlm1, delta;
+    if constexpr (kClamp) {
+        asm volatile(
+            "s_andn2_b64 %[opl], %[m], %[bl]\n\t"
+            "s_cselect_b32 %[hl], 1, 0\n\t"
+            "s_add_i32 %[hlm1], %[hl], -1\n\t"
+            "s_andn2_b64 %[opr], %[m], %[br]\n\t"
+            "s_cselect_b64 %[nm], %[opr], %[opl]\n\t"
+            "s_cselect_b32 %[nx], %[cr], %[cl]\n\t"
+            "s_cselect_b32 %[dl], %[hl], %[hlm1]\n\t"
+            "s_cselect_b32 %[both], %[hl], 0\n\t"
+            "s_add_u32 %[spn], %[sp], %[dl]\n\t"
+            "s_min_u32 %[spn], %[spn], 63"
+            : [opl] "=&s"(openl), [opr] "=&s"(openr), [nm] "=&s"(nmask), [nx] "=&s"(next),
+              [hl] "=&s"(hl), [hlm1] "=&s"(hlm1), [dl] "=&s"(delta), [spn] "=&s"(sp_next),
+              [both] "=&s"(both)
+            : [m] "s"(mask), [bl] "s"(bl), [br] "s"(br), [cl] "s"(ch_l), [cr] "s"(ch_r),
+              [sp] "s"(sp)
+            : "scc");
+    } else {
+        asm volatile(
+            "s_andn2_b64 %[opl], %[m], %[bl]\n\t"
+            "s_cselect_b32 %[hl], 1, 0\n\t"
+            "s_add_i32 %[hlm1], %[hl], -1\n\t"
+            "s_andn2_b64 %[opr], %[m], %[br]\n\t"
+            "s_cselect_b64 %[nm], %[opr], %[opl]\n\t"
+            "s_cselect_b32 %[nx], %[cr], %[cl]\n\t"
+            "s_cselect_b32 %[dl], %[hl], %[hlm1]\n\t"
+            "s_add_u32 %[spn], %[sp], %[dl]"
+            : [opl] "=&s"(openl), [opr] "=&s"(openr), [nm] "=&s"(nmask), [nx] "=&s"(next),
+              [hl] "=&s"(hl), [hlm1] "=&s"(hlm1), [dl] "=&s"(delta), [spn] "=&s"(sp_next)
+            : [m] "s"(mask), [bl] "s"(bl), [br] "s"(br), [cl] "s"(ch_l), [cr] "s"(ch_r),
+              [sp] "s"(sp)
+            : "scc");
+        both = 0u;
+    }
+}
 __device__ __forceinline__ float uniform_f32(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
@@ -67,6 +122,18 @@ __device__ __forceinline__ float uniform_f32(float v) {
 struct PairPad {
     uint32_t w[6];
 };
+// A whole pair record as one 16-dword scalar load.
+typedef uint32_t RecordWords __attribute__((ext_vector_type(16)));
+// A wave-uniform value moved to a register of its own (s_mov_b32), so that
+// the register it came from can take a new load while the copy stays in use.
+__device__ __forceinline__ uint32_t scalar_copy_u32(uint32_t v) {
+    uint32_t r;
+    asm volatile("s_mov_b32 %0, %1" : "=s"(r) : "s"(v));
+    return r;
+}
+__device__ __forceinline__ float scalar_copy(float v) {
+    return __uint_as_float(scalar_copy_u32(__float_as_uint(v)));
+}
 __device__ __forceinline__ void load_pad(PairPad &p, const_as<uint32_t> *ru) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) p.w[k] = ru[10 + k];
@@ -149,22 +216,49 @@ struct RegStack {
 // Distance term of one lane: lengths, distances and 1/sqrt exactly as the
 // forward walk computes them (barnes_hut.wgsl:81-94), so every walk built on
 // it takes the same opening decisions.  Returns inv = 1/sqrt(|r|^2 + eps^2).
+// The fast arithmetic's squared softened distance |r|^2 + eps^2.
+__device__ __forceinline__ float soft_d2(float rx, float ry, float eps2) {
+    return __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2));
+}
 template <bool kExact>
 __device__ __forceinline__ float inv_dist(float rx, float ry, float eps2) {
     if constexpr (kExact) {
         const float d2 = rx * rx + ry * ry;
         return 1.0f / sqrtf(d2 + eps2);
     } else {
-        return __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, eps2)));
+        return __builtin_amdgcn_rsqf(soft_d2(rx, ry, eps2));
     }
 }
 
 // The reference's opening test, length * inv < theta (barnes_hut.wgsl:89),
 // written as !(length * inv >= theta): the same for every number, and it
 // passes when the product is not a number (a leaf, length kLeafLength, with
-// inv == 0), so every leaf passes and none is ever opened.
+// inv == 0), so every leaf passes and none is ever opened.  The exact
+// arithmetic's test.
 __device__ __forceinline__ bool passes(float length, float inv, float theta) {
     return !(length * inv >= theta);
+}
+// The fast arithmetic's test on a node's opening key (open_key) and the
+// squared softened distance d2: every leaf passes (key -inf or NaN).
+__device__ __forceinline__ bool passes_key(float key, float d2) { return !(key >= d2); }
+
+// One body's opening test of node record c at offset (rx, ry): true when the
+// body accepts the node; inv = 1/sqrt(|r|^2 + eps^2) for its pull.  Every
+// walk of one arithmetic mode decides through this function (the pair walks
+// read the same keys from their records), so all of them, forward and
+// adjoint, take the same decisions.
+template <bool kExact>
+__device__ __forceinline__ bool probe_node(const float4 &c, float rx, float ry, float eps2,
+                                           float theta, float open_scale, float open_bias,
+                                           float &inv) {
+    if constexpr (kExact) {
+        inv = inv_dist<true>(rx, ry, eps2);
+        return passes(c.w, inv, theta);
+    } else {
+        const float d2 = soft_d2(rx, ry, eps2);
+        inv = __builtin_amdgcn_rsqf(d2);
+        return passes_key(open_key(c.w, open_scale, open_bias), d2);
+    }
 }
 
 // Depth-first traversal of one wavefront from (node, mask, c) with `sp`
