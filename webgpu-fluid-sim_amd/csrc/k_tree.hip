@@ -288,11 +288,15 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
         const Split sp = karras_node(delta, i, n32);
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
-        // the walk reads the indices from the pair record
-        *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
+        // the walk reads the indices from the pair record; the fill writes
+        // the records of nodes inside one block whole
+        const uint32_t ib = in_block(sp);
+        if (!ib)
+            *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) =
+                make_uint2(sp.lc, sp.rc);
         tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
         // lbvh.wgsl:155-161, with the fill's in-block flag
-        const uint32_t pw = (uint32_t)i | in_block(sp);
+        const uint32_t pw = (uint32_t)i | ib;
         tb.parent[sp.lc] = pw;
         tb.parent[sp.rc] = pw | kRightChild;
         if (i == 0) tb.parent[0] = kNoNode;
@@ -320,10 +324,12 @@ __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t
         const Split sp = karras_node(delta, i, n32);
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
-        // the walk reads the indices from the pair record
-        *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) = make_uint2(sp.lc, sp.rc);
+        const uint32_t ib = in_block(sp);
+        if (!ib)  // (the fill writes in-block records whole)
+            *reinterpret_cast<uint2 *>(tb.pcm + kPairStride * (size_t)i + 2) =
+                make_uint2(sp.lc, sp.rc);
         tb.range[i] = make_uint2((uint32_t)sp.lo, (uint32_t)sp.hi);
-        const uint32_t pw = (uint32_t)i | in_block(sp);
+        const uint32_t pw = (uint32_t)i | ib;
         tb.parent[sp.lc] = pw;
         tb.parent[sp.rc] = pw | kRightChild;
         tb.status[i] = 0u;
@@ -473,16 +479,49 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
     __syncthreads();
 
     // Write the block's completed nodes where their readers look for them.
+    // A completed internal node writes its own pair record whole (its
+    // children are complete in this block): one 64-byte record and one
+    // 32-byte AABB pair per node, no partial lines.  Only the nodes whose
+    // parent lies outside the block (block tops, and leaves or nodes under a
+    // spanning parent) publish themselves into their parent's slots, which
+    // the upper phases read.
     if (t < (uint64_t)n) {
         float4 c = s_cm[kFillBlock + j];
         c.w = kLeafLength;
         if (n == 1)
             publish(tb, 0u, 0u, c, make_float4(c.x, c.y, c.x, c.y));
-        else
+        else if (!(pw_leaf & kInBlock))
             publish(tb, nint + t, pw_leaf, c, make_float4(c.x, c.y, c.x, c.y));
     }
-    if (t < nint && s_arrive[j] == 2u)
-        publish(tb, t, t ? s_par_int[j] : 0u, s_cm[j], s_bb[j]);
+    if (t < nint && s_arrive[j] == 2u) {
+        const uint32_t pw = t ? s_par_int[j] : 0u;
+        if (t == 0u || !(pw & kInBlock)) publish(tb, t, pw, s_cm[j], s_bb[j]);
+        float4 cd[2], bd[2];
+        const uint32_t cc[2] = {ch.x, ch.y};
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            if (cc[side] >= nint) {
+                cd[side] = s_cm[kFillBlock + (cc[side] - nint - base)];
+                cd[side].w = kLeafLength;
+                bd[side] = make_float4(cd[side].x, cd[side].y, cd[side].x, cd[side].y);
+            } else {
+                cd[side] = s_cm[cc[side] - base];
+                bd[side] = s_bb[cc[side] - base];
+            }
+        }
+        float4 *rec = tb.pcm + kPairStride * (size_t)t;
+        rec[0] = cd[0];
+        rec[1] = cd[1];
+        float4 w2 = make_float4(__uint_as_float(ch.x), __uint_as_float(ch.y), 0.0f, 0.0f);
+        if (tb.pair_keys) {
+            w2.z = open_key(cd[0].w, tb.open_scale, tb.open_bias);
+            w2.w = open_key(cd[1].w, tb.open_scale, tb.open_bias);
+        }
+        rec[2] = w2;
+        rec[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        tb.paabb[2 * (size_t)t] = bd[0];
+        tb.paabb[2 * (size_t)t + 1] = bd[1];
+    }
 }
 
 // Climb from `cur` (its data already published) while the parents' leaf
