@@ -384,7 +384,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     mark(3);
     // phase 3: Karras build (lbvh.wgsl:85-165); it also places the sorted
     // leaf records and zeroes the sort's scratch for the next step
-    const int64_t clear = sort_clear_words(n, s->key_bits, s->radix_bits);
+    const int64_t clear = sort_clear_words(n, s->key_bits, s->radix_bits, s->ensemble);
     const LeafPlace lpl{s->pos, s->mass, s->rank};
     if (s->ensemble)
         launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree, lpl,

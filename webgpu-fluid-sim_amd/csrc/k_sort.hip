@@ -30,13 +30,21 @@ namespace nbs {
 
 namespace {
 
-// Workgroup size per digit width: 8- and 10-bit digits rank 4 keys per
-// thread in 1024-thread workgroups (16 wavefronts hide the load and ranking
-// latencies); 11-bit digits keep 256 threads x 16 keys, as the
+// Tile size by problem size (sort_tile_for): a pass lasts about one tile's
+// load, ranking, look-back and store, so small sorts take small tiles (more
+// workgroups at once) and large ones large tiles (shorter look-back
+// chains): 2048 keys below 2^16, 4096 up to 2^21, 8192 above (32-bit keys;
+// 64-bit keys stop at 4096, as keys, values and the per-wavefront digit
+// counts must fit in LDS).  Workgroup size per tile and digit width: 8- and
+// 10-bit digits rank 4 or 8 keys per thread (16 wavefronts hide the load
+// and ranking latencies), 11-bit digits 8 or 16, in fewer threads, as the
 // per-wavefront digit counts (kWaves x 2^RB words) must fit in LDS.
-constexpr int kTile = kSortTile;
-template <int RB>
-constexpr int sort_threads() { return RB <= 10 ? 1024 : 256; }
+template <int RB, int kTile>
+constexpr int sort_threads() {
+    return kTile == 2048 ? (RB <= 10 ? 512 : 256)
+                         : (kTile == 4096 ? (RB <= 10 ? 1024 : 256) : (RB <= 10 ? 1024 : 512));
+}
+constexpr int kTileMin = 2048;
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
@@ -72,7 +80,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
 //
 // kLeaf (the last pass): each item's body also gets its output position,
 // rank[body] (a scattered write, which no barrier waits for).
-template <typename K, int RB, int kThreads, bool kLeaf>
+template <typename K, int RB, int kTile, int kThreads, bool kLeaf>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_count,
@@ -284,21 +292,21 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     }
 }
 
-template <typename K, int RB>
-int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int passes,
-            const SortScratch &sc, const LeafGather &lg, hipStream_t s) {
+template <typename K, int RB, int kTile>
+int sort_rb_tile(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int passes,
+                 const SortScratch &sc, const LeafGather &lg, hipStream_t s) {
     constexpr int D = 1 << RB;
-    const int64_t tiles = sort_tiles(n);
+    const int64_t tiles = (n + kTile - 1) / kTile;
     K *ki = keys_a, *ko = keys_b;
     uint32_t *vi = vals_a, *vo = vals_b;
-    constexpr int T = sort_threads<RB>();
+    constexpr int T = sort_threads<RB, kTile>();
     for (int p = 0; p < passes; ++p) {
         if (p == passes - 1 && lg.rank)
-            onesweep_kernel<K, RB, T, true><<<(unsigned)tiles, T, 0, s>>>(
+            onesweep_kernel<K, RB, kTile, T, true><<<(unsigned)tiles, T, 0, s>>>(
                 ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
                 sc.counters + p, lg);
         else
-            onesweep_kernel<K, RB, T, false><<<(unsigned)tiles, T, 0, s>>>(
+            onesweep_kernel<K, RB, kTile, T, false><<<(unsigned)tiles, T, 0, s>>>(
                 ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
                 sc.counters + p, lg);
         K *tk = ki;
@@ -309,6 +317,18 @@ int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n,
         vo = tv;
     }
     return passes & 1;
+}
+
+template <typename K, int RB>
+int sort_rb(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t n, int passes,
+            const SortScratch &sc, const LeafGather &lg, hipStream_t s) {
+    const int tile = sort_tile_for(n, sizeof(K) == 8);
+    if (tile == 2048)
+        return sort_rb_tile<K, RB, 2048>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
+    if constexpr (sizeof(K) == 4)
+        if (tile == 8192)
+            return sort_rb_tile<K, RB, 8192>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
+    return sort_rb_tile<K, RB, 4096>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
 }
 
 template <typename K>
@@ -325,7 +345,14 @@ int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t 
 
 }  // namespace
 
-int64_t sort_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
+int sort_tile_for(int64_t n, bool wide_keys) {
+    if (n < (int64_t(1) << 16)) return 2048;
+    if (wide_keys || n <= (int64_t(1) << 21)) return 4096;
+    return 8192;
+}
+
+// the most tiles a sort of n keys can have (the smallest tile)
+int64_t sort_tiles(int64_t n) { return (n + kTileMin - 1) / kTileMin; }
 
 int sort_passes(int key_bits, int radix_bits) { return (key_bits + radix_bits - 1) / radix_bits; }
 
@@ -336,9 +363,10 @@ int64_t sort_scratch_words(int64_t cap) {
     return kSortHistWords + 64 + (int64_t)4 * 2048 * sort_tiles(cap);
 }
 
-int64_t sort_clear_words(int64_t n, int key_bits, int radix_bits) {
-    return kSortHistWords + 64 +
-           (int64_t)sort_passes(key_bits, radix_bits) * sort_tiles(n) * ((int64_t)1 << radix_bits);
+int64_t sort_clear_words(int64_t n, int key_bits, int radix_bits, bool wide_keys) {
+    const int tile = sort_tile_for(n, wide_keys);
+    return kSortHistWords + 64 + (int64_t)sort_passes(key_bits, radix_bits) *
+                                     ((n + tile - 1) / tile) * ((int64_t)1 << radix_bits);
 }
 
 void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t cap) {

@@ -70,7 +70,6 @@ inline void open_params(float theta, float &scale, float &bias) {
 }
 
 // ---------------------------------------------------------------- sort scratch
-constexpr int kSortTile = 4096;       // keys per one-sweep tile
 constexpr int kSortMaxPasses = 5;     // 38-bit ensemble keys at 8 bits per digit
 constexpr int kSortHistWords = kSortMaxPasses * 2048;  // [pass][2^radix_bits] digit counts
 // The persistent walk's queue counter: a word of the sort's counter block
@@ -83,11 +82,12 @@ struct SortScratch {
     uint32_t *counters;  // [passes] dynamic tile ids
     int64_t words;       // u32 words of the whole scratch
 };
-int64_t sort_tiles(int64_t n);
+int sort_tile_for(int64_t n, bool wide_keys);  // keys per one-sweep tile (64-bit keys: wide)
+int64_t sort_tiles(int64_t n);                 // the most tiles of any tile size
 int sort_passes(int key_bits, int radix_bits);
-int64_t sort_scratch_words(int64_t cap);  // enough for every radix width
+int64_t sort_scratch_words(int64_t cap);  // enough for every radix width and tile size
 // Words to clear (from sc.hist on) after a sort of n keys, for the next one.
-int64_t sort_clear_words(int64_t n, int key_bits, int radix_bits);
+int64_t sort_clear_words(int64_t n, int key_bits, int radix_bits, bool wide_keys);
 void sort_bind_scratch(SortScratch &sc, uint32_t *base, int64_t cap);
 
 // ---------------------------------------------------------------- keys
