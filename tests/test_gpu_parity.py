@@ -172,6 +172,49 @@ def test_fast_mode_pinned_against_oracle():
         assert np.all(ok), f"{(~ok).sum()} flipped bodies match neither nudged oracle walk"
 
 
+@pytest.mark.parametrize("math", ["exact", "fast"])
+@pytest.mark.parametrize("theta", [0.0, -0.5, float("nan")], ids=["zero", "negative", "nan"])
+def test_theta_without_approximation_opens_every_node(theta, math):
+    """theta <= 0 or NaN: the reference's test length * inv < theta
+    (barnes_hut.wgsl:89) fails for every internal node, so each body opens all
+    n - 1 of them and accepts the leaves, a direct sum in tree order: the
+    oracle's theta = 0 walk, counter for counter; its accelerations bit for
+    bit in the exact arithmetic, and in the fast one (opening keys; the pair
+    walk's own summation order) within 1e-5 or ROUND_BOUND of the sum of the
+    terms' lengths."""
+    from oracle import oracle as O
+
+    mass, pos, vel = nbody_sim.generate("plummer", 700, seed=9)
+    cfg, _ = params(theta=theta, eps=1e-2)
+    _, P0 = params(theta=0.0, eps=1e-2)
+    sim = nbody_sim.Simulation(cfg.with_(mathMode=math))
+    try:
+        sim.setBodies(mass, pos, vel)
+        sim.setDebug(accel=True, stats=True)
+        sim.step(1)
+        sim.synchronize()
+        acc = sim.debugBuffer("accel")
+        totals = sim.walkStats()
+        # per-body counters: the fast arithmetic's pair walk
+        counts = sim.debugBuffer("walk_counts").astype(np.int64) if math == "fast" else None
+    finally:
+        sim.close()
+    r = oracle_step(P0, mass, pos, vel)
+    assert np.all(r["stats"][:, 1] == len(mass) - 1)
+    assert totals["accepted"] == int(r["stats"][:, 0].sum())
+    assert totals["opened"] == int(r["stats"][:, 1].sum())
+    if counts is not None:
+        assert np.array_equal(counts, r["stats"][:, :2])
+    if math == "exact":
+        assert np.array_equal(acc, r["accel"])
+    else:
+        tree = (r["idx"], r["node_cm"], r["left"], r["right"], mass, pos)
+        _, _, _, tnorm = O.walk(*tree, P0, margin=True)
+        dabs = np.linalg.norm(acc.astype(np.float64) - r["accel"], axis=1)
+        assert np.all(dabs <= np.maximum(1e-5 * np.linalg.norm(r["accel"], axis=1),
+                                         ROUND_BOUND * tnorm))
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 257])
 def test_small_sizes(n):
     rng = np.random.default_rng(n)

@@ -50,7 +50,9 @@ struct StepParams {
 // where len / d lies within a few units in the last place of theta.  Leaves
 // (length -inf) get the key -inf, or NaN when theta <= 0: both pass for every
 // d2.  theta <= 0 (or NaN) makes the bias +inf, so every internal node opens,
-// as the reference's test does for such a theta.  The pair records carry the
+// as the reference's test does for such a theta.  A body whose d2 is NaN (a
+// NaN position) passes every node: its pull is NaN, as in the reference,
+// which opens every node for it instead.  The pair records carry the
 // children's keys in their first two padding words (monopole), written by the
 // fill with this same expression.
 __host__ __device__ inline float open_key(float len, float scale, float bias) {

@@ -231,12 +231,13 @@ __device__ __forceinline__ float inv_dist(float rx, float ry, float eps2) {
 }
 
 // The reference's opening test, length * inv < theta (barnes_hut.wgsl:89),
-// written as !(length * inv >= theta): the same for every number, and it
-// passes when the product is not a number (a leaf, length kLeafLength, with
-// inv == 0), so every leaf passes and none is ever opened.  The exact
-// arithmetic's test.
+// with its leaf rule (a leaf is accepted whatever the test says,
+// barnes_hut.wgsl:94): a leaf's length is kLeafLength, the only negative
+// one.  A product that is not a number (an internal node of length 0 at
+// distance 0, or theta NaN) fails, so the node opens, as in the reference.
+// The exact arithmetic's test.
 __device__ __forceinline__ bool passes(float length, float inv, float theta) {
-    return !(length * inv >= theta);
+    return length * inv < theta || length < 0.0f;
 }
 // The fast arithmetic's test on a node's opening key (open_key) and the
 // squared softened distance d2: every leaf passes (key -inf or NaN).
