@@ -753,6 +753,55 @@ __device__ __forceinline__ void setup_entry(const TreeBufs &tb, ListLds &ls,
     ls.cnt[t] = 0u;
 }
 
+// A top list longer than kLdsList is resolved in parts once every phase-2
+// workgroup is done: part h < H holds the top nodes whose leaf range lies in
+// hyper-block h (2^kHyperShift leaves), part H the ones spanning
+// hyper-blocks.  Set up entry t of a part (cross: part H); tb.slot holds
+// each node's entry in its part.  Every input that is not an entry of the
+// part is in memory: phase 1's records and the earlier parts' as written by
+// this workgroup, the phase-2 workgroups' at device scope.
+constexpr int kHyperShift = 20;
+__device__ __forceinline__ void setup_part_entry(const TreeBufs &tb, ListLds &ls,
+                                                 const uint32_t *part, uint32_t t, uint32_t nint,
+                                                 bool cross) {
+    const uint32_t p = part[t];
+    const uint32_t pw = tb.parent[p];
+    const uint2 ch = tb.child[p];
+    const uint32_t par = pw & kParentMask;
+    uint32_t q = kNone;
+    // the parent, a top node, is in the part when it lies in the same
+    // hyper-block as the node, or when both span hyper-blocks
+    if (par < nint && (cross || !spans(tb.range[par], kHyperShift))) q = tb.slot[par];
+    uint32_t pend = 0u;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t c = side ? ch.y : ch.x;
+        bool entry = false, mid = false;
+        if (c < nint) {
+            const uint2 rc = tb.range[c];
+            if (spans(rc, kSuperShift))
+                entry = !cross || spans(rc, kHyperShift);
+            else
+                mid = spans(rc, kBlockShift);
+        }
+        if (entry) {
+            ++pend;
+        } else if (mid) {  // a phase-2 node: another XCD's L2 may hold the line
+            ls.in[t][2 * side] = load_device_f4(tb.pcm + kPairStride * (size_t)p + side);
+            ls.in[t][2 * side + 1] = load_device_f4(tb.paabb + 2 * (size_t)p + side);
+        } else {
+            ls.in[t][2 * side] = tb.pcm[kPairStride * (size_t)p + side];
+            ls.in[t][2 * side + 1] = tb.paabb[2 * (size_t)p + side];
+        }
+    }
+    ls.node[t] = p;
+    ls.pw[t] = pw;
+    ls.par[t] = q;
+    ls.pend[t] = pend;
+    ls.src[t] = 0u;
+    ls.cnt[t] = 0u;
+}
+
 // Phases 2 and 3, S + 1 workgroups.  Workgroup sb < S resolves the phase-2
 // list of super-block sb in LDS (or, longer than lds_limit, climbs from the
 // phase-2 nodes' other children), then, behind a device-scope fence, sets
@@ -760,15 +809,19 @@ __device__ __forceinline__ void setup_entry(const TreeBufs &tb, ListLds &ls,
 // climb that needs a record from a phase-2 workgroup waits for that
 // super-block's flag and reads the record at device scope, so the top
 // levels resolve while the lower super-blocks still run (the deep chains of
-// a dense region are not on one path with every super-block).  A top list
-// longer than lds_limit waits for every flag and climbs from the top nodes'
-// other children.  Workgroup S, the last one dispatched, is the only one
-// that waits, so every flag it waits for is set.  It then clears the
-// counters and flags for the next build.
+// a dense region are not on one path with every super-block).  A longer top
+// list waits for every flag and is resolved in LDS part by part (one part
+// per hyper-block, then the nodes spanning hyper-blocks); one with a part
+// longer than kLdsList, or any with the climb test hook (lds_limit 0),
+// climbs from the top nodes' other children.  Workgroup S, the last one
+// dispatched, is the only one that waits, so every flag it waits for is
+// set.  It then clears the counters and flags for the next build.
 __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBufs tb,
                                                               uint32_t num_super,
                                                               uint32_t lds_limit) {
     __shared__ ListLds ls;
+    constexpr uint32_t kMaxParts = 1024;  // hyper-blocks of trees below 2^30 leaves, + 1
+    __shared__ uint32_t s_part_len[kMaxParts + 1];
     const uint32_t t = threadIdx.x;
     const uint32_t sb = blockIdx.x;
     const uint32_t nint = (uint32_t)(n - 1);
@@ -821,6 +874,41 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     } else {
         wait_all();
         __threadfence();  // acquire: the phase-2 records, past this XCD's L2
+        // the parts' lengths: part h < H of the nodes in hyper-block h, part H
+        const uint32_t H = (uint32_t)((n + (int64_t(1) << kHyperShift) - 1) >> kHyperShift);
+        for (uint32_t h = t; h <= H; h += kFillBlock) s_part_len[h] = 0u;
+        __syncthreads();
+        auto part_of = [&](uint32_t node) {
+            const uint2 r = tb.range[node];
+            return spans(r, kHyperShift) ? H : (r.x >> kHyperShift);
+        };
+        for (uint32_t k = t; k < m3; k += kFillBlock) atomicAdd(&s_part_len[part_of(top[k])], 1u);
+        __syncthreads();
+        bool fits = lds_limit > 0u;
+        for (uint32_t h = 0; h <= H; ++h) fits = fits && s_part_len[h] <= kLdsList;
+        if (fits) {
+            // one part at a time: its nodes listed (in the phase-2 lists'
+            // space, read by now), each node's entry in tb.slot
+            uint32_t *part = tb.lists;
+            for (uint32_t h = 0; h <= H; ++h) {
+                if (s_part_len[h] == 0u) continue;
+                if (t == 0) ls.nstart = 0u;
+                __syncthreads();
+                for (uint32_t k = t; k < m3; k += kFillBlock) {
+                    const uint32_t node = top[k];
+                    if (part_of(node) != h) continue;
+                    const uint32_t e = atomicAdd(&ls.nstart, 1u);
+                    part[e] = node;
+                    tb.slot[node] = e;
+                }
+                __syncthreads();
+                const uint32_t mp = s_part_len[h];
+                if (t < mp) setup_part_entry(tb, ls, part, t, nint, h == H);
+                __syncthreads();
+                resolve_list<false>(tb, ls, mp);
+                __syncthreads();
+            }
+        } else {
         // climb from every child of a top node that is not a top node
         for (uint32_t k = t; k < m3; k += kFillBlock) {
             const uint2 ch = tb.child[top[k]];
@@ -829,6 +917,7 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
                 const uint32_t c = side ? ch.y : ch.x;
                 if (!(c < nint && spans(tb.range[c], kSuperShift))) climb_wg(tb, nint, c, 0u, nint);
             }
+        }
         }
     }
     __syncthreads();
