@@ -3,7 +3,8 @@
 Bit-exact: sorted Morton keys, the sorted permutation (ties by body index),
 the LBVH child/parent arrays, node mass/COM/AABB/length, and -- in the exact
 arithmetic mode -- accelerations and the kicked/drifted state.  The fast mode
-(hardware rsqrt + FMA) is held to 1e-5 relative and identical walk counters
+(hardware rsqrt + FMA, a square-root-free opening test) is held to 1e-5
+relative and identical walk counters
 per body, except bodies whose opening test lies within rounding of theta,
 which must equal the oracle walk at a nudged theta instead.
 """
@@ -101,9 +102,11 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(a[k], b[k]), k
 
 
-# The fast mode's hardware rsqrt and fused multiply-adds move length * inv by
-# a few binary32 ULPs; an opening test closer than this to theta (relative,
-# about 30 ULPs) may take the other branch than the oracle's.
+# The fast mode's opening test (length |length| / theta^2 against the squared
+# softened distance, both rounded in binary32) moves the decision boundary by
+# a few ULPs from the reference's length * inv < theta; an opening test closer
+# than this to theta (relative, about 30 ULPs) may take the other branch than
+# the oracle's.
 FLIP_MARGIN = 2e-6
 NUDGE = 1e-6  # relative theta nudge that flips exactly such near-boundary tests
 # Rounding of the fast sums (FMA, hardware rsqrt, G applied once) relative to
