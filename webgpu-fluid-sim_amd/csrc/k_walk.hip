@@ -704,12 +704,7 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
             }
             const uint64_t bl = __ballot(passes_key(kl, d2l));
             const uint64_t br = __ballot(passes_key(kr, d2r));
-            uint64_t accl = mask & bl, accr = mask & br;
-            if constexpr (kSelf) {
-                accl &= __ballot(my_leaf != ch.x);
-                accr &= __ballot(my_leaf != ch.y);
-            }
-            // leaves always pass, so an opened child is an internal node
+            // leaves always pass, so an opened child is an internal node.
             // The next entry: the right child first, the left one waits on
             // the stack (written to the free slot every time, kept by
             // advancing the top); with neither child opened, the top entry
@@ -719,9 +714,13 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
             // top; ensembles' 38-bit keys and larger trees keep the clamp.
             constexpr bool kClamp = kEnsemble || kStats || !kOff32;
             uint32_t next, sp_next, both;
-            uint64_t openl, openr, nmask;
-            pair_next<kClamp>(mask, bl, br, ch.x, ch.y, (uint32_t)sp, openl, openr, nmask, next,
-                              sp_next, both);
+            uint64_t accl, accr, openl, openr;
+            pair_next<kClamp>(mask, bl, br, ch.x, ch.y, (uint32_t)sp, accl, accr, openl, openr,
+                              next, sp_next, both);  // mask: the next entry's
+            if constexpr (kSelf) {
+                accl &= __ballot(my_leaf != ch.x);
+                accr &= __ballot(my_leaf != ch.y);
+            }
             if constexpr (kStats) {
                 n_acc += __builtin_popcountll(accl) + __builtin_popcountll(accr);
                 n_open += __builtin_popcountll(openl) + __builtin_popcountll(openr);
@@ -743,7 +742,7 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
                 }
             }
             sp = (int)sp_next;
-            if (nmask == 0) st.get(sp, next, nmask);
+            if (mask == 0) st.get(sp, next, mask);
             // what the pulls take from this record, copied out of the
             // record's registers so that the next record lands in them whole
             const float ml = scalar_copy(cl.z), mr = scalar_copy(cr.z);
@@ -770,7 +769,6 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
             }
             if (accr) pull(mr, q.w[3], q.w[4], q.w[5], rxr, ryr, d2r, accr);
             if (accl) pull(ml, q.w[0], q.w[1], q.w[2], rxl, ryl, d2l, accl);
-            mask = nmask;
             if (mask == 0) break;
         }
     }

@@ -56,58 +56,58 @@ __device__ __forceinline__ uint32_t nonzero_flag(uint64_t m) {
 }
 // The pair walk's step on the scalar unit: from the visiting lanes `mask`
 // and the lanes that accept the left and right child (bl, br), the lanes
-// that open each child, and the next entry: the right child (ch_r, with its
-// open mask) when some lane opens it, else the left one, else the stack's top
-// entry (nmask is then 0 and the caller pops entry sp_next).  The caller
-// writes (ch_l, openl) to slot sp, which becomes an entry when both children
-// are opened: sp_next = sp + 1 (push), sp (one child) or sp - 1 (pop).  With
-// kClamp the top stays at kPairStackTop (a push past it drops the left child:
-// both = 1 reports it); without, the caller guarantees that the tree is less
+// that accept (accl, accr) and open (openl, openr) each child, and the next
+// entry, written over `mask`: the right child (ch_r, with its open mask) when
+// some lane opens it, else the left one, else the stack's top entry (mask is
+// then 0 and the caller pops entry sp_next).  The caller writes (ch_l, openl)
+// to slot sp, which becomes an entry when both children are opened: sp_next =
+// sp + 1 (push), sp (one child) or sp - 1 (pop).  With kClamp the top stays
+// at kPairStackTop (a push there lands on the free slot and the left child is
+// dropped: both = 1 reports it); without, the caller guarantees a tree less
 // than kPairStackTop levels deep.  Straight-line scalar ALU code on the flags
 // the mask operations set (written in C++, each select becomes a lane-mask
 // materialisation of several instructions).
 template <bool kClamp>
-__device__ __forceinline__ void pair_next(uint64_t mask, uint64_t bl, uint64_t br, uint32_t ch_l,
-                                          uint32_t ch_r, uint32_t sp, uint64_t &openl,
-                                          uint64_t &openr, uint64_t &nmask, uint32_t &next,
-                                          uint32_t &sp_next, uint32_t &both) {
+__device__ __forceinline__ void pair_next(uint64_t &mask, uint64_t bl, uint64_t br, uint32_t ch_l,
+                                          uint32_t ch_r, uint32_t sp, uint64_t &accl,
+                                          uint64_t &accr, uint64_t &openl, uint64_t &openr,
+                                          uint32_t &next, uint32_t &sp_next, uint32_t &both) {
     // delta = R ? L : L - 1 with L, R the two flags (1: some lane opens)
     uint32_t hl, hlm1, delta;
+#define NBS_PAIR_NEXT_HEAD                     \
+    "s_and_b64 %[al], %[m], %[bl]\n\t"        \
+    "s_and_b64 %[ar], %[m], %[br]\n\t"        \
+    "s_andn2_b64 %[opl], %[m], %[bl]\n\t"     \
+    "s_cselect_b32 %[hl], 1, 0\n\t"           \
+    "s_add_i32 %[hlm1], %[hl], -1\n\t"        \
+    "s_andn2_b64 %[opr], %[m], %[br]\n\t"     \
+    "s_cselect_b64 %[m], %[opr], %[opl]\n\t"  \
+    "s_cselect_b32 %[nx], %[cr], %[cl]\n\t"   \
+    "s_cselect_b32 %[dl], %[hl], %[hlm1]\n\t"
+#define NBS_PAIR_NEXT_OUT                                                                  \
+    [m] "+s"(mask), [al] "=&s"(accl), [ar] "=&s"(accr), [opl] "=&s"(openl),              \
+        [opr] "=&s"(openr), [nx] "=&s"(next), [hl] "=&s"(hl), [hlm1] "=&s"(hlm1),          \
+        [dl] "=&s"(delta), [spn] "=&s"(sp_next)
+#define NBS_PAIR_NEXT_IN \
+    [bl] "s"(bl), [br] "s"(br), [cl] "s"(ch_l), [cr] "s"(ch_r), [sp] "s"(sp)
     if constexpr (kClamp) {
-        asm volatile(
-            "s_andn2_b64 %[opl], %[m], %[bl]\n\t"
-            "s_cselect_b32 %[hl], 1, 0\n\t"
-            "s_add_i32 %[hlm1], %[hl], -1\n\t"
-            "s_andn2_b64 %[opr], %[m], %[br]\n\t"
-            "s_cselect_b64 %[nm], %[opr], %[opl]\n\t"
-            "s_cselect_b32 %[nx], %[cr], %[cl]\n\t"
-            "s_cselect_b32 %[dl], %[hl], %[hlm1]\n\t"
-            "s_cselect_b32 %[both], %[hl], 0\n\t"
-            "s_add_u32 %[spn], %[sp], %[dl]\n\t"
-            "s_min_u32 %[spn], %[spn], 63"
-            : [opl] "=&s"(openl), [opr] "=&s"(openr), [nm] "=&s"(nmask), [nx] "=&s"(next),
-              [hl] "=&s"(hl), [hlm1] "=&s"(hlm1), [dl] "=&s"(delta), [spn] "=&s"(sp_next),
-              [both] "=&s"(both)
-            : [m] "s"(mask), [bl] "s"(bl), [br] "s"(br), [cl] "s"(ch_l), [cr] "s"(ch_r),
-              [sp] "s"(sp)
-            : "scc");
+        asm volatile(NBS_PAIR_NEXT_HEAD
+                     "s_cselect_b32 %[both], %[hl], 0\n\t"
+                     "s_add_u32 %[spn], %[sp], %[dl]"
+                     : NBS_PAIR_NEXT_OUT, [both] "=&s"(both)
+                     : NBS_PAIR_NEXT_IN
+                     : "scc");
+        sp_next = min(sp_next, (uint32_t)kPairStackTop);
     } else {
-        asm volatile(
-            "s_andn2_b64 %[opl], %[m], %[bl]\n\t"
-            "s_cselect_b32 %[hl], 1, 0\n\t"
-            "s_add_i32 %[hlm1], %[hl], -1\n\t"
-            "s_andn2_b64 %[opr], %[m], %[br]\n\t"
-            "s_cselect_b64 %[nm], %[opr], %[opl]\n\t"
-            "s_cselect_b32 %[nx], %[cr], %[cl]\n\t"
-            "s_cselect_b32 %[dl], %[hl], %[hlm1]\n\t"
-            "s_add_u32 %[spn], %[sp], %[dl]"
-            : [opl] "=&s"(openl), [opr] "=&s"(openr), [nm] "=&s"(nmask), [nx] "=&s"(next),
-              [hl] "=&s"(hl), [hlm1] "=&s"(hlm1), [dl] "=&s"(delta), [spn] "=&s"(sp_next)
-            : [m] "s"(mask), [bl] "s"(bl), [br] "s"(br), [cl] "s"(ch_l), [cr] "s"(ch_r),
-              [sp] "s"(sp)
-            : "scc");
+        asm volatile(NBS_PAIR_NEXT_HEAD "s_add_u32 %[spn], %[sp], %[dl]"
+                     : NBS_PAIR_NEXT_OUT
+                     : NBS_PAIR_NEXT_IN
+                     : "scc");
         both = 0u;
     }
+#undef NBS_PAIR_NEXT_HEAD
+#undef NBS_PAIR_NEXT_OUT
+#undef NBS_PAIR_NEXT_IN
 }
 __device__ __forceinline__ float uniform_f32(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
