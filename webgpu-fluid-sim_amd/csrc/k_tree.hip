@@ -209,27 +209,103 @@ __device__ __forceinline__ void list_node(const TreeBufs &tb, uint32_t i, const 
     }
 }
 
-template <typename Delta>
-__device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n) {
-    // lbvh.wgsl:101-152; probe i + d * x is written as a select on d
-    const auto dl = delta.at(i);
-    const int d = dl(i + 1) > dl(i - 1) ? 1 : -1;
-    const int delta_min = dl(i - d);
-    int l_max = 2;
-    while (dl(d > 0 ? i + l_max : i - l_max) > delta_min) l_max *= 2;
-    int l = 0;
-    for (int t = l_max >> 1; t >= 1; t >>= 1)
-        if (dl(d > 0 ? i + l + t : i - l - t) > delta_min) l += t;
-    const int j = d > 0 ? i + l : i - l;
-    const int delta_node = dl(j);
-    int s = 0;
-    int t = (l + 1) >> 1;
-    for (;;) {
-        if (dl(d > 0 ? i + s + t : i - s - t) > delta_node) s += t;
-        if (t == 1) break;
-        t = (t + 1) >> 1;
+__device__ __forceinline__ int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ uint32_t bcast(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ uint64_t bcast(uint64_t v, int lane) {
+    return (uint64_t)bcast((uint32_t)v, lane) | ((uint64_t)bcast((uint32_t)(v >> 32), lane) << 32);
+}
+
+// The last step x in [lo, hi) along direction d from position i at which the
+// common prefix with key ki exceeds `bound`, given that it does at lo and
+// does not at hi (every argument wavefront-uniform, every lane active).
+// Each round the 64 lanes probe 63 evenly spaced steps at once, so R
+// positions take log64(R) rounds of independent loads where a binary
+// search takes log2(R) dependent ones.
+template <typename Delta, typename K>
+__device__ __forceinline__ int wave_search(const Delta &delta, K ki, int i, int d, int bound, int lo,
+                                           int hi) {
+    const int lane = threadIdx.x & 63;
+    const typename Delta::Node nd{delta, ki, i};
+    while (hi - lo > 1) {
+        const int step = (hi - lo + 63) / 64;
+        const int x = lo + lane * step;
+        const bool p = lane > 0 && x < hi && nd(d > 0 ? i + x : i - x) > bound;
+        // the prefix only shrinks with distance: lanes 1..c hold
+        const int c = __builtin_popcountll(__ballot(p));
+        const int next_lo = lo + c * step;
+        hi = min(lo + (c + 1) * step, hi);
+        lo = next_lo;
     }
-    Split r;
+    return lo;
+}
+
+// lbvh.wgsl:101-152 for node i (valid lanes; every lane of the wavefront
+// calls it).  Both searches look for the end of a run of steps whose common
+// prefix with key i exceeds a bound (the run ends at a unique step, so any
+// correct search returns the reference's split).  A lane searches inside the
+// workgroup's key window (LDS); the nodes whose answer lies beyond it (wide
+// leaf ranges: about one node in 256) are searched by the whole wavefront
+// together, one node after the other.  Probe i + d * x is written as a
+// select on d.
+template <typename Delta>
+__device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n, bool valid) {
+    using Node = typename Delta::Node;
+    using K = decltype(Node::ki);
+    const int lane = threadIdx.x & 63;
+    K ki = 0;
+    int d = 1, delta_min = 0, xw = 0, l = 0;
+    bool far = false;
+    if (valid) {
+        const Node dl = delta.at(i);
+        ki = dl.ki;
+        d = dl(i + 1) > dl(i - 1) ? 1 : -1;
+        delta_min = dl(i - d);
+        // the farthest step inside the window, and inside the array
+        xw = d > 0 ? delta.k.lo + delta.k.len - 1 - i : i - delta.k.lo;
+        const int xa = d > 0 ? n - 1 - i : i;
+        int l_max = 2;
+        while (l_max <= xw && dl(d > 0 ? i + l_max : i - l_max) > delta_min) l_max *= 2;
+        far = l_max > xw && xw < xa && dl(d > 0 ? i + xw : i - xw) > delta_min;
+        if (!far)  // the range ends inside the window: steps beyond xw fail
+            for (int t = l_max >> 1; t >= 1; t >>= 1)
+                if (l + t <= xw && dl(d > 0 ? i + l + t : i - l - t) > delta_min) l += t;
+    }
+    for (uint64_t m = __ballot(far); m; m &= m - 1) {
+        const int ld = __builtin_ctzll(m);
+        const int xa = bcast(d, ld) > 0 ? n - 1 - bcast(i, ld) : bcast(i, ld);
+        const int r = wave_search(delta, bcast(ki, ld), bcast(i, ld), bcast(d, ld),
+                                  bcast(delta_min, ld), bcast(xw, ld), xa + 1);
+        if (lane == ld) l = r;
+    }
+    int s = 0, delta_node = 0;
+    bool far2 = false;
+    if (valid) {
+        const Node dl{delta, ki, i};
+        delta_node = dl(d > 0 ? i + l : i - l);
+        int bound = l;
+        if (l > xw) {  // a wide node: is the split beyond the window too?
+            far2 = dl(d > 0 ? i + xw : i - xw) > delta_node;
+            bound = xw;
+        }
+        if (!far2) {
+            int t = (bound + 1) >> 1;
+            for (;;) {
+                if (dl(d > 0 ? i + s + t : i - s - t) > delta_node) s += t;
+                if (t == 1) break;
+                t = (t + 1) >> 1;
+            }
+        }
+    }
+    for (uint64_t m = __ballot(far2); m; m &= m - 1) {
+        const int ld = __builtin_ctzll(m);
+        const int r = wave_search(delta, bcast(ki, ld), bcast(i, ld), bcast(d, ld),
+                                  bcast(delta_node, ld), bcast(xw, ld), bcast(l, ld));
+        if (lane == ld) s = r;
+    }
+    Split r{};
+    const int j = d > 0 ? i + l : i - l;
     r.gamma = d > 0 ? i + s : i - s - 1;
     r.lo = i < j ? i : j;
     r.hi = i < j ? j : i;
@@ -282,10 +358,14 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
     const DeltaU32 delta{load_window(keys, n32, s_keys), n32};
     for (int r = 0; r < kBuildPerThread; ++r) {
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
-        if (i >= n32 - 1) return;
-        place_leaf(tb, lpl, i);
-        if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
-        const Split sp = karras_node(delta, i, n32);
+        const bool valid = i < n32 - 1;
+        if (valid) {
+            place_leaf(tb, lpl, i);
+            if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
+        }
+        // every lane takes part in its wavefront's searches of wide nodes
+        const Split sp = karras_node(delta, i, n32, valid);
+        if (!valid) return;
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
         // the walk reads the indices from the pair record; the fill writes
@@ -318,10 +398,14 @@ __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t
     const DeltaU64 delta{load_window(keys, n32, s_keys), sim_off, n32};
     for (int r = 0; r < kBuildPerThread; ++r) {
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
-        if (i >= n32 - 1) return;
-        place_leaf(tb, lpl, i);
-        if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
-        const Split sp = karras_node(delta, i, n32);
+        const bool valid = i < n32 - 1;
+        if (valid) {
+            place_leaf(tb, lpl, i);
+            if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
+        }
+        // every lane takes part in its wavefront's searches of wide nodes
+        const Split sp = karras_node(delta, i, n32, valid);
+        if (!valid) return;
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
         const uint32_t ib = in_block(sp);
