@@ -177,10 +177,15 @@ int nbody_use_graph(nbody_sim *sim, int32_t steps_per_graph);
  * acceleration; NBODY_DEBUG_STATS counts accepted and opened nodes;
  * NBODY_DEBUG_FILL_CLIMB makes the tree fill complete its upper levels by
  * climbing through global arrival counters (the path it takes on its own
- * only when a level's node list is too long for workgroup memory). */
+ * only when a level's node list is too long for workgroup memory);
+ * NBODY_DEBUG_SORT_RADIX makes every sort take the radix passes (plain
+ * handles otherwise sort up to 4096 bodies in one workgroup, and up to
+ * 4194304 between the previous step's splitters; the permutation is the
+ * same). */
 #define NBODY_DEBUG_ACCEL 1
 #define NBODY_DEBUG_STATS 2
 #define NBODY_DEBUG_FILL_CLIMB 4
+#define NBODY_DEBUG_SORT_RADIX 8
 int nbody_set_debug(nbody_sim *sim, int32_t flags);
 
 /* Intermediate buffers of the last step (reference buffers in comments). */

@@ -107,6 +107,13 @@ struct nbody_sim {
     uint2 *body_counts = nullptr;  // per-body walk counters (stats mode)
     uint32_t *overflow = nullptr;
     int result_buf = 0;  // which sort buffer holds the last sorted keys/values
+    // bucket sort (plain handles): the splitters the last build wrote, each
+    // body's bucket, and whether the splitters are those of the current
+    // bodies' last step (else the sort takes the radix passes)
+    uint32_t *split = nullptr;
+    uint16_t *bucket = nullptr;
+    bool split_valid = false;
+    bool graph_cold_sort = false;  // the graph's first step sorts without splitters
 
     int32_t debug = 0;
     int walk_block = 256, walk_stack = 128;
@@ -235,6 +242,10 @@ void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
     s->sim_root = a.take<uint32_t>(s->num_sims);
     s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
     s->accel = a.take<float2>(cap);
+    const bool bucket_sort = !s->ensemble && !external_bodies && cap > kSortLdsMax;
+    // splitters, then the bucket starts of the bucket sort's placing pass
+    s->split = bucket_sort ? a.take<uint32_t>(2 * kMaxBuckets) : nullptr;
+    s->bucket = bucket_sort ? a.take<uint16_t>(cap) : nullptr;
     s->radius = external_bodies ? nullptr : a.take<float>(cap);
     // the quadrupole pass's prefix sums (the allocation-free step is monopole only)
     s->quad_prefix = external_bodies ? nullptr : a.take<double>(quad_prefix_words(cap));
@@ -271,6 +282,7 @@ int allocate(nbody_sim *s, int64_t cap) {
     sort_bind_scratch(s->sort, s->sort_words, cap);
     HIP_TRY(s, hipMemsetAsync(s->sort_words, 0, 4 * (size_t)sort_words, s->stream));
     s->cap = cap;
+    s->split_valid = false;
     HIP_TRY(s, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->overflow, 0, sizeof(uint32_t), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->tree.counts, 0, 4 * (size_t)fill_count_words(cap), s->stream));
@@ -341,6 +353,20 @@ void prime_bbox(nbody_sim *s) {
     s->bbox_fresh = true;
 }
 
+// Which sort a plain step takes: one workgroup for n <= kSortLdsMax, the
+// bucket sort between the last step's splitters when they are there, else
+// (the first step after the bodies were set, ensembles, NBODY_DEBUG_SORT_RADIX)
+// the radix passes.  Every path gives the same permutation.
+enum class SortPath { kRadix, kSmall, kBuckets };
+int64_t bucket_sort_max() { return (int64_t)kMaxBuckets << kBucketShift; }
+SortPath sort_path(const nbody_sim *s) {
+    const int64_t n = s->n;
+    if (s->ensemble || (s->debug & NBODY_DEBUG_SORT_RADIX) || n <= 1) return SortPath::kRadix;
+    if (n <= kSortLdsMax) return SortPath::kSmall;
+    if (s->split && s->split_valid && n <= bucket_sort_max()) return SortPath::kBuckets;
+    return SortPath::kRadix;
+}
+
 // Enqueue one step: simulation.ts:393-421 (tree_only: phases 0-4, which
 // rebuild the tree of the current positions without moving the bodies).
 int enqueue_step(nbody_sim *s, bool tree_only = false) {
@@ -365,33 +391,49 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
                     nullptr, st);
     mark(1);
     // phase 1: Morton encode (lbvh.wgsl:42-69) and the sort's digit counts
+    // (radix sort) or bucket counts (bucket sort)
     const int passes = n > 1 ? sort_passes(s->key_bits, s->radix_bits) : 0;
-    launch_encode(s->pos, n, s->cfg.morton_mode, s->bbox, slots ? s->bbox_slots : nullptr,
-                  s->body_sim, s->ensemble ? nullptr : (uint32_t *)s->keys[0],
-                  s->ensemble ? (uint64_t *)s->keys[0] : nullptr, s->vals[0], s->sort.hist,
-                  passes, s->radix_bits, st);
+    const SortPath path = sort_path(s);
+    uint32_t *keys0 = (uint32_t *)s->keys[0], *keys1 = (uint32_t *)s->keys[1];
+    if (path == SortPath::kBuckets)
+        launch_encode_buckets(s->pos, n, s->cfg.morton_mode, s->bbox,
+                              slots ? s->bbox_slots : nullptr, keys0, s->vals[0], s->bucket,
+                              s->split, sort_buckets(n), s->sort.hist, st);
+    else
+        launch_encode(s->pos, n, s->cfg.morton_mode, s->bbox, slots ? s->bbox_slots : nullptr,
+                      s->body_sim, s->ensemble ? nullptr : keys0,
+                      s->ensemble ? (uint64_t *)s->keys[0] : nullptr, s->vals[0], s->sort.hist,
+                      path == SortPath::kRadix ? passes : 0, s->radix_bits, st);
     mark(2);
-    // phase 2: stable radix sort (simulation.ts:400); its last pass also
-    // writes each body's sorted position
-    const LeafGather lg{s->rank};
+    // phase 2: stable sort (simulation.ts:400); it also writes each body's
+    // sorted position
+    const LeafGather lg{s->rank, s->split ? s->split + kMaxBuckets : nullptr};
     if (s->ensemble)
         s->result_buf = launch_sort_u64((uint64_t *)s->keys[0], s->vals[0], (uint64_t *)s->keys[1],
                                         s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
+    else if (path == SortPath::kSmall)
+        s->result_buf = launch_sort_small(keys0, s->vals[0], keys1, s->vals[1], n, s->key_bits, lg, st);
+    else if (path == SortPath::kBuckets)
+        s->result_buf = launch_sort_buckets(keys0, s->vals[0], keys1, s->vals[1], n, s->key_bits,
+                                            sort_buckets(n), s->split, s->bucket, s->sort, lg, st);
     else
-        s->result_buf = launch_sort_u32((uint32_t *)s->keys[0], s->vals[0], (uint32_t *)s->keys[1],
-                                        s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
+        s->result_buf = launch_sort_u32(keys0, s->vals[0], keys1, s->vals[1], n, s->key_bits,
+                                        s->radix_bits, s->sort, lg, st);
     const uint32_t *idx = s->vals[s->result_buf];
     mark(3);
     // phase 3: Karras build (lbvh.wgsl:85-165); it also places the sorted
     // leaf records and zeroes the sort's scratch for the next step
     const int64_t clear = sort_clear_words(n, s->key_bits, s->radix_bits, s->ensemble);
-    const LeafPlace lpl{s->pos, s->mass, s->rank};
+    // the build of a bucket-sortable size writes the next sort's splitters
+    const bool splits = s->split && n > kSortLdsMax && n <= bucket_sort_max();
+    const LeafPlace lpl{s->pos, s->mass, s->rank, splits ? s->split : nullptr};
     if (s->ensemble)
         launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree, lpl,
                          s->sim_root, s->sort.hist, clear, nullptr, st);
     else
         launch_build_u32((const uint32_t *)s->keys[s->result_buf], n, s->tree, lpl, s->sort.hist,
                          clear, slots ? s->bbox_slots : nullptr, st);
+    s->split_valid = splits;
     mark(4);
     // phase 4: bottom-up fill (lbvh.wgsl:168-233), then the second moments
     // of the quadrupole expansion
@@ -465,11 +507,18 @@ int accumulate_profile(nbody_sim *s) {
 
 int capture_graph(nbody_sim *s) {
     release_graph(s);
+    // a graph whose first step cannot use splitters yet is captured again
+    // once the splitters are there (nbody_step)
+    const bool was_valid = s->split_valid;
+    s->graph_cold_sort = sort_path(s) == SortPath::kRadix && s->split && !s->split_valid &&
+                         s->n > kSortLdsMax && s->n <= bucket_sort_max();
     HIP_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = NBODY_OK;
     for (int k = 0; k < s->graph_steps && rc == NBODY_OK; ++k) rc = enqueue_step(s);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
+    // nothing captured has run: the splitters are what they were
+    s->split_valid = was_valid;
     if (rc != NBODY_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -690,6 +739,7 @@ int nbody_set_config(nbody_sim *s, const nbody_config *cfg) {
         return fail(s, NBODY_ERR_INVALID_ARG, "unknown morton_mode");
     s->cfg = *cfg;
     s->graph_dirty = true;
+    s->split_valid = false;  // the keys may change form
     s->bbox_fresh = false;  // the accumulators are kept only in the bbox30 mode
     return NBODY_OK;
 }
@@ -737,6 +787,7 @@ int nbody_set_bodies(nbody_sim *s, int64_t n, const float *mass, const float *po
     if (n > 0)  // setScenario's radiusMultiplierData.fill(1.0) (simulation.ts:240-241)
         launch_fill_f32(s->radius, 1.0f, n, s->stream);
     s->steps_taken = 0;
+    s->split_valid = false;  // splitters of other bodies
     if (s->ensemble) {
         std::vector<uint8_t> bs((size_t)n);
         for (int i = 0; i < s->num_sims; ++i)
@@ -774,12 +825,16 @@ int nbody_step(nbody_sim *s, int32_t nsteps) {
     s->steps_taken += nsteps;
     if (s->use_graph) {
         prime_bbox(s);  // eagerly, before the captured steps that expect it
-        if (s->graph_dirty || !s->graph_exec) {
+        if (s->graph_dirty || !s->graph_exec || (s->graph_cold_sort && s->split_valid)) {
             int rc = capture_graph(s);
             if (rc != NBODY_OK) return rc;
         }
         const int k = s->graph_steps;
-        for (int i = 0; i < nsteps / k; ++i) HIP_TRY(s, hipGraphLaunch(s->graph_exec, s->stream));
+        for (int i = 0; i < nsteps / k; ++i) {
+            HIP_TRY(s, hipGraphLaunch(s->graph_exec, s->stream));
+            // every replay's builds write splitters for the current bodies
+            s->split_valid = s->split && s->n > kSortLdsMax && s->n <= bucket_sort_max();
+        }
         for (int i = 0; i < nsteps % k; ++i) {  // the rest, launched one by one
             int rc = enqueue_step(s);
             if (rc != NBODY_OK) return rc;
@@ -832,7 +887,8 @@ int nbody_use_graph(nbody_sim *s, int32_t steps_per_graph) {
 
 int nbody_set_debug(nbody_sim *s, int32_t flags) {
     if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
-    if (flags & ~(NBODY_DEBUG_ACCEL | NBODY_DEBUG_STATS | NBODY_DEBUG_FILL_CLIMB))
+    if (flags & ~(NBODY_DEBUG_ACCEL | NBODY_DEBUG_STATS | NBODY_DEBUG_FILL_CLIMB |
+                  NBODY_DEBUG_SORT_RADIX))
         return fail(s, NBODY_ERR_INVALID_ARG, "unknown debug flag");
     s->debug = flags;
     s->graph_dirty = true;

@@ -246,6 +246,64 @@ __global__ __launch_bounds__(kEncodeThreads) void encode_kernel(const float2 *__
     }
 }
 
+// The plain encode for the bucket sort: keys, and each body's bucket (the
+// number of splitters <= its key: a binary search in LDS), counted per
+// bucket (LDS histogram, one global atomic per non-empty bucket).
+template <int kMode>
+__global__ __launch_bounds__(kEncodeThreads) void encode_bucket_kernel(
+    const float2 *__restrict__ pos, int64_t n, float4 *__restrict__ bbox,
+    const uint32_t *__restrict__ slots, uint32_t *__restrict__ keys32, uint32_t *__restrict__ idx,
+    uint16_t *__restrict__ bucket, const uint32_t *__restrict__ split, int nb,
+    uint32_t *__restrict__ hist) {
+    __shared__ uint32_t s_split[kMaxBuckets];
+    __shared__ uint32_t s_hist[kMaxBuckets];
+    constexpr int kItems = kEncodeTile / kEncodeThreads;
+    const int64_t base = (int64_t)blockIdx.x * kEncodeTile;
+    float2 p[kItems];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t i = base + it * kEncodeThreads + threadIdx.x;
+        p[it] = i < n ? pos[i] : make_float2(0.0f, 0.0f);
+    }
+    for (int j = threadIdx.x; j < nb; j += kEncodeThreads) {
+        s_hist[j] = 0u;
+        if (j < nb - 1) s_split[j] = split[j];
+    }
+    float4 box = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (kMode == 1) {
+        if (slots) {
+            box = bbox_from_slots(slots);
+            if (blockIdx.x == 0 && threadIdx.x == 0) bbox[0] = box;
+        } else {
+            box = bbox[0];
+        }
+    }
+    __syncthreads();
+    // the largest power of two <= nb - 1 (the search's first step)
+    const int ns = nb - 1;
+    int top = 1;
+    while (top * 2 <= ns) top *= 2;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t i = base + it * kEncodeThreads + threadIdx.x;
+        if (i >= n) break;
+        const uint32_t key = kMode == 0 ? code_reference(p[it]) : code_bbox30(p[it], box);
+        keys32[i] = key;
+        idx[i] = (uint32_t)i;
+        int b = 0;  // splitters <= key
+        if (ns > 0)
+            for (int step = top; step >= 1; step >>= 1)
+                if (b + step <= ns && s_split[b + step - 1] <= key) b += step;
+        bucket[i] = (uint16_t)b;
+        atomicAdd(&s_hist[b], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nb; j += kEncodeThreads) {
+        const uint32_t c = s_hist[j];
+        if (c) atomicAdd(&hist[j], c);
+    }
+}
+
 __global__ __launch_bounds__(256) void clear_u32_kernel(uint32_t *__restrict__ p, int64_t words) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < words;
          i += (int64_t)gridDim.x * 256)
@@ -294,6 +352,20 @@ void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
     else
         launch_encode_rb<8>(pos, n, morton_mode, bbox, slots, body_sim, keys32, keys64, idx,
                             hist, passes, s);
+}
+
+void launch_encode_buckets(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                           const uint32_t *slots, uint32_t *keys32, uint32_t *idx,
+                           uint16_t *bucket, const uint32_t *split, int nb, uint32_t *hist,
+                           hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned grid = (unsigned)((n + kEncodeTile - 1) / kEncodeTile);
+    if (morton_mode == 0)
+        encode_bucket_kernel<0><<<grid, kEncodeThreads, 0, s>>>(pos, n, bbox, slots, keys32, idx,
+                                                                bucket, split, nb, hist);
+    else
+        encode_bucket_kernel<1><<<grid, kEncodeThreads, 0, s>>>(pos, n, bbox, slots, keys32, idx,
+                                                                bucket, split, nb, hist);
 }
 
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s) {

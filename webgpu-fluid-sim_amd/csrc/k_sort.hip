@@ -80,11 +80,14 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
 //
 // kLeaf (the last pass): each item's body also gets its output position,
 // rank[body] (a scattered write, which no barrier waits for).
-template <typename K, int RB, int kTile, int kThreads, bool kLeaf>
+//
+// kBucket: the digit of item p is its bucket id bucket[p] instead of key bits
+// (the bucket sort's placing pass).
+template <typename K, int RB, int kTile, int kThreads, bool kLeaf, bool kBucket = false>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_count,
-    uint32_t *lookback, uint32_t *counter, LeafGather lg) {
+    uint32_t *lookback, uint32_t *counter, LeafGather lg, const uint16_t *__restrict__ bucket) {
     constexpr int D = 1 << RB;
     constexpr int kWaves = kThreads / kWave;
     constexpr int kItems = kTile / kThreads;
@@ -100,6 +103,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     __shared__ uint32_t s_shw[kWaves];
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
+    __shared__ uint16_t s_bk[kBucket ? kTile : 1];  // digits of the staged items
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
@@ -117,12 +121,14 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     uint32_t v[kItems];
     uint32_t dg[kItems];
     uint32_t rank[kItems];
+    uint32_t bk[kItems];
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const int64_t p = base + (int64_t)wid * (64 * kItems) + it * 64 + lane;
         const bool ok = p < n;
         k[it] = ok ? kin[p] : (K)0;
         v[it] = ok ? vin[p] : 0u;
+        if constexpr (kBucket) bk[it] = ok ? bucket[p] : 0u;
     }
     // where each owned digit starts in the output (exclusive digit scan)
     uint32_t dbase[DPT];
@@ -139,11 +145,19 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
             dbase[j] = run;
             run += c[j];
         }
+        // the bucket sort's per-bucket kernel reads the bucket starts
+        if constexpr (kBucket)
+            if (tile == 0 && owner)
+#pragma unroll
+                for (int j = 0; j < DPT; ++j) lg.bucket_start[tid * DPT + j] = dbase[j];
     }
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const int64_t p = base + (int64_t)wid * (64 * kItems) + it * 64 + lane;
-        dg[it] = p < n ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
+        if constexpr (kBucket)
+            dg[it] = p < n ? bk[it] : (uint32_t)D;
+        else
+            dg[it] = p < n ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
     }
 
     // Stable rank inside the wave: peers = lanes holding the same digit.
@@ -273,6 +287,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
             const uint32_t lp = s_lstart[dd] + s_wcnt[wid * D + dd] + rank[it];
             s_keys[lp] = k[it];
             s_vals[lp] = v[it];
+            if constexpr (kBucket) s_bk[lp] = (uint16_t)dd;
             if constexpr (kLeaf) lg.rank[v[it]] = s_gstart[dd] + (lp - s_lstart[dd]);
         }
     }
@@ -284,7 +299,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         const int p = it * kThreads + tid;
         if (p < tile_n) {
             const K kk = s_keys[p];
-            const uint32_t dd = (uint32_t)((kk >> shift) & kMask);
+            const uint32_t dd = kBucket ? (uint32_t)s_bk[p] : (uint32_t)((kk >> shift) & kMask);
             const uint32_t dst = s_gstart[dd] + (uint32_t)(p - (int)s_lstart[dd]);
             kout[dst] = kk;
             vout[dst] = s_vals[p];
@@ -304,11 +319,11 @@ int sort_rb_tile(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64
         if (p == passes - 1 && lg.rank)
             onesweep_kernel<K, RB, kTile, T, true><<<(unsigned)tiles, T, 0, s>>>(
                 ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
-                sc.counters + p, lg);
+                sc.counters + p, lg, nullptr);
         else
             onesweep_kernel<K, RB, kTile, T, false><<<(unsigned)tiles, T, 0, s>>>(
                 ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
-                sc.counters + p, lg);
+                sc.counters + p, lg, nullptr);
         K *tk = ki;
         ki = ko;
         ko = tk;
@@ -341,6 +356,236 @@ int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t 
     if (radix_bits == 10)
         return sort_rb<K, 10>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
     return sort_rb<K, 8>(keys_a, vals_a, keys_b, vals_b, n, passes, sc, lg, s);
+}
+
+// ---------------------------------------------------------------- bucket sort
+// Bodies move little between steps, so the previous step's sorted keys split
+// the next step's keys into nearly equal buckets: splitter j (j = 1 .. B-1)
+// is the key at sorted position j * S.  A body's bucket is the number of
+// splitters <= its key (the encode counts them, so equal keys share a
+// bucket, as does every key of a bucket's range).  One stable one-sweep pass
+// over the bucket ids places every bucket, then bucket_sort_kernel sorts
+// each bucket, one workgroup per bucket, with stable 8-bit LSD passes over
+// the key bits that vary inside the bucket's key range.  A sort of at most
+// kSortLdsMax keys is one bucket, sorted by one workgroup straight from the
+// encode.  The result is the radix sort's: the stable order is unique.
+constexpr int kBucketThreads = 1024;
+constexpr int kBucketWaves = kBucketThreads / kWave;
+constexpr int kBucketRB = 8;
+constexpr int kBucketD = 1 << kBucketRB;
+
+// Stable rank of each of the wavefront's first `rows` items among its items
+// of the same digit (dg == kBucketD: an empty slot), counting into the
+// wavefront's digit counters wc (kBucketD words, zero before the first
+// item): ballot matching as in the one-sweep passes.  `rows` is uniform.
+template <int kItems>
+__device__ __forceinline__ void wave_rank8(const uint32_t (&dg)[kItems], uint32_t (&rank)[kItems],
+                                           uint32_t *wc, int rows) {
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        if (it >= rows) break;
+        const uint32_t d = dg[it];
+        const bool ok = d < (uint32_t)kBucketD;
+        const uint64_t valid = __ballot(ok);
+        uint32_t plo = (uint32_t)valid, phi = (uint32_t)(valid >> 32);
+#pragma unroll
+        for (int b = 0; b < kBucketRB; ++b) {
+            const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+            const uint64_t bb = __ballot(m != 0u);
+            plo &= ~((uint32_t)bb ^ m);
+            phi &= ~((uint32_t)(bb >> 32) ^ m);
+        }
+        const uint64_t peers = ((uint64_t)phi << 32) | plo;
+        const uint32_t below = lanes_below(peers);
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        uint32_t before = 0;
+        if (ok) before = wc[d];
+        __builtin_amdgcn_wave_barrier();
+        if (ok && (below == cnt - 1u)) wc[d] = before + cnt;
+        __builtin_amdgcn_wave_barrier();
+        rank[it] = before + below;
+    }
+}
+
+// After wave_rank8 in every wavefront (and a barrier): s_wcnt[w][d] becomes
+// the count of digit d in the wavefronts before w, s_start[d] the first slot
+// of digit d, s_total[d] its count.  Ends with a barrier.
+__device__ __forceinline__ void digit_starts(uint32_t *s_wcnt, uint32_t *s_start,
+                                             uint32_t *s_total, uint32_t *s_shw) {
+    const int tid = threadIdx.x;
+    uint32_t t = 0;
+    if (tid < kBucketD) {
+        uint32_t c[kBucketWaves];
+#pragma unroll
+        for (int w = 0; w < kBucketWaves; ++w) c[w] = s_wcnt[w * kBucketD + tid];
+#pragma unroll
+        for (int w = 0; w < kBucketWaves; ++w) {
+            s_wcnt[w * kBucketD + tid] = t;
+            t += c[w];
+        }
+    }
+    const uint32_t start = block_exclusive_scan<kBucketThreads>(t, s_shw);
+    if (tid < kBucketD) {
+        s_start[tid] = start;
+        s_total[tid] = t;
+    }
+    __syncthreads();
+}
+
+// One workgroup per bucket (blockIdx.x), kCap keys at most in LDS (larger
+// buckets go through memory in chunks of kCap).  Bucket b starts at
+// position start[b] and holds count[b] keys in [split[b-1], split[b] - 1]
+// (split[-1] = 0, split[B-1] = the largest key); start == null: one bucket
+// of n keys.  Input (kin, vin) and output (kout, vout) cover the same
+// positions; a bucket too large for LDS also overwrites its input.
+template <int kCap>
+__global__ __launch_bounds__(kBucketThreads) void bucket_sort_kernel(
+    uint32_t *kin, uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t n, uint32_t key_max,
+    int nb, const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ split, uint32_t *__restrict__ rank_out) {
+    constexpr int kItems = kCap / kBucketThreads;
+    static_assert(kItems * kBucketThreads == kCap, "bucket capacity");
+    __shared__ uint32_t s_k[kCap];
+    __shared__ uint32_t s_v[kCap];
+    __shared__ uint32_t s_wcnt[kBucketWaves * kBucketD];
+    __shared__ uint32_t s_start[kBucketD], s_total[kBucketD], s_run[kBucketD];
+    __shared__ uint32_t s_shw[kBucketWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int b = (int)blockIdx.x;
+
+    const uint32_t off = start ? start[b] : 0u;
+    const uint32_t cnt = start ? count[b] : (uint32_t)n;
+    // bits that vary inside the bucket's key range
+    const uint32_t klo = b == 0 ? 0u : split[b - 1];
+    const uint32_t khi = b == nb - 1 ? key_max : split[b] - 1u;
+    if (cnt == 0u) return;
+    const uint32_t diff = klo ^ khi;
+    const int passes = diff ? (32 - __clz((int)diff) + kBucketRB - 1) / kBucketRB : 0;
+
+    uint32_t k[kItems], v[kItems], dg[kItems], rk[kItems];
+    if (cnt <= (uint32_t)kCap) {
+        // item it < rows of lane l in wave w: bucket position
+        // w*64*rows + it*64 + l (every wavefront gets the same rows)
+        const int rows = (int)((cnt + kBucketThreads - 1) / kBucketThreads);
+        auto slot = [&](int it) { return (uint32_t)(wid * 64 * rows + it * 64 + lane); };
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            if (it >= rows) break;
+            const uint32_t p = slot(it);
+            k[it] = p < cnt ? kin[off + p] : 0u;
+            v[it] = p < cnt ? vin[off + p] : 0u;
+        }
+        for (int q = 0; q < passes; ++q) {
+            const int shift = kBucketRB * q;
+            for (int i = tid; i < kBucketWaves * kBucketD; i += kBucketThreads) s_wcnt[i] = 0u;
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < kItems; ++it)
+                dg[it] = it < rows && slot(it) < cnt ? (k[it] >> shift) & (kBucketD - 1u)
+                                                     : (uint32_t)kBucketD;
+            wave_rank8<kItems>(dg, rk, s_wcnt + wid * kBucketD, rows);
+            __syncthreads();
+            digit_starts(s_wcnt, s_start, s_total, s_shw);
+#pragma unroll
+            for (int it = 0; it < kItems; ++it) {
+                if (it >= rows) break;
+                if (dg[it] < (uint32_t)kBucketD) {
+                    const uint32_t lp = s_start[dg[it]] + s_wcnt[wid * kBucketD + dg[it]] + rk[it];
+                    s_k[lp] = k[it];
+                    s_v[lp] = v[it];
+                    // the last pass knows each body's place: its scattered
+                    // write overlaps the rest of the kernel
+                    if (q == passes - 1) rank_out[v[it]] = off + lp;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < kItems; ++it) {
+                if (it >= rows) break;
+                const uint32_t p = slot(it);
+                if (p < cnt) {
+                    k[it] = s_k[p];
+                    v[it] = s_v[p];
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            if (it >= rows) break;
+            const uint32_t p = slot(it);
+            if (p < cnt) {
+                kout[off + p] = k[it];
+                vout[off + p] = v[it];
+                if (passes == 0) rank_out[v[it]] = off + p;
+            }
+        }
+        return;
+    }
+
+    // A bucket larger than kCap: each pass counts its digits over the whole
+    // bucket, then moves it chunk by chunk in input order (stable) between
+    // the input and output positions.
+    auto slot = [&](int it) { return (uint32_t)(wid * (64 * kItems) + it * 64 + lane); };
+    uint32_t *sk = kin + off, *sv = vin + off, *dk = kout + off, *dv = vout + off;
+    for (int q = 0; q < passes; ++q) {
+        const int shift = kBucketRB * q;
+        if (tid < kBucketD) s_run[tid] = 0u;
+        __syncthreads();
+        for (uint32_t c0 = 0; c0 < cnt; c0 += (uint32_t)kCap)
+#pragma unroll
+            for (int it = 0; it < kItems; ++it) {
+                const uint32_t p = c0 + slot(it);
+                if (p < cnt) atomicAdd(&s_run[(sk[p] >> shift) & (kBucketD - 1u)], 1u);
+            }
+        __syncthreads();
+        {
+            const uint32_t t = tid < kBucketD ? s_run[tid] : 0u;
+            const uint32_t first = block_exclusive_scan<kBucketThreads>(t, s_shw);
+            if (tid < kBucketD) s_run[tid] = first;
+            __syncthreads();
+        }
+        for (uint32_t c0 = 0; c0 < cnt; c0 += (uint32_t)kCap) {
+#pragma unroll
+            for (int it = 0; it < kItems; ++it) {
+                const uint32_t p = c0 + slot(it);
+                k[it] = p < cnt ? sk[p] : 0u;
+                v[it] = p < cnt ? sv[p] : 0u;
+                dg[it] = p < cnt ? (k[it] >> shift) & (kBucketD - 1u) : (uint32_t)kBucketD;
+            }
+            for (int i = tid; i < kBucketWaves * kBucketD; i += kBucketThreads) s_wcnt[i] = 0u;
+            __syncthreads();
+            wave_rank8<kItems>(dg, rk, s_wcnt + wid * kBucketD, kItems);
+            __syncthreads();
+            digit_starts(s_wcnt, s_start, s_total, s_shw);
+#pragma unroll
+            for (int it = 0; it < kItems; ++it) {
+                if (dg[it] < (uint32_t)kBucketD) {
+                    const uint32_t dst = s_run[dg[it]] + s_wcnt[wid * kBucketD + dg[it]] + rk[it];
+                    dk[dst] = k[it];
+                    dv[dst] = v[it];
+                }
+            }
+            __syncthreads();  // the chunk's stores are done before the next chunk's loads
+            if (tid < kBucketD) s_run[tid] += s_total[tid];
+            __syncthreads();
+        }
+        uint32_t *tk = sk, *tv = sv;
+        sk = dk;
+        sv = dv;
+        dk = tk;
+        dv = tv;
+    }
+    // the sorted bucket is at (sk, sv): move it to the output if it is not
+    // there already, and record every body's position
+    const bool moved = sk != kout + off;
+    for (uint32_t p = (uint32_t)tid; p < cnt; p += kBucketThreads) {
+        const uint32_t kk = sk[p], vv = sv[p];
+        if (moved) {
+            kout[off + p] = kk;
+            vout[off + p] = vv;
+        }
+        rank_out[vv] = off + p;
+    }
 }
 
 }  // namespace
@@ -380,6 +625,54 @@ int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
                     const LeafGather &lg, hipStream_t s) {
     return sort_impl<uint32_t>(keys_a, vals_a, keys_b, vals_b, n, key_bits, radix_bits, sc, lg, s);
+}
+
+uint32_t sort_key_max(int key_bits) {
+    return key_bits >= 32 ? 0xFFFFFFFFu : (1u << key_bits) - 1u;
+}
+
+int launch_sort_small(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
+                      int64_t n, int key_bits, const LeafGather &lg, hipStream_t s) {
+    bucket_sort_kernel<kBucketCap><<<1, kBucketThreads, 0, s>>>(
+        keys_a, vals_a, keys_b, vals_b, n, sort_key_max(key_bits), 1, nullptr, nullptr, nullptr,
+        lg.rank);
+    return 1;
+}
+
+template <int RB, int kTile>
+void launch_place_buckets(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
+                          int64_t n, const uint16_t *bucket, const SortScratch &sc,
+                          const LeafGather &lg, hipStream_t s) {
+    constexpr int T = sort_threads<RB, kTile>();
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    onesweep_kernel<uint32_t, RB, kTile, T, false, true><<<(unsigned)tiles, T, 0, s>>>(
+        keys_a, vals_a, keys_b, vals_b, n, 0, sc.hist, sc.lookback, sc.counters, lg, bucket);
+}
+
+int launch_sort_buckets(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
+                        int64_t n, int key_bits, int nb, const uint32_t *split,
+                        const uint16_t *bucket, const SortScratch &sc, const LeafGather &lg,
+                        hipStream_t s) {
+    const int tile = sort_tile_for(n, false);
+    if (nb <= 256) {
+        if (tile == 2048)
+            launch_place_buckets<8, 2048>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
+        else if (tile == 4096)
+            launch_place_buckets<8, 4096>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
+        else
+            launch_place_buckets<8, 8192>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
+    } else {
+        if (tile == 2048)
+            launch_place_buckets<10, 2048>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
+        else if (tile == 4096)
+            launch_place_buckets<10, 4096>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
+        else
+            launch_place_buckets<10, 8192>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
+    }
+    bucket_sort_kernel<kBucketCap><<<(unsigned)nb, kBucketThreads, 0, s>>>(
+        keys_b, vals_b, keys_a, vals_a, n, sort_key_max(key_bits), nb, lg.bucket_start, sc.hist,
+        split, lg.rank);
+    return 0;
 }
 
 int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,

@@ -105,6 +105,13 @@ void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
                    const uint32_t *slots, const uint8_t *body_sim, uint32_t *keys32,
                    uint64_t *keys64, uint32_t *idx, uint32_t *hist, int passes, int radix_bits,
                    hipStream_t s);
+// The same keys (plain handles) with each body's bucket among the nb - 1
+// splitters split[0..nb-2] (the number of splitters <= its key) in bucket[],
+// counted into hist[0..nb) (which must be zero).
+void launch_encode_buckets(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                           const uint32_t *slots, uint32_t *keys32, uint32_t *idx,
+                           uint16_t *bucket, const uint32_t *split, int nb, uint32_t *hist,
+                           hipStream_t s);
 
 // ---------------------------------------------------------------- sort
 // Stable LSD sort of (keys, vals) on the low `key_bits` bits, radix_bits
@@ -115,10 +122,31 @@ void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
 // when `rank` is set (the build then places the leaf records).
 struct LeafGather {
     uint32_t *rank;
+    uint32_t *bucket_start = nullptr;  // bucket sort: each bucket's first position
 };
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
                     const LeafGather &lg, hipStream_t s);
+// The bucket sort of plain handles (bit-identical to the radix sort): one
+// workgroup sorts n <= kSortLdsMax keys (launch_sort_small, result in the b
+// buffers); larger sorts up to kMaxBuckets << kBucketShift keys split them
+// between the previous step's splitters (launch_sort_buckets, result in the
+// a buffers; the encode's bucket counts in sc.hist, the bucket starts in
+// lg.bucket_start).
+constexpr int64_t kSortLdsMax = 4096;
+constexpr int kBucketShift = 12;     // splitter spacing: 4096 sorted positions
+constexpr int kMaxBuckets = 1024;
+constexpr int kBucketCap = 6144;     // largest bucket sorted in LDS (larger: through memory)
+__host__ __device__ inline int sort_buckets(int64_t n) {
+    return (int)((n + (int64_t(1) << kBucketShift) - 1) >> kBucketShift);
+}
+uint32_t sort_key_max(int key_bits);
+int launch_sort_small(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
+                      int64_t n, int key_bits, const LeafGather &lg, hipStream_t s);
+int launch_sort_buckets(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
+                        int64_t n, int key_bits, int nb, const uint32_t *split,
+                        const uint16_t *bucket, const SortScratch &sc, const LeafGather &lg,
+                        hipStream_t s);
 int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
                     const LeafGather &lg, hipStream_t s);
@@ -176,6 +204,7 @@ struct LeafPlace {
     const float2 *pos;
     const float *mass;
     const uint32_t *rank;
+    uint32_t *split = nullptr;  // u32 builds: the next sort's splitters (keys at j << kBucketShift)
 };
 void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, const LeafPlace &lp,
                       uint32_t *clear, int64_t clear_words, uint32_t *slots, hipStream_t s);

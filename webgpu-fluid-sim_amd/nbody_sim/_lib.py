@@ -32,6 +32,7 @@ EXPANSION_QUADRUPOLE = 1
 DEBUG_ACCEL = 1
 DEBUG_STATS = 2
 DEBUG_FILL_CLIMB = 4
+DEBUG_SORT_RADIX = 8
 
 BUF_KEYS = 1
 BUF_BODY_INDICES = 2

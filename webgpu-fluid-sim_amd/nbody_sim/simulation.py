@@ -147,11 +147,14 @@ class _Handle:
     def _after_load(self):
         pass
 
-    def setDebug(self, accel: bool = False, stats: bool = False, fill_climb: bool = False):
+    def setDebug(self, accel: bool = False, stats: bool = False, fill_climb: bool = False,
+                 sort_radix: bool = False):
         """Record accelerations / walk statistics; fill_climb forces the tree
-        fill's global-counter climb for its upper levels (a test hook)."""
+        fill's global-counter climb for its upper levels, sort_radix the
+        radix passes of the sort for every step (test hooks)."""
         flags = ((_lib.DEBUG_ACCEL if accel else 0) | (_lib.DEBUG_STATS if stats else 0)
-                 | (_lib.DEBUG_FILL_CLIMB if fill_climb else 0))
+                 | (_lib.DEBUG_FILL_CLIMB if fill_climb else 0)
+                 | (_lib.DEBUG_SORT_RADIX if sort_radix else 0))
         self._check(_lib.lib().nbody_set_debug(self.handle, flags))
 
     def getTuning(self) -> dict:
