@@ -373,6 +373,14 @@ constexpr int kBucketThreads = 1024;
 constexpr int kBucketWaves = kBucketThreads / kWave;
 constexpr int kBucketRB = 8;
 constexpr int kBucketD = 1 << kBucketRB;
+constexpr int kBinBits = 11;  // bins of the in-LDS bucket sort
+constexpr int kBins = 1 << kBinBits;
+constexpr int kBinMax = 32;   // largest bin ranked by counting
+// Counting beats the LSD passes when those are four or more (wide key
+// ranges: the few buckets of small sorts); at three (the 4096-key buckets
+// of a million bodies) the passes are a little faster (22.6 against
+// 24.5 us at 800k).
+constexpr int kBinMinPasses = 4;
 
 // Stable rank of each of the wavefront's first `rows` items among its items
 // of the same digit (dg == kBucketD: an empty slot), counting into the
@@ -475,6 +483,76 @@ __global__ __launch_bounds__(kBucketThreads) void bucket_sort_kernel(
             k[it] = p < cnt ? kin[off + p] : 0u;
             v[it] = p < cnt ? vin[off + p] : 0u;
         }
+        // Bins: the top (up to kBinBits) of the bits that vary in the
+        // bucket, so a bin is a key interval and bins are in key order.
+        // Each key goes to its bin (any order), then takes its place in
+        // the bin by counting the bin's keys ordered before it (key, then
+        // body index: the stable order).  Bins of more than kBinMax keys
+        // (clustered or repeated keys) take the LSD passes instead.
+        const int vbits = diff ? 32 - __clz((int)diff) : 0;
+        const int bbits = vbits < kBinBits ? vbits : kBinBits;
+        const int bsh = vbits - bbits;
+        const uint32_t bmask = (1u << bbits) - 1u;
+        uint32_t *s_bcnt = s_wcnt, *s_bstart = s_wcnt + kBins;
+        for (int i = tid; i < kBins; i += kBucketThreads) s_bcnt[i] = 0u;
+        if (tid == 0) s_run[0] = 0u;  // the largest bin
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            if (it >= rows) break;
+            if (slot(it) < cnt) atomicAdd(&s_bcnt[(k[it] >> bsh) & bmask], 1u);
+        }
+        __syncthreads();
+        uint32_t big = 0;  // the largest bin's count (when above kBinMax)
+        {
+            // kBins / kBucketThreads bins per thread: exclusive scan, largest bin
+            constexpr int kPer = kBins / kBucketThreads;
+            uint32_t c[kPer], sum = 0;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                c[j] = s_bcnt[tid * kPer + j];
+                sum += c[j];
+                big = c[j] > big ? c[j] : big;
+            }
+            uint32_t run = block_exclusive_scan<kBucketThreads>(sum, s_shw);
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                s_bstart[tid * kPer + j] = run;
+                s_bcnt[tid * kPer + j] = 0u;  // now the bins' fill cursors
+                run += c[j];
+            }
+            if (big > (uint32_t)kBinMax) atomicMax(&s_run[0], big);
+            __syncthreads();
+            big = s_run[0];
+        }
+        if (big <= (uint32_t)kBinMax && passes >= kBinMinPasses) {
+#pragma unroll
+            for (int it = 0; it < kItems; ++it) {
+                if (it >= rows) break;
+                if (slot(it) < cnt) {
+                    const uint32_t d = (k[it] >> bsh) & bmask;
+                    const uint32_t at = s_bstart[d] + atomicAdd(&s_bcnt[d], 1u);
+                    s_k[at] = k[it];
+                    s_v[at] = v[it];
+                }
+            }
+            __syncthreads();
+            for (uint32_t p = (uint32_t)tid; p < cnt; p += kBucketThreads) {
+                const uint32_t kp = s_k[p], vp = s_v[p];
+                const uint32_t d = (kp >> bsh) & bmask;
+                const uint32_t b0 = s_bstart[d], b1 = b0 + s_bcnt[d];
+                uint32_t r = 0;
+                for (uint32_t q = b0; q < b1; ++q) {
+                    const uint32_t kq = s_k[q];
+                    r += (kq < kp || (kq == kp && s_v[q] < vp)) ? 1u : 0u;
+                }
+                kout[off + b0 + r] = kp;
+                vout[off + b0 + r] = vp;
+                rank_out[vp] = off + b0 + r;
+            }
+            return;
+        }
+        __syncthreads();  // the bins' counters are the passes' counters
         for (int q = 0; q < passes; ++q) {
             const int shift = kBucketRB * q;
             for (int i = tid; i < kBucketWaves * kBucketD; i += kBucketThreads) s_wcnt[i] = 0u;
