@@ -73,6 +73,22 @@ def test_sort_paths_heavy_duplicates(morton):
     assert np.max(np.unique(keys, return_counts=True)[1]) >= 25000
 
 
+def test_sort_paths_small_clusters():
+    """Clusters of 100 coincident bodies: bins of more than 32 equal keys,
+    which the in-LDS bucket sort ranks by its LSD passes, not by counting."""
+    mass, pos, vel = nbody_sim.generate("uniform-disk", 60000, seed=5)
+    pos = pos.copy()
+    vel = vel.copy()
+    for c in range(50):
+        pos[c * 1000:c * 1000 + 100] = pos[c * 1000]
+        vel[c * 1000:c * 1000 + 100] = 0.0
+    cfg, _ = params(theta=0.5, dt=1e-4)
+    cfg = cfg.with_(mathMode="fast")
+    a = run(cfg, (mass, pos, vel), [1, 1, 2], radix=False)
+    same(a, run(cfg, (mass, pos, vel), [1, 1, 2], radix=True))
+    assert np.max(np.unique(a[-1][0], return_counts=True)[1]) >= 100
+
+
 def test_sort_paths_all_bodies_coincide():
     mass = np.full(20000, 1e-4, np.float32)
     pos = np.zeros((20000, 2), np.float32)
