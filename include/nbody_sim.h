@@ -231,8 +231,8 @@ int nbody_phase_times(nbody_sim *sim, float *ms_out, int64_t *steps, int32_t res
  *                device queue (ensembles: largest simulations first)
  *   xcd_order    order of the walk's workgroups over the 8 XCDs (each has its
  *                own L2): 0 hardware order (round-robin); 1 each XCD walks one
- *                contiguous run of Morton-ordered bodies (default); k > 1
- *                runs of k workgroups dealt to the XCDs in turn
+ *                contiguous run of Morton-ordered bodies; k > 1 runs of k
+ *                workgroups dealt to the XCDs in turn (default 32)
  *   lane_bodies  bodies per lane of the VGPR-stack walk and the adjoint walk:
  *                1 (default), 2 or 4; a wavefront walks 64 * lane_bodies
  *                Morton-consecutive bodies together (the LDS and pair walks

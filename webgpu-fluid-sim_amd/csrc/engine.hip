@@ -120,7 +120,7 @@ struct nbody_sim {
     int walk_variant = 2;  // 0: LDS-stack walk, 1: register-stack walk, 2: pair walk (fast mode)
     int lane_bodies = 1;   // bodies per lane in the register-stack walks (1, 2, 4)
     int wave_bodies = 0;   // bodies per wavefront in the pair walks (64, 32, 16, 8; 0 = auto)
-    int xcd_order = 1;  // walk workgroup order over the XCDs (nbody_tuning.xcd_order)
+    int xcd_order = 32;  // walk workgroup order over the XCDs (nbody_tuning.xcd_order)
 
     bool use_graph = false;
     int graph_steps = 1;  // steps captured per graph

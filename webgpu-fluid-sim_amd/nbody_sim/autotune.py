@@ -18,7 +18,7 @@ import itertools
 DEFAULT_VARIANTS = (2, 4, 1, 0)
 DEFAULT_WAVE_BODIES = (64, 32, 16, 8)
 DEFAULT_LANE_BODIES = (1, 2, 4)
-DEFAULT_XCD = (1, 0, 16, 64)
+DEFAULT_XCD = (32, 1, 0, 16)
 DEFAULT_BLOCKS = (64, 128, 256, 512)
 DEFAULT_STACKS = (64, 128)
 
