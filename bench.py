@@ -146,6 +146,7 @@ def plain_config(name, steps, math="fast", phases=False):
     sim.setBodies(mass, pos, vel)
     sim.useGraph(True, GRAPH_STEPS)
     sim.step(GRAPH_STEPS)
+    sim.prepareGraph()
     el = timed_steps(sim, steps)
     out = {"n": w.n, "steps": steps, "math_mode": math, "value": w.n * steps / el,
            "ms_per_step": 1e3 * el / steps}
@@ -358,6 +359,7 @@ def main():
     sim.setBodies(mass, pos, vel)
     sim.useGraph(True, GRAPH_STEPS)
     sim.step(args.warmup)
+    sim.prepareGraph()  # capture (untimed) the graph of the warm step
     sim.synchronize()
     log(f"warmup {args.warmup} steps done")
 

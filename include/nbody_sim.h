@@ -173,6 +173,14 @@ int nbody_set_stream(nbody_sim *sim, void *hip_stream);
  * steps per command buffer, simulation.ts:393).  Results do not depend on it. */
 int nbody_use_graph(nbody_sim *sim, int32_t steps_per_graph);
 
+/* Capture the step graph now (graphs on, bodies set), so that the next
+ * nbody_step replays it without capturing first; a no-op when the captured
+ * graph is still valid.  nbody_step captures by itself when it needs to.
+ * No counterpart in the reference, which encodes a new command buffer per
+ * call (simulation.ts:386-425); for callers that keep capture out of a timed
+ * or latency-critical loop. */
+int nbody_prepare_graph(nbody_sim *sim);
+
 /* Debug flags for later steps: NBODY_DEBUG_ACCEL records each body's
  * acceleration; NBODY_DEBUG_STATS counts accepted and opened nodes;
  * NBODY_DEBUG_FILL_CLIMB makes the tree fill complete its upper levels by

@@ -119,3 +119,25 @@ def test_multi_step_graphs_equal_eager(per_graph, steps):
         sim.close()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2] == steps
+
+
+@pytest.mark.parametrize("n", [16384, 200000])
+def test_prepared_graph_equals_eager(n):
+    """Warm-up steps launched one by one, the graph captured by prepareGraph
+    (after the sort's splitters exist), then replays: the eager steps' bits."""
+    bodies = nbody_sim.generate("exp-disk", n, seed=2)
+    cfg, _ = params(theta=0.7)
+    out = []
+    for g in (0, 10):
+        sim = nbody_sim.Simulation(cfg.with_(mathMode="fast"))
+        sim.setBodies(*bodies)
+        sim.useGraph(g > 0, steps=max(g, 1))
+        sim.step(3)
+        sim.prepareGraph()
+        sim.prepareGraph()  # still valid: a no-op
+        sim.step(20)
+        sim.synchronize()
+        out.append(sim.getBodies()[1:] + (sim.getStepCount(),))
+        sim.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2] == 23

@@ -530,7 +530,23 @@ int capture_graph(nbody_sim *s) {
         release_graph(s);
         return hip_fail(s, e, "hipGraphInstantiate");
     }
+    // move the graph's launch resources to the device now, not on the first replay
+    e = hipGraphUpload(s->graph_exec, s->stream);
+    if (e != hipSuccess) {
+        release_graph(s);
+        return hip_fail(s, e, "hipGraphUpload");
+    }
     s->graph_dirty = false;
+    return NBODY_OK;
+}
+
+// Capture the step graph unless the current one is still valid: the bbox
+// priming runs eagerly first (captured steps expect it), and a graph whose
+// first step had to sort without splitters is captured again once they exist.
+int prepare_graph(nbody_sim *s) {
+    prime_bbox(s);
+    if (s->graph_dirty || !s->graph_exec || (s->graph_cold_sort && s->split_valid))
+        return capture_graph(s);
     return NBODY_OK;
 }
 
@@ -824,12 +840,11 @@ int nbody_step(nbody_sim *s, int32_t nsteps) {
     if (s->n <= 0 || nsteps == 0) return NBODY_OK;
     s->steps_taken += nsteps;
     if (s->use_graph) {
-        prime_bbox(s);  // eagerly, before the captured steps that expect it
-        if (s->graph_dirty || !s->graph_exec || (s->graph_cold_sort && s->split_valid)) {
-            int rc = capture_graph(s);
+        const int k = s->graph_steps;
+        if (nsteps >= k) {
+            int rc = prepare_graph(s);
             if (rc != NBODY_OK) return rc;
         }
-        const int k = s->graph_steps;
         for (int i = 0; i < nsteps / k; ++i) {
             HIP_TRY(s, hipGraphLaunch(s->graph_exec, s->stream));
             // every replay's builds write splitters for the current bodies
@@ -850,6 +865,13 @@ int nbody_step(nbody_sim *s, int32_t nsteps) {
         }
     }
     return NBODY_OK;
+}
+
+int nbody_prepare_graph(nbody_sim *s) {
+    if (!s) return fail(nullptr, NBODY_ERR_INVALID_ARG, "sim is NULL");
+    HIP_TRY(s, hipSetDevice(s->device));
+    if (!s->use_graph || s->n <= 0) return NBODY_OK;
+    return prepare_graph(s);
 }
 
 int nbody_synchronize(nbody_sim *s) {

@@ -62,7 +62,7 @@ EXPORTS = (
     "nbody_default_config", "nbody_create", "nbody_ensemble_create", "nbody_destroy",
     "nbody_set_config", "nbody_get_config", "nbody_set_bodies", "nbody_get_bodies",
     "nbody_get_num_bodies", "nbody_step", "nbody_synchronize", "nbody_set_stream",
-    "nbody_use_graph", "nbody_set_debug", "nbody_debug_get", "nbody_walk_stats",
+    "nbody_use_graph", "nbody_prepare_graph", "nbody_set_debug", "nbody_debug_get", "nbody_walk_stats",
     "nbody_set_profiling", "nbody_phase_times", "nbody_set_tuning", "nbody_get_tuning",
     "nbody_last_error",
     "nbody_generate", "nbody_philox4x32", "nbody_version", "nbody_adjoint_forward",
@@ -159,6 +159,7 @@ def lib() -> ctypes.CDLL:
         "nbody_synchronize": (ctypes.c_int, [vp]),
         "nbody_set_stream": (ctypes.c_int, [vp, vp]),
         "nbody_use_graph": (ctypes.c_int, [vp, i32]),
+        "nbody_prepare_graph": (ctypes.c_int, [vp]),
         "nbody_set_debug": (ctypes.c_int, [vp, i32]),
         "nbody_debug_get": (ctypes.c_int, [vp, i32, vp, i64]),
         "nbody_walk_stats": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64), i32]),

@@ -118,6 +118,10 @@ class _Handle:
         """Replay `steps` steps per captured HIP graph (enable False: no graph)."""
         self._check(_lib.lib().nbody_use_graph(self.handle, int(steps) if enable else 0))
 
+    def prepareGraph(self):
+        """Capture the step graph now, so the next step() replays it at once."""
+        self._check(_lib.lib().nbody_prepare_graph(self.handle))
+
     def setRadiusMultiplier(self, r):
         """updateRadiusMultiplierBuffer (simulation.ts:382-384)."""
         r = np.ascontiguousarray(r, np.float32)

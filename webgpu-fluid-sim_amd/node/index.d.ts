@@ -40,6 +40,7 @@ export declare class Simulation {
     step(nsteps?: number): void;
     synchronize(): void;
     useGraph(enable?: boolean, stepsPerGraph?: number): void;
+    prepareGraph(): void;
     setRadiusMultiplier(r: Float32Array): void;
     saveSnapshot(dir: string): void;
     loadSnapshot(dir: string): void;

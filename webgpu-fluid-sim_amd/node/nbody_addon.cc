@@ -234,6 +234,16 @@ napi_value UseGraph(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
+// prepareGraph(handle): capture the step graph now (nbody_prepare_graph).
+napi_value PrepareGraph(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    if (!get_args(env, info, 1, argv)) return nullptr;
+    nbody_sim *h = unwrap(env, argv[0]);
+    int rc = nbody_prepare_graph(h);
+    if (rc) return throw_nbody(env, rc, h);
+    return nullptr;
+}
+
 // updateRadiusMultiplierBuffer (simulation.ts:382-384).
 napi_value SetRadius(napi_env env, napi_callback_info info) {
     napi_value argv[2];
@@ -415,6 +425,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"synchronize", nullptr, Synchronize, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"useGraph", nullptr, UseGraph, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"prepareGraph", nullptr, PrepareGraph, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"setRadius", nullptr, SetRadius, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"getRadius", nullptr, GetRadius, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"generateRadius", nullptr, GenerateRadius, nullptr, nullptr, nullptr, napi_default, nullptr},

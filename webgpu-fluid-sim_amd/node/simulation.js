@@ -111,6 +111,11 @@ class Simulation {
         addon.useGraph(this.handle, enable ? stepsPerGraph : 0);
     }
 
+    // Capture the step graph now, so the next step() replays it at once.
+    prepareGraph() {
+        addon.prepareGraph(this.handle);
+    }
+
     getBuffers() {
         const b = addon.getBodies(this.handle);
         b.radiusMultiplier = addon.getRadius(this.handle);
