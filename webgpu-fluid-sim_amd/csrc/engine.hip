@@ -252,6 +252,7 @@ void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
     s->stats = a.take<unsigned long long>(4);
     s->body_counts = a.take<uint2>(cap);
     s->overflow = a.take<uint32_t>(1);
+    s->tree.fault = s->overflow;
 }
 
 // (Re)allocate every buffer for `cap` bodies.  Contents are not preserved.
@@ -407,7 +408,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     mark(2);
     // phase 2: stable sort (simulation.ts:400); it also writes each body's
     // sorted position
-    const LeafGather lg{s->rank, s->split ? s->split + kMaxBuckets : nullptr};
+    const LeafGather lg{s->rank, s->split ? s->split + kMaxBuckets : nullptr, s->overflow};
     if (s->ensemble)
         s->result_buf = launch_sort_u64((uint64_t *)s->keys[0], s->vals[0], (uint64_t *)s->keys[1],
                                         s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
@@ -882,6 +883,10 @@ int nbody_synchronize(nbody_sim *s) {
     HIP_TRY(s, hipMemcpy(&of, s->overflow, sizeof(of), hipMemcpyDeviceToHost));
     if (of) {
         HIP_TRY(s, hipMemset(s->overflow, 0, sizeof(uint32_t)));
+        if (of & kFaultWait)
+            return fail(s, NBODY_ERR_STATE,
+                        "a workgroup's wait on another gave up (sort look-back or tree fill): "
+                        "the step's results are not valid");
         return fail(s, NBODY_ERR_STACK_OVERFLOW,
                     "tree walk stack overflow: nodes were dropped (raise walk_stack)");
     }

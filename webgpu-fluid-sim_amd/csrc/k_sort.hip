@@ -235,7 +235,11 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
 #pragma unroll
         for (int j = 0; j < DPT; ++j) pj[j] = (int64_t)tile - 1;
         uint32_t pend = (1u << DPT) - 1u;
-        while (pend) {
+        for (uint32_t spins = 0; pend; ++spins) {
+            if (spins == kSpinLimit) {
+                if (lg.fault) atomicOr(lg.fault, kFaultWait);
+                break;
+            }
             uint32_t w[DPT][kLook];
 #pragma unroll
             for (int j = 0; j < DPT; ++j)

@@ -671,8 +671,14 @@ __device__ __forceinline__ float4 load_device_f4(const float4 *p) {
 __device__ __forceinline__ void wait_input(const TreeBufs &tb, uint32_t sb1, uint32_t node,
                                            uint32_t side, float4 &V, float4 &VB) {
     const uint32_t *flag = tb.counts + done_flag(sb1 - 1u);
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+    for (uint32_t spins = 0;
+         __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u; ++spins) {
+        if (spins == kSpinLimit) {
+            atomicOr(tb.fault, kFaultWait);
+            break;
+        }
         __builtin_amdgcn_s_sleep(2);
+    }
     // acquire (agent scope): pairs with the producer's release fence before
     // its flag store, so the records below are the producer's
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -951,9 +957,15 @@ __global__ __launch_bounds__(kFillBlock) void fill_mid_kernel(int64_t n, TreeBuf
     const uint32_t m3 = tb.counts[list_count(S)];
     auto wait_all = [&]() {
         for (uint32_t k = t; k < S; k += kFillBlock)
-            while (__hip_atomic_load(tb.counts + done_flag(k), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) == 0u)
+            for (uint32_t spins = 0; __hip_atomic_load(tb.counts + done_flag(k), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) == 0u;
+                 ++spins) {
+                if (spins == kSpinLimit) {
+                    atomicOr(tb.fault, kFaultWait);
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(2);
+            }
         __syncthreads();
     };
     if (m3 <= lds_limit) {

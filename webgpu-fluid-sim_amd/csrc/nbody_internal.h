@@ -8,6 +8,13 @@
 namespace nbs {
 
 constexpr int kWave = 64;
+// Waits of one workgroup on another (the sort's look-back, the fill's done
+// flags) are bounded: the workgroups they wait for are always dispatched
+// first and wait on nothing, so a wait ends in microseconds; one that polls
+// kSpinLimit times (seconds) stops, and sets kFaultWait in the handle's
+// fault word, which nbody_synchronize reports, rather than hang the GPU.
+constexpr uint32_t kSpinLimit = 1u << 24;
+constexpr uint32_t kFaultWait = 2u;
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 constexpr int kSimShift = 30;     // ensemble keys: sim id << 30 | 30-bit Morton code
 constexpr int kMaxSims = 256;     // 8-bit sim prefix
@@ -123,6 +130,7 @@ void launch_encode_buckets(const float2 *pos, int64_t n, int morton_mode, float4
 struct LeafGather {
     uint32_t *rank;
     uint32_t *bucket_start = nullptr;  // bucket sort: each bucket's first position
+    uint32_t *fault = nullptr;         // kFaultWait is or-ed in when a look-back wait gives up
 };
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
@@ -189,6 +197,7 @@ struct TreeBufs {
     uint32_t *counts;    // fill counters (fill_count_words)
     uint32_t *lists;     // the fill's top node list (fill_list_words)
     uint32_t *slot;      // [n-1] position of a top node in the top list
+    uint32_t *fault;     // kFaultWait is or-ed in when a wait on another workgroup gives up
     // with pair_keys set, the fill also writes each child's opening key
     // open_key(length, open_scale, open_bias) into its pair record's padding
     // (word 10 left, 11 right)
