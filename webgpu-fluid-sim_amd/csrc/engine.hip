@@ -111,7 +111,7 @@ struct nbody_sim {
     // body's bucket, and whether the splitters are those of the current
     // bodies' last step (else the sort takes the radix passes)
     uint32_t *split = nullptr;
-    uint16_t *bucket = nullptr;
+    uint32_t *spill = nullptr;  // the bucket sort's oversized buckets: keys, then values
     bool split_valid = false;
     bool graph_cold_sort = false;  // the graph's first step sorts without splitters
 
@@ -243,9 +243,9 @@ void layout(nbody_sim *s, Arena &a, int64_t cap, bool external_bodies) {
     s->chunk_order = s->ensemble ? a.take<uint32_t>((cap + 63) / 64) : nullptr;
     s->accel = a.take<float2>(cap);
     const bool bucket_sort = !s->ensemble && !external_bodies && cap > kSortLdsMax;
-    // splitters, then the bucket starts of the bucket sort's placing pass
-    s->split = bucket_sort ? a.take<uint32_t>(2 * kMaxBuckets) : nullptr;
-    s->bucket = bucket_sort ? a.take<uint16_t>(cap) : nullptr;
+    // the bucket sort's splitters
+    s->split = bucket_sort ? a.take<uint32_t>(kMaxBuckets) : nullptr;
+    s->spill = bucket_sort ? a.take<uint32_t>(2 * cap) : nullptr;
     s->radius = external_bodies ? nullptr : a.take<float>(cap);
     // the quadrupole pass's prefix sums (the allocation-free step is monopole only)
     s->quad_prefix = external_bodies ? nullptr : a.take<double>(quad_prefix_words(cap));
@@ -396,10 +396,16 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     const int passes = n > 1 ? sort_passes(s->key_bits, s->radix_bits) : 0;
     const SortPath path = sort_path(s);
     uint32_t *keys0 = (uint32_t *)s->keys[0], *keys1 = (uint32_t *)s->keys[1];
+    // the bucket sort: each tile of bodies grouped by bucket as it is encoded
+    // (the tiles' bucket counts and starts in the look-back words, which this
+    // path does not use otherwise)
+    const int nb = path == SortPath::kBuckets ? sort_buckets(n) : 0;
+    uint32_t *tcount = s->sort.lookback;
+    uint32_t *tstart = s->sort.lookback + (int64_t)nb * encode_group_tiles(n);
     if (path == SortPath::kBuckets)
-        launch_encode_buckets(s->pos, n, s->cfg.morton_mode, s->bbox,
-                              slots ? s->bbox_slots : nullptr, keys0, s->vals[0], s->bucket,
-                              s->split, sort_buckets(n), s->sort.hist, st);
+        launch_encode_group(s->pos, n, s->cfg.morton_mode, s->bbox,
+                            slots ? s->bbox_slots : nullptr, keys1, s->vals[1], s->split, nb,
+                            s->sort.hist, tcount, tstart, st);
     else
         launch_encode(s->pos, n, s->cfg.morton_mode, s->bbox, slots ? s->bbox_slots : nullptr,
                       s->body_sim, s->ensemble ? nullptr : keys0,
@@ -408,15 +414,19 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     mark(2);
     // phase 2: stable sort (simulation.ts:400); it also writes each body's
     // sorted position
-    const LeafGather lg{s->rank, s->split ? s->split + kMaxBuckets : nullptr, s->overflow};
+    const LeafGather lg{s->rank, s->overflow};
     if (s->ensemble)
         s->result_buf = launch_sort_u64((uint64_t *)s->keys[0], s->vals[0], (uint64_t *)s->keys[1],
                                         s->vals[1], n, s->key_bits, s->radix_bits, s->sort, lg, st);
     else if (path == SortPath::kSmall)
         s->result_buf = launch_sort_small(keys0, s->vals[0], keys1, s->vals[1], n, s->key_bits, lg, st);
-    else if (path == SortPath::kBuckets)
-        s->result_buf = launch_sort_buckets(keys0, s->vals[0], keys1, s->vals[1], n, s->key_bits,
-                                            sort_buckets(n), s->split, s->bucket, s->sort, lg, st);
+    else if (path == SortPath::kBuckets) {
+        if (launch_sort_gathered(keys1, s->vals[1], keys0, s->vals[0], s->spill, s->spill + s->cap,
+                                 n, s->key_bits, nb, s->split, s->sort.hist, tcount, tstart, lg,
+                                 st) != 0)
+            return fail(s, NBODY_ERR_CAPACITY, "bucket sort: too many bodies");
+        s->result_buf = 0;
+    }
     else
         s->result_buf = launch_sort_u32(keys0, s->vals[0], keys1, s->vals[1], n, s->key_bits,
                                         s->radix_bits, s->sort, lg, st);

@@ -246,61 +246,136 @@ __global__ __launch_bounds__(kEncodeThreads) void encode_kernel(const float2 *__
     }
 }
 
-// The plain encode for the bucket sort: keys, and each body's bucket (the
-// number of splitters <= its key: a binary search in LDS), counted per
-// bucket (LDS histogram, one global atomic per non-empty bucket).
-template <int kMode>
-__global__ __launch_bounds__(kEncodeThreads) void encode_bucket_kernel(
+// The bucket sort's encode and placing in one pass: the keys, each body's
+// bucket (the number of splitters <= its key: a binary search in LDS), then
+// the tile's bodies grouped by bucket, stably (a
+// body's place among its bucket's bodies of the tile is its rank by input
+// order: ballots over the bucket id's bits, as the radix passes rank a
+// digit), at the tile's own positions.  The tile writes, for every bucket b,
+// its count and first position among the tile's bodies to tcount[b * tiles +
+// tile] and tstart[b * tiles + tile], and adds its counts to hist; the
+// per-bucket sort gathers bucket b's bodies tile after tile, which is input
+// order, with no pass over all bodies between the two kernels.  Item it of
+// lane l in wave w is tile position w * 64 * kItems + it * 64 + l, so that
+// (w, it, l) order is input order.
+template <int kMode, int RB, int kItems>
+__global__ __launch_bounds__(kEncodeThreads) void encode_group_kernel(
     const float2 *__restrict__ pos, int64_t n, float4 *__restrict__ bbox,
-    const uint32_t *__restrict__ slots, uint32_t *__restrict__ keys32, uint32_t *__restrict__ idx,
-    uint16_t *__restrict__ bucket, const uint32_t *__restrict__ split, int nb,
-    uint32_t *__restrict__ hist) {
+    const uint32_t *__restrict__ slots, uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+    const uint32_t *__restrict__ split, int nb, uint32_t *__restrict__ hist,
+    uint32_t *__restrict__ tcount, uint32_t *__restrict__ tstart, int tiles) {
+    constexpr int D = 1 << RB;
+    constexpr int kWaves = kEncodeThreads / kWave;
+    constexpr int kTile = kItems * kEncodeThreads;
+    static_assert(D <= kEncodeThreads && D >= kMaxBuckets / 4, "one bucket per owner thread");
     __shared__ uint32_t s_split[kMaxBuckets];
-    __shared__ uint32_t s_hist[kMaxBuckets];
-    constexpr int kItems = kEncodeTile / kEncodeThreads;
-    const int64_t base = (int64_t)blockIdx.x * kEncodeTile;
+    __shared__ uint32_t s_wcnt[kWaves * D];  // wave bucket counts, then wave offsets
+    __shared__ uint32_t s_lstart[D];         // bucket start inside the tile
+    __shared__ uint32_t s_shw[kWaves];
+    __shared__ uint32_t s_keys[kTile];
+    __shared__ uint32_t s_vals[kTile];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tile = (int)blockIdx.x;
+    const int64_t base = (int64_t)tile * kTile;
     float2 p[kItems];
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
-        const int64_t i = base + it * kEncodeThreads + threadIdx.x;
+        const int64_t i = base + wid * (64 * kItems) + it * 64 + lane;
         p[it] = i < n ? pos[i] : make_float2(0.0f, 0.0f);
     }
-    for (int j = threadIdx.x; j < nb; j += kEncodeThreads) {
-        s_hist[j] = 0u;
-        if (j < nb - 1) s_split[j] = split[j];
-    }
+    for (int j = tid; j < nb - 1; j += kEncodeThreads) s_split[j] = split[j];
+    for (int j = tid; j < kWaves * D; j += kEncodeThreads) s_wcnt[j] = 0u;
     float4 box = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if constexpr (kMode == 1) {
         if (slots) {
             box = bbox_from_slots(slots);
-            if (blockIdx.x == 0 && threadIdx.x == 0) bbox[0] = box;
+            if (tile == 0 && tid == 0) bbox[0] = box;
         } else {
             box = bbox[0];
         }
     }
     __syncthreads();
-    // the largest power of two <= nb - 1 (the search's first step)
     const int ns = nb - 1;
     int top = 1;
     while (top * 2 <= ns) top *= 2;
+    uint32_t key[kItems], dg[kItems], rank[kItems];
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
-        const int64_t i = base + it * kEncodeThreads + threadIdx.x;
-        if (i >= n) break;
-        const uint32_t key = kMode == 0 ? code_reference(p[it]) : code_bbox30(p[it], box);
-        keys32[i] = key;
-        idx[i] = (uint32_t)i;
-        int b = 0;  // splitters <= key
-        if (ns > 0)
-            for (int step = top; step >= 1; step >>= 1)
-                if (b + step <= ns && s_split[b + step - 1] <= key) b += step;
-        bucket[i] = (uint16_t)b;
-        atomicAdd(&s_hist[b], 1u);
+        const int64_t i = base + wid * (64 * kItems) + it * 64 + lane;
+        key[it] = 0u;
+        dg[it] = (uint32_t)D;  // no body
+        if (i < n) {
+            key[it] = kMode == 0 ? code_reference(p[it]) : code_bbox30(p[it], box);
+            int b = 0;  // splitters <= key
+            if (ns > 0)
+                for (int step = top; step >= 1; step >>= 1)
+                    if (b + step <= ns && s_split[b + step - 1] <= key[it]) b += step;
+            dg[it] = (uint32_t)b;
+        }
+    }
+    // stable rank inside the wave among the lanes of the same bucket
+    uint32_t *wc = s_wcnt + wid * D;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint32_t d = dg[it];
+        const bool ok = d < (uint32_t)D;
+        const uint64_t valid = __ballot(ok);
+        uint32_t plo = (uint32_t)valid, phi = (uint32_t)(valid >> 32);
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+            const uint64_t bb = __ballot(m != 0u);
+            plo &= ~((uint32_t)bb ^ m);
+            phi &= ~((uint32_t)(bb >> 32) ^ m);
+        }
+        const uint64_t peers = ((uint64_t)phi << 32) | plo;
+        const uint32_t below = lanes_below(peers);
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        uint32_t before = 0;
+        if (ok) before = wc[d];
+        __builtin_amdgcn_wave_barrier();
+        if (ok && below == cnt - 1u) wc[d] = before + cnt;
+        __builtin_amdgcn_wave_barrier();
+        rank[it] = before + below;
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < nb; j += kEncodeThreads) {
-        const uint32_t c = s_hist[j];
-        if (c) atomicAdd(&hist[j], c);
+    // bucket `tid`: wave offsets, the tile's count and start
+    uint32_t total = 0;
+    if (tid < D) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_wcnt[w * D + tid];
+            s_wcnt[w * D + tid] = total;
+            total += c;
+        }
+    }
+    const uint32_t lstart = block_exclusive_scan<kEncodeThreads>(total, s_shw);
+    if (tid < D) s_lstart[tid] = lstart;
+    if (tid < nb) {
+        tcount[(int64_t)tid * tiles + tile] = total;
+        tstart[(int64_t)tid * tiles + tile] = lstart;
+        if (total) atomicAdd(&hist[tid], total);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint32_t d = dg[it];
+        if (d < (uint32_t)D) {
+            const uint32_t lp = s_lstart[d] + s_wcnt[wid * D + d] + rank[it];
+            s_keys[lp] = key[it];
+            s_vals[lp] = (uint32_t)(base + wid * (64 * kItems) + it * 64 + lane);
+        }
+    }
+    __syncthreads();
+    const int64_t rem = n - base;
+    const int tile_n = rem < kTile ? (int)rem : kTile;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int q = it * kEncodeThreads + tid;
+        if (q < tile_n) {
+            kout[base + q] = s_keys[q];
+            vout[base + q] = s_vals[q];
+        }
     }
 }
 
@@ -354,18 +429,45 @@ void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
                             hist, passes, s);
 }
 
-void launch_encode_buckets(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
-                           const uint32_t *slots, uint32_t *keys32, uint32_t *idx,
-                           uint16_t *bucket, const uint32_t *split, int nb, uint32_t *hist,
-                           hipStream_t s) {
-    if (n <= 0) return;
-    const unsigned grid = (unsigned)((n + kEncodeTile - 1) / kEncodeTile);
+template <int RB, int kItems>
+void launch_encode_group_rb(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                            const uint32_t *slots, uint32_t *keys32, uint32_t *idx,
+                            const uint32_t *split, int nb, uint32_t *hist, uint32_t *tcount,
+                            uint32_t *tstart, hipStream_t s) {
+    const int tiles = encode_group_tiles(n);
     if (morton_mode == 0)
-        encode_bucket_kernel<0><<<grid, kEncodeThreads, 0, s>>>(pos, n, bbox, slots, keys32, idx,
-                                                                bucket, split, nb, hist);
+        encode_group_kernel<0, RB, kItems><<<(unsigned)tiles, kEncodeThreads, 0, s>>>(
+            pos, n, bbox, slots, keys32, idx, split, nb, hist, tcount, tstart, tiles);
     else
-        encode_bucket_kernel<1><<<grid, kEncodeThreads, 0, s>>>(pos, n, bbox, slots, keys32, idx,
-                                                                bucket, split, nb, hist);
+        encode_group_kernel<1, RB, kItems><<<(unsigned)tiles, kEncodeThreads, 0, s>>>(
+            pos, n, bbox, slots, keys32, idx, split, nb, hist, tcount, tstart, tiles);
+}
+
+// Tiles of 1024 bodies below 2^16 (a small sort's few tiles would leave most
+// CUs idle), 4096 above.
+int encode_group_tile(int64_t n) { return n < (int64_t(1) << 16) ? 1024 : 4096; }
+int encode_group_tiles(int64_t n) {
+    const int t = encode_group_tile(n);
+    return (int)((n + t - 1) / t);
+}
+
+void launch_encode_group(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                         const uint32_t *slots, uint32_t *keys32, uint32_t *idx,
+                         const uint32_t *split, int nb, uint32_t *hist, uint32_t *tcount,
+                         uint32_t *tstart, hipStream_t s) {
+    if (n <= 0) return;
+    const bool small = encode_group_tile(n) == 1024;
+    if (nb <= 256) {
+        if (small)
+            launch_encode_group_rb<8, 1>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
+                                         hist, tcount, tstart, s);
+        else
+            launch_encode_group_rb<8, 4>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
+                                         hist, tcount, tstart, s);
+    } else {
+        launch_encode_group_rb<10, 4>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
+                                      hist, tcount, tstart, s);
+    }
 }
 
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s) {

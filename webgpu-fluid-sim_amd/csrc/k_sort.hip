@@ -50,29 +50,6 @@ constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 constexpr int kLook = 8;  // predecessors read per look-back round trip
 
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Exclusive scan of one value per thread over a kThreads-thread block.
-template <int kThreads>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *shw) {
-    constexpr int kWaves = kThreads / kWave;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) shw[wid] = x;
-    __syncthreads();
-    uint32_t before = 0;
-    for (int w = 0; w < kWaves; ++w)
-        if (w < wid) before += shw[w];
-    __syncthreads();
-    return before + x - v;
-}
 
 // One pass over digit (k >> shift) & (2^RB - 1).  Owner thread t (t <
 // kOwners) owns digits [t * DPT, t * DPT + DPT) for the scans and the
@@ -80,14 +57,11 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
 //
 // kLeaf (the last pass): each item's body also gets its output position,
 // rank[body] (a scattered write, which no barrier waits for).
-//
-// kBucket: the digit of item p is its bucket id bucket[p] instead of key bits
-// (the bucket sort's placing pass).
-template <typename K, int RB, int kTile, int kThreads, bool kLeaf, bool kBucket = false>
+template <typename K, int RB, int kTile, int kThreads, bool kLeaf>
 __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ digit_count,
-    uint32_t *lookback, uint32_t *counter, LeafGather lg, const uint16_t *__restrict__ bucket) {
+    uint32_t *lookback, uint32_t *counter, LeafGather lg) {
     constexpr int D = 1 << RB;
     constexpr int kWaves = kThreads / kWave;
     constexpr int kItems = kTile / kThreads;
@@ -103,7 +77,6 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     __shared__ uint32_t s_shw[kWaves];
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
-    __shared__ uint16_t s_bk[kBucket ? kTile : 1];  // digits of the staged items
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
@@ -121,14 +94,12 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
     uint32_t v[kItems];
     uint32_t dg[kItems];
     uint32_t rank[kItems];
-    uint32_t bk[kItems];
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const int64_t p = base + (int64_t)wid * (64 * kItems) + it * 64 + lane;
         const bool ok = p < n;
         k[it] = ok ? kin[p] : (K)0;
         v[it] = ok ? vin[p] : 0u;
-        if constexpr (kBucket) bk[it] = ok ? bucket[p] : 0u;
     }
     // where each owned digit starts in the output (exclusive digit scan)
     uint32_t dbase[DPT];
@@ -145,19 +116,11 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
             dbase[j] = run;
             run += c[j];
         }
-        // the bucket sort's per-bucket kernel reads the bucket starts
-        if constexpr (kBucket)
-            if (tile == 0 && owner)
-#pragma unroll
-                for (int j = 0; j < DPT; ++j) lg.bucket_start[tid * DPT + j] = dbase[j];
     }
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
         const int64_t p = base + (int64_t)wid * (64 * kItems) + it * 64 + lane;
-        if constexpr (kBucket)
-            dg[it] = p < n ? bk[it] : (uint32_t)D;
-        else
-            dg[it] = p < n ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
+        dg[it] = p < n ? (uint32_t)((k[it] >> shift) & kMask) : (uint32_t)D;
     }
 
     // Stable rank inside the wave: peers = lanes holding the same digit.
@@ -291,7 +254,6 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
             const uint32_t lp = s_lstart[dd] + s_wcnt[wid * D + dd] + rank[it];
             s_keys[lp] = k[it];
             s_vals[lp] = v[it];
-            if constexpr (kBucket) s_bk[lp] = (uint16_t)dd;
             if constexpr (kLeaf) lg.rank[v[it]] = s_gstart[dd] + (lp - s_lstart[dd]);
         }
     }
@@ -303,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
         const int p = it * kThreads + tid;
         if (p < tile_n) {
             const K kk = s_keys[p];
-            const uint32_t dd = kBucket ? (uint32_t)s_bk[p] : (uint32_t)((kk >> shift) & kMask);
+            const uint32_t dd = (uint32_t)((kk >> shift) & kMask);
             const uint32_t dst = s_gstart[dd] + (uint32_t)(p - (int)s_lstart[dd]);
             kout[dst] = kk;
             vout[dst] = s_vals[p];
@@ -323,11 +285,11 @@ int sort_rb_tile(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64
         if (p == passes - 1 && lg.rank)
             onesweep_kernel<K, RB, kTile, T, true><<<(unsigned)tiles, T, 0, s>>>(
                 ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
-                sc.counters + p, lg, nullptr);
+                sc.counters + p, lg);
         else
             onesweep_kernel<K, RB, kTile, T, false><<<(unsigned)tiles, T, 0, s>>>(
                 ki, vi, ko, vo, n, RB * p, sc.hist + p * D, sc.lookback + (int64_t)p * tiles * D,
-                sc.counters + p, lg, nullptr);
+                sc.counters + p, lg);
         K *tk = ki;
         ki = ko;
         ko = tk;
@@ -450,11 +412,25 @@ __device__ __forceinline__ void digit_starts(uint32_t *s_wcnt, uint32_t *s_start
 // (split[-1] = 0, split[B-1] = the largest key); start == null: one bucket
 // of n keys.  Input (kin, vin) and output (kout, vout) cover the same
 // positions; a bucket too large for LDS also overwrites its input.
-template <int kCap>
+//
+// kGather: the input is grouped by bucket inside tiles of bg.tile positions
+// (encode_group_kernel); bucket b's keys are gathered tile after tile, at
+// tile t's positions [t * bg.tile + tstart[b * tiles + t], + tcount[b * tiles
+// + t]), which is input order; the bucket starts at the sum of the counts
+// of the buckets before it.  A bucket too large for LDS is gathered into
+// its output positions and sorted there, through (sk2, sv2) at the same
+// positions.
+struct BucketGather {
+    const uint32_t *tcount = nullptr, *tstart = nullptr;
+    int tiles = 0, tile = 0;  // tiles of `tile` positions
+    uint32_t *sk2 = nullptr, *sv2 = nullptr;
+};
+constexpr int kGMaxTiles = 1024;
+template <int kCap, bool kGather = false>
 __global__ __launch_bounds__(kBucketThreads) void bucket_sort_kernel(
     uint32_t *kin, uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t n, uint32_t key_max,
     int nb, const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
-    const uint32_t *__restrict__ split, uint32_t *__restrict__ rank_out) {
+    const uint32_t *__restrict__ split, uint32_t *__restrict__ rank_out, BucketGather bg = {}) {
     constexpr int kItems = kCap / kBucketThreads;
     static_assert(kItems * kBucketThreads == kCap, "bucket capacity");
     __shared__ uint32_t s_k[kCap];
@@ -462,15 +438,46 @@ __global__ __launch_bounds__(kBucketThreads) void bucket_sort_kernel(
     __shared__ uint32_t s_wcnt[kBucketWaves * kBucketD];
     __shared__ uint32_t s_start[kBucketD], s_total[kBucketD], s_run[kBucketD];
     __shared__ uint32_t s_shw[kBucketWaves];
+    __shared__ uint32_t s_seg[kGather ? kGMaxTiles : 1], s_tst[kGather ? kGMaxTiles : 1];
+    __shared__ uint32_t s_off;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int b = (int)blockIdx.x;
 
-    const uint32_t off = start ? start[b] : 0u;
-    const uint32_t cnt = start ? count[b] : (uint32_t)n;
+    uint32_t off, cnt;
+    if constexpr (kGather) {
+        // the bucket's start (counts of the buckets before it) and, per tile,
+        // where its keys begin among the bucket's
+        cnt = count[b];
+        if (cnt == 0u) return;
+        uint32_t before = 0;
+        for (int i = tid; i < b; i += kBucketThreads) before += count[i];
+        uint32_t h = 0;
+        if (tid < bg.tiles) {
+            h = bg.tcount[(int64_t)b * bg.tiles + tid];
+            s_tst[tid] = bg.tstart[(int64_t)b * bg.tiles + tid];
+        }
+        const uint32_t seg = block_exclusive_scan<kBucketThreads>(h, s_shw);
+        if (tid < bg.tiles) s_seg[tid] = seg;
+        off = block_exclusive_scan<kBucketThreads>(before, s_shw);
+        if (tid == kBucketThreads - 1) s_off = off + before;  // the total
+        __syncthreads();
+        off = s_off;
+    } else {
+        off = start ? start[b] : 0u;
+        cnt = start ? count[b] : (uint32_t)n;
+        if (cnt == 0u) return;
+    }
+    // position p (< cnt) of the bucket's gathered input: the last tile whose
+    // segment starts at or before p
+    auto src = [&](uint32_t p) {
+        int t = 0;
+        for (int step = kGMaxTiles / 2; step >= 1; step >>= 1)
+            if (t + step < bg.tiles && s_seg[t + step] <= p) t += step;
+        return (uint32_t)t * (uint32_t)bg.tile + s_tst[t] + (p - s_seg[t]);
+    };
     // bits that vary inside the bucket's key range
     const uint32_t klo = b == 0 ? 0u : split[b - 1];
     const uint32_t khi = b == nb - 1 ? key_max : split[b] - 1u;
-    if (cnt == 0u) return;
     const uint32_t diff = klo ^ khi;
     const int passes = diff ? (32 - __clz((int)diff) + kBucketRB - 1) / kBucketRB : 0;
 
@@ -484,8 +491,14 @@ __global__ __launch_bounds__(kBucketThreads) void bucket_sort_kernel(
         for (int it = 0; it < kItems; ++it) {
             if (it >= rows) break;
             const uint32_t p = slot(it);
-            k[it] = p < cnt ? kin[off + p] : 0u;
-            v[it] = p < cnt ? vin[off + p] : 0u;
+            if constexpr (kGather) {
+                const uint32_t q = p < cnt ? src(p) : 0u;
+                k[it] = p < cnt ? kin[q] : 0u;
+                v[it] = p < cnt ? vin[q] : 0u;
+            } else {
+                k[it] = p < cnt ? kin[off + p] : 0u;
+                v[it] = p < cnt ? vin[off + p] : 0u;
+            }
         }
         // Bins: the top (up to kBinBits) of the bits that vary in the
         // bucket, so a bin is a key interval and bins are in key order.
@@ -609,6 +622,20 @@ __global__ __launch_bounds__(kBucketThreads) void bucket_sort_kernel(
     // the input and output positions.
     auto slot = [&](int it) { return (uint32_t)(wid * (64 * kItems) + it * 64 + lane); };
     uint32_t *sk = kin + off, *sv = vin + off, *dk = kout + off, *dv = vout + off;
+    if constexpr (kGather) {
+        // gathered into the output positions first, then sorted between them
+        // and the same positions of (sk2, sv2)
+        for (uint32_t p = (uint32_t)tid; p < cnt; p += kBucketThreads) {
+            const uint32_t q = src(p);
+            kout[off + p] = kin[q];
+            vout[off + p] = vin[q];
+        }
+        __syncthreads();
+        sk = kout + off;
+        sv = vout + off;
+        dk = bg.sk2 + off;
+        dv = bg.sv2 + off;
+    }
     for (int q = 0; q < passes; ++q) {
         const int shift = kBucketRB * q;
         if (tid < kBucketD) s_run[tid] = 0u;
@@ -721,39 +748,22 @@ int launch_sort_small(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint
     return 1;
 }
 
-template <int RB, int kTile>
-void launch_place_buckets(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
-                          int64_t n, const uint16_t *bucket, const SortScratch &sc,
-                          const LeafGather &lg, hipStream_t s) {
-    constexpr int T = sort_threads<RB, kTile>();
-    const int64_t tiles = (n + kTile - 1) / kTile;
-    onesweep_kernel<uint32_t, RB, kTile, T, false, true><<<(unsigned)tiles, T, 0, s>>>(
-        keys_a, vals_a, keys_b, vals_b, n, 0, sc.hist, sc.lookback, sc.counters, lg, bucket);
-}
-
-int launch_sort_buckets(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
-                        int64_t n, int key_bits, int nb, const uint32_t *split,
-                        const uint16_t *bucket, const SortScratch &sc, const LeafGather &lg,
-                        hipStream_t s) {
-    const int tile = sort_tile_for(n, false);
-    if (nb <= 256) {
-        if (tile == 2048)
-            launch_place_buckets<8, 2048>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
-        else if (tile == 4096)
-            launch_place_buckets<8, 4096>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
-        else
-            launch_place_buckets<8, 8192>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
-    } else {
-        if (tile == 2048)
-            launch_place_buckets<10, 2048>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
-        else if (tile == 4096)
-            launch_place_buckets<10, 4096>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
-        else
-            launch_place_buckets<10, 8192>(keys_a, vals_a, keys_b, vals_b, n, bucket, sc, lg, s);
-    }
-    bucket_sort_kernel<kBucketCap><<<(unsigned)nb, kBucketThreads, 0, s>>>(
-        keys_b, vals_b, keys_a, vals_a, n, sort_key_max(key_bits), nb, lg.bucket_start, sc.hist,
-        split, lg.rank);
+int launch_sort_gathered(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
+                         uint32_t *vals_out, uint32_t *keys_spill, uint32_t *vals_spill, int64_t n,
+                         int key_bits, int nb, const uint32_t *split, const uint32_t *hist,
+                         const uint32_t *tcount, const uint32_t *tstart, const LeafGather &lg,
+                         hipStream_t s) {
+    BucketGather bg;
+    bg.tcount = tcount;
+    bg.tstart = tstart;
+    bg.tile = encode_group_tile(n);
+    bg.tiles = encode_group_tiles(n);
+    bg.sk2 = keys_spill;
+    bg.sv2 = vals_spill;
+    if (bg.tiles > kGMaxTiles) return -1;
+    bucket_sort_kernel<kBucketCap, true><<<(unsigned)nb, kBucketThreads, 0, s>>>(
+        const_cast<uint32_t *>(keys_in), const_cast<uint32_t *>(vals_in), keys_out, vals_out, n,
+        sort_key_max(key_bits), nb, nullptr, hist, split, lg.rank, bg);
     return 0;
 }
 

@@ -15,6 +15,32 @@ constexpr int kWave = 64;
 // fault word, which nbody_synchronize reports, rather than hang the GPU.
 constexpr uint32_t kSpinLimit = 1u << 24;
 constexpr uint32_t kFaultWait = 2u;
+
+// Lanes of mask m below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Exclusive scan of one value per thread over a kThreads-thread block
+// (every thread calls it; shw holds kThreads / 64 words).
+template <int kThreads>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *shw) {
+    constexpr int kWaves = kThreads / kWave;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) shw[wid] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < kWaves; ++w)
+        if (w < wid) before += shw[w];
+    __syncthreads();
+    return before + x - v;
+}
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 constexpr int kSimShift = 30;     // ensemble keys: sim id << 30 | 30-bit Morton code
 constexpr int kMaxSims = 256;     // 8-bit sim prefix
@@ -112,13 +138,16 @@ void launch_encode(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
                    const uint32_t *slots, const uint8_t *body_sim, uint32_t *keys32,
                    uint64_t *keys64, uint32_t *idx, uint32_t *hist, int passes, int radix_bits,
                    hipStream_t s);
-// The same keys (plain handles) with each body's bucket among the nb - 1
-// splitters split[0..nb-2] (the number of splitters <= its key) in bucket[],
-// counted into hist[0..nb) (which must be zero).
-void launch_encode_buckets(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
-                           const uint32_t *slots, uint32_t *keys32, uint32_t *idx,
-                           uint16_t *bucket, const uint32_t *split, int nb, uint32_t *hist,
-                           hipStream_t s);
+// The same keys, grouped by bucket inside each tile of encode_group_tile(n)
+// bodies (stable; keys32/idx at the tile's positions), with every tile's
+// count and first position of each bucket in tcount / tstart (bucket-major,
+// nb x encode_group_tiles(n) words each) and the counts added to hist[0..nb).
+void launch_encode_group(const float2 *pos, int64_t n, int morton_mode, float4 *bbox,
+                         const uint32_t *slots, uint32_t *keys32, uint32_t *idx,
+                         const uint32_t *split, int nb, uint32_t *hist, uint32_t *tcount,
+                         uint32_t *tstart, hipStream_t s);
+int encode_group_tiles(int64_t n);
+int encode_group_tile(int64_t n);
 
 // ---------------------------------------------------------------- sort
 // Stable LSD sort of (keys, vals) on the low `key_bits` bits, radix_bits
@@ -129,7 +158,6 @@ void launch_encode_buckets(const float2 *pos, int64_t n, int morton_mode, float4
 // when `rank` is set (the build then places the leaf records).
 struct LeafGather {
     uint32_t *rank;
-    uint32_t *bucket_start = nullptr;  // bucket sort: each bucket's first position
     uint32_t *fault = nullptr;         // kFaultWait is or-ed in when a look-back wait gives up
 };
 int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
@@ -138,9 +166,8 @@ int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32
 // The bucket sort of plain handles (bit-identical to the radix sort): one
 // workgroup sorts n <= kSortLdsMax keys (launch_sort_small, result in the b
 // buffers); larger sorts up to kMaxBuckets << kBucketShift keys split them
-// between the previous step's splitters (launch_sort_buckets, result in the
-// a buffers; the encode's bucket counts in sc.hist, the bucket starts in
-// lg.bucket_start).
+// between the previous step's splitters (launch_encode_group groups each
+// tile's keys by bucket, launch_sort_gathered sorts every bucket).
 constexpr int64_t kSortLdsMax = 4096;
 constexpr int kBucketShift = 12;     // splitter spacing: 4096 sorted positions
 constexpr int kMaxBuckets = 1024;
@@ -151,10 +178,17 @@ __host__ __device__ inline int sort_buckets(int64_t n) {
 uint32_t sort_key_max(int key_bits);
 int launch_sort_small(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
                       int64_t n, int key_bits, const LeafGather &lg, hipStream_t s);
-int launch_sort_buckets(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
-                        int64_t n, int key_bits, int nb, const uint32_t *split,
-                        const uint16_t *bucket, const SortScratch &sc, const LeafGather &lg,
-                        hipStream_t s);
+// The per-bucket sort of launch_encode_group's output: bucket b's keys
+// gathered tile after tile from (keys_in, vals_in), sorted to (keys_out,
+// vals_out) at the bucket's start (the counts of the buckets before it, from
+// hist); buckets too large for LDS are sorted between their output positions
+// and the same positions of (keys_spill, vals_spill).  Returns -1 (nothing
+// launched) for more than 2^22 keys.
+int launch_sort_gathered(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
+                         uint32_t *vals_out, uint32_t *keys_spill, uint32_t *vals_spill, int64_t n,
+                         int key_bits, int nb, const uint32_t *split, const uint32_t *hist,
+                         const uint32_t *tcount, const uint32_t *tstart, const LeafGather &lg,
+                         hipStream_t s);
 int launch_sort_u64(uint64_t *keys_a, uint32_t *vals_a, uint64_t *keys_b, uint32_t *vals_b,
                     int64_t n, int key_bits, int radix_bits, const SortScratch &sc,
                     const LeafGather &lg, hipStream_t s);
