@@ -45,7 +45,10 @@ def same(a, b):
             assert np.array_equal(u, v), (c, name)
 
 
-@pytest.mark.parametrize("n", [2, 3, 1000, 4096, 4097, 16384, 4096 * 5 + 1, 50000, 262144])
+# bucket spacings: 1024 positions up to 2^18 keys (encode tiles of 1024
+# bodies below 2^18, 4096 from there), 2048 up to 2^19, 4096 above
+@pytest.mark.parametrize("n", [2, 3, 1000, 4096, 4097, 16384, 4096 * 5 + 1, 50000, 131079,
+                               262144, 262145, 524288, 524289])
 def test_sort_paths_identical(n):
     bodies = nbody_sim.generate("exp-disk", n, seed=7)
     cfg, _ = params(theta=0.7, dt=5e-5)

@@ -359,7 +359,7 @@ void prime_bbox(nbody_sim *s) {
 // (the first step after the bodies were set, ensembles, NBODY_DEBUG_SORT_RADIX)
 // the radix passes.  Every path gives the same permutation.
 enum class SortPath { kRadix, kSmall, kBuckets };
-int64_t bucket_sort_max() { return (int64_t)kMaxBuckets << kBucketShift; }
+int64_t bucket_sort_max() { return (int64_t)kMaxBuckets << kBucketShiftMax; }
 SortPath sort_path(const nbody_sim *s) {
     const int64_t n = s->n;
     if (s->ensemble || (s->debug & NBODY_DEBUG_SORT_RADIX) || n <= 1) return SortPath::kRadix;
@@ -437,7 +437,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     const int64_t clear = sort_clear_words(n, s->key_bits, s->radix_bits, s->ensemble);
     // the build of a bucket-sortable size writes the next sort's splitters
     const bool splits = s->split && n > kSortLdsMax && n <= bucket_sort_max();
-    const LeafPlace lpl{s->pos, s->mass, s->rank, splits ? s->split : nullptr};
+    const LeafPlace lpl{s->pos, s->mass, s->rank, splits ? s->split : nullptr, bucket_shift(n)};
     if (s->ensemble)
         launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree, lpl,
                          s->sim_root, s->sort.hist, clear, nullptr, st);

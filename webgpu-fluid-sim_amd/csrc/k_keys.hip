@@ -443,9 +443,11 @@ void launch_encode_group_rb(const float2 *pos, int64_t n, int morton_mode, float
             pos, n, bbox, slots, keys32, idx, split, nb, hist, tcount, tstart, tiles);
 }
 
-// Tiles of 1024 bodies below 2^16 (a small sort's few tiles would leave most
-// CUs idle), 4096 above.
-int encode_group_tile(int64_t n) { return n < (int64_t(1) << 16) ? 1024 : 4096; }
+// Tiles of 1024 bodies below 2^18 (a small sort's few tiles would leave most
+// CUs idle: 65 536 bodies, encode 12.3 -> 9.1 us; 131 072, 12.8 -> 10.5; no
+// gain at 2^18), 4096 above.  Below 2^18 keys there are at most 256 buckets
+// (bucket_shift), which the 1024-body tiles' 8-bit bucket ranking needs.
+int encode_group_tile(int64_t n) { return n < (int64_t(1) << 18) ? 1024 : 4096; }
 int encode_group_tiles(int64_t n) {
     const int t = encode_group_tile(n);
     return (int)((n + t - 1) / t);
@@ -456,18 +458,17 @@ void launch_encode_group(const float2 *pos, int64_t n, int morton_mode, float4 *
                          const uint32_t *split, int nb, uint32_t *hist, uint32_t *tcount,
                          uint32_t *tstart, hipStream_t s) {
     if (n <= 0) return;
-    const bool small = encode_group_tile(n) == 1024;
-    if (nb <= 256) {
-        if (small)
-            launch_encode_group_rb<8, 1>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
-                                         hist, tcount, tstart, s);
-        else
-            launch_encode_group_rb<8, 4>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
-                                         hist, tcount, tstart, s);
-    } else {
+    // the kernel's tile (kItems x 1024) is encode_group_tile(n); 1024-body
+    // tiles come with at most 256 buckets (n < 2^18)
+    if (encode_group_tile(n) == 1024)
+        launch_encode_group_rb<8, 1>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
+                                     hist, tcount, tstart, s);
+    else if (nb <= 256)
+        launch_encode_group_rb<8, 4>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
+                                     hist, tcount, tstart, s);
+    else
         launch_encode_group_rb<10, 4>(pos, n, morton_mode, bbox, slots, keys32, idx, split, nb,
                                       hist, tcount, tstart, s);
-    }
 }
 
 void launch_clear_u32(uint32_t *p, int64_t words, hipStream_t s) {

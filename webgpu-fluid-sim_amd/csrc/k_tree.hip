@@ -367,10 +367,10 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
         const Split sp = karras_node(delta, i, n32, valid);
         if (!valid) return;
         if (lpl.split) {  // the splitters of the next step's bucket sort
-            constexpr int kMask = (1 << kBucketShift) - 1;
-            if (i > 0 && (i & kMask) == 0) lpl.split[(i >> kBucketShift) - 1] = keys[i];
-            if (i == n32 - 2 && ((n32 - 1) & kMask) == 0)
-                lpl.split[((n32 - 1) >> kBucketShift) - 1] = keys[n32 - 1];
+            const int sh = lpl.split_shift, mask = (1 << sh) - 1;
+            if (i > 0 && (i & mask) == 0) lpl.split[(i >> sh) - 1] = keys[i];
+            if (i == n32 - 2 && ((n32 - 1) & mask) == 0)
+                lpl.split[((n32 - 1) >> sh) - 1] = keys[n32 - 1];
         }
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);

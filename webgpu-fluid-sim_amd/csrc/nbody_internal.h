@@ -165,15 +165,24 @@ int launch_sort_u32(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32
                     const LeafGather &lg, hipStream_t s);
 // The bucket sort of plain handles (bit-identical to the radix sort): one
 // workgroup sorts n <= kSortLdsMax keys (launch_sort_small, result in the b
-// buffers); larger sorts up to kMaxBuckets << kBucketShift keys split them
+// buffers); larger sorts up to kMaxBuckets << kBucketShiftMax keys split them
 // between the previous step's splitters (launch_encode_group groups each
 // tile's keys by bucket, launch_sort_gathered sorts every bucket).
 constexpr int64_t kSortLdsMax = 4096;
-constexpr int kBucketShift = 12;     // splitter spacing: 4096 sorted positions
+constexpr int kBucketShiftMax = 12;  // widest splitter spacing: 4096 sorted positions
 constexpr int kMaxBuckets = 1024;
 constexpr int kBucketCap = 6144;     // largest bucket sorted in LDS (larger: through memory)
+// log2 of the splitter spacing for n keys: the narrowest of 1024, 2048 and
+// 4096 positions that makes at most 256 buckets (small sorts want many
+// workgroups, large ones few passes per key)
+__host__ __device__ inline int bucket_shift(int64_t n) {
+    for (int sh = 10; sh < kBucketShiftMax; ++sh)
+        if (((n + (int64_t(1) << sh) - 1) >> sh) <= 256) return sh;
+    return kBucketShiftMax;
+}
 __host__ __device__ inline int sort_buckets(int64_t n) {
-    return (int)((n + (int64_t(1) << kBucketShift) - 1) >> kBucketShift);
+    const int sh = bucket_shift(n);
+    return (int)((n + (int64_t(1) << sh) - 1) >> sh);
 }
 uint32_t sort_key_max(int key_bits);
 int launch_sort_small(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
@@ -247,7 +256,8 @@ struct LeafPlace {
     const float2 *pos;
     const float *mass;
     const uint32_t *rank;
-    uint32_t *split = nullptr;  // u32 builds: the next sort's splitters (keys at j << kBucketShift)
+    uint32_t *split = nullptr;  // u32 builds: the next sort's splitters (keys at j << split_shift)
+    int split_shift = kBucketShiftMax;
 };
 void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, const LeafPlace &lp,
                       uint32_t *clear, int64_t clear_words, uint32_t *slots, hipStream_t s);
