@@ -709,27 +709,52 @@ __device__ __forceinline__ void resolve_list(const TreeBufs &tb, ListLds &ls, ui
     __syncthreads();
     const uint32_t nstart = ls.nstart;
     // Top list: the last wavefront loads the phase-2 inputs into LDS as their
-    // super-blocks finish and then clears the entry's src (release), so a
-    // climb waits on LDS; when the climbs need that wavefront too, each climb
-    // reads its own phase-2 inputs.  A wavefront's LDS operations execute in
-    // order, so a climb that reads src as 0 and then an input reads it
-    // complete.
+    // super-blocks finish and then clears that side of the entry's src
+    // (release), so a climb waits on LDS; when the climbs need that wavefront
+    // too, each climb reads its own phase-2 inputs.  A wavefront's LDS
+    // operations execute in order, so a climb that reads a side of src as 0
+    // and then that input reads it complete.  Each loader lane polls the done
+    // flags of all its pending inputs (lane l: entries l + 64k, bit 2k + side)
+    // and loads every input whose super-block is done, behind one acquire per
+    // round: the last super-block's inputs are one round away from its flag,
+    // whichever lanes hold them.
     constexpr uint32_t kLoader = kFillBlock - 64;
     const bool loaders = kTop && nstart <= kLoader;
     if (kTop && loaders && threadIdx.x >= kLoader) {
-        for (uint32_t e = threadIdx.x - kLoader; e < m; e += 64) {
-            const uint32_t sc = ls.src[e];
-            if (!sc) continue;
-#pragma unroll
-            for (uint32_t side = 0; side < 2; ++side) {
-                const uint32_t sb1 = (sc >> (16u * side)) & 0xFFFFu;
-                if (!sb1) continue;
-                float4 V, VB;
-                wait_input(tb, sb1, ls.node[e], side, V, VB);
-                ls.in[e][2 * side] = V;
-                ls.in[e][2 * side + 1] = VB;
+        const uint32_t l = threadIdx.x - kLoader;  // m <= kLdsList: k < 16
+        uint32_t pend = 0u;
+        for (uint32_t k = 0; l + 64u * k < m; ++k) {
+            const uint32_t sc = ls.src[l + 64u * k];
+            pend |= ((sc & 0xFFFFu) ? 1u : 0u) << (2u * k);
+            pend |= ((sc >> 16) ? 1u : 0u) << (2u * k + 1u);
+        }
+        for (uint32_t spins = 0; __ballot(pend != 0u); ++spins) {
+            const bool give_up = spins == kSpinLimit;
+            uint32_t ready = 0u;
+            for (uint32_t b = pend; b; b &= b - 1u) {
+                const uint32_t bit = (uint32_t)__builtin_ctz(b), e = l + 64u * (bit >> 1);
+                const uint32_t sb1 = (ls.src[e] >> (16u * (bit & 1u))) & 0xFFFFu;
+                if (give_up || __hip_atomic_load(tb.counts + done_flag(sb1 - 1u), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                    ready |= 1u << bit;
             }
-            __hip_atomic_store(&ls.src[e], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (give_up && pend) atomicOr(tb.fault, kFaultWait);
+            if (!__ballot(ready != 0u)) {
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+            // acquire (agent scope): pairs with the producers' release fences
+            // before their flag stores
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            for (uint32_t b = ready; b; b &= b - 1u) {
+                const uint32_t bit = (uint32_t)__builtin_ctz(b), e = l + 64u * (bit >> 1);
+                const uint32_t side = bit & 1u, node = ls.node[e];
+                ls.in[e][2 * side] = load_device_f4(tb.pcm + kPairStride * (size_t)node + side);
+                ls.in[e][2 * side + 1] = load_device_f4(tb.paabb + 2 * (size_t)node + side);
+                __hip_atomic_fetch_and(&ls.src[e], ~(0xFFFFu << (16u * side)), __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            pend &= ~ready;
         }
     }
     // input `side` of entry e, produced by a phase-2 workgroup
