@@ -78,6 +78,8 @@ struct KeyWindow {
         if (!in) v = global_read(g + j);
         return v;
     }
+    // a position known to lie in the window
+    __device__ __forceinline__ K lds(int j) const { return s[j - lo]; }
 };
 // Load the window of the workgroup's nodes [b0, b0 + kBuildNodes) into s_keys.
 template <typename K>
@@ -86,7 +88,19 @@ __device__ __forceinline__ KeyWindow<K> load_window(const K *__restrict__ keys, 
     const int b0 = (int)blockIdx.x * kBuildNodes;
     const int lo = b0 > kBuildHalo ? b0 - kBuildHalo : 0;
     const int hi = b0 + kBuildNodes + kBuildHalo < n ? b0 + kBuildNodes + kBuildHalo : n;
-    for (int j = lo + (int)threadIdx.x; j < hi; j += kBuildThreads) s_keys[j - lo] = keys[j];
+    // every load issued before the first LDS write: one memory round trip
+    constexpr int kPer = (kBuildWindow + kBuildThreads - 1) / kBuildThreads;
+    K v[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int j = lo + (int)threadIdx.x + r * kBuildThreads;
+        v[r] = j < hi ? keys[j] : K(0);
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int j = lo + (int)threadIdx.x + r * kBuildThreads;
+        if (j < hi) s_keys[j - lo] = v[r];
+    }
     __syncthreads();
     return KeyWindow<K>{keys, s_keys, lo, hi - lo};
 }
@@ -101,14 +115,23 @@ struct DeltaU32 {
         const DeltaU32 &d;
         uint32_t ki;
         int i;
-        __device__ __forceinline__ int operator()(int j) const {
-            if ((uint32_t)j >= (uint32_t)d.n) return -1;
-            const uint32_t x = ki ^ d.k[j];
+        __device__ __forceinline__ int cmp(int j, uint32_t kj) const {
+            const uint32_t x = ki ^ kj;
             const int by_key = __clz((int)x), by_pos = 32 + __clz(i ^ j);
             return x ? by_key : by_pos;
         }
+        __device__ __forceinline__ int operator()(int j) const {
+            if ((uint32_t)j >= (uint32_t)d.n) return -1;
+            return cmp(j, d.k[j]);
+        }
+        // j in the window (or outside [0, n))
+        __device__ __forceinline__ int local(int j) const {
+            if ((uint32_t)j >= (uint32_t)d.n) return -1;
+            return cmp(j, d.k.lds(j));
+        }
     };
-    __device__ __forceinline__ Node at(int i) const { return Node{*this, k[i], i}; }
+    // i in the window
+    __device__ __forceinline__ Node at(int i) const { return Node{*this, k.lds(i), i}; }
 };
 
 // Ensemble keys carry the sim id above bit 30.  Equal keys share a sim and
@@ -122,17 +145,25 @@ struct DeltaU64 {
         const DeltaU64 &d;
         uint64_t ki;
         int i;
-        __device__ __forceinline__ int operator()(int j) const {
-            if ((uint32_t)j >= (uint32_t)d.n) return -1;
-            const uint64_t b = d.k[j];
+        __device__ __forceinline__ int cmp(int j, uint64_t b) const {
             if (ki == b) {
                 const int off = (int)d.sim_off[ki >> kSimShift];
                 return 64 + __clz((i - off) ^ (j - off));
             }
             return __clzll((long long)(ki ^ b));
         }
+        __device__ __forceinline__ int operator()(int j) const {
+            if ((uint32_t)j >= (uint32_t)d.n) return -1;
+            return cmp(j, d.k[j]);
+        }
+        // j in the window (or outside [0, n))
+        __device__ __forceinline__ int local(int j) const {
+            if ((uint32_t)j >= (uint32_t)d.n) return -1;
+            return cmp(j, d.k.lds(j));
+        }
     };
-    __device__ __forceinline__ Node at(int i) const { return Node{*this, k[i], i}; }
+    // i in the window
+    __device__ __forceinline__ Node at(int i) const { return Node{*this, k.lds(i), i}; }
 };
 
 struct Split {
@@ -260,17 +291,18 @@ __device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n, b
     if (valid) {
         const Node dl = delta.at(i);
         ki = dl.ki;
-        d = dl(i + 1) > dl(i - 1) ? 1 : -1;
-        delta_min = dl(i - d);
+        // every probe of this part lies in the window
+        d = dl.local(i + 1) > dl.local(i - 1) ? 1 : -1;
+        delta_min = dl.local(i - d);
         // the farthest step inside the window, and inside the array
         xw = d > 0 ? delta.k.lo + delta.k.len - 1 - i : i - delta.k.lo;
         const int xa = d > 0 ? n - 1 - i : i;
         int l_max = 2;
-        while (l_max <= xw && dl(d > 0 ? i + l_max : i - l_max) > delta_min) l_max *= 2;
-        far = l_max > xw && xw < xa && dl(d > 0 ? i + xw : i - xw) > delta_min;
+        while (l_max <= xw && dl.local(d > 0 ? i + l_max : i - l_max) > delta_min) l_max *= 2;
+        far = l_max > xw && xw < xa && dl.local(d > 0 ? i + xw : i - xw) > delta_min;
         if (!far)  // the range ends inside the window: steps beyond xw fail
             for (int t = l_max >> 1; t >= 1; t >>= 1)
-                if (l + t <= xw && dl(d > 0 ? i + l + t : i - l - t) > delta_min) l += t;
+                if (l + t <= xw && dl.local(d > 0 ? i + l + t : i - l - t) > delta_min) l += t;
     }
     for (uint64_t m = __ballot(far); m; m &= m - 1) {
         const int ld = __builtin_ctzll(m);
@@ -286,13 +318,17 @@ __device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n, b
         delta_node = dl(d > 0 ? i + l : i - l);
         int bound = l;
         if (l > xw) {  // a wide node: is the split beyond the window too?
-            far2 = dl(d > 0 ? i + xw : i - xw) > delta_node;
+            far2 = dl.local(d > 0 ? i + xw : i - xw) > delta_node;
             bound = xw;
         }
         if (!far2) {
+            // the test fails at every step >= bound (the range's end, or a
+            // window end past which the prefix is no longer than delta_node),
+            // so steps beyond it are not probed and every probe lies in the
+            // window
             int t = (bound + 1) >> 1;
             for (;;) {
-                if (dl(d > 0 ? i + s + t : i - s - t) > delta_node) s += t;
+                if (s + t <= bound && dl.local(d > 0 ? i + s + t : i - s - t) > delta_node) s += t;
                 if (t == 1) break;
                 t = (t + 1) >> 1;
             }
@@ -318,13 +354,16 @@ __device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n, b
 // build, which runs after the sort has finished with it), and the walk's
 // bounding-box accumulators (the encode has read them; the walk that
 // follows refills them).
+// (kThreads: the workgroup size, a constant, so that no wave waits for the
+// dispatch packet's copy of it)
+template <int kThreads>
 __device__ __forceinline__ void clear_words(uint32_t *__restrict__ p, int64_t words,
                                             uint32_t *__restrict__ slots) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
-         i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < words;
+         i += (int64_t)gridDim.x * kThreads)
         p[i] = 0u;
     if (slots && blockIdx.x == 0)
-        for (uint32_t k = threadIdx.x; k < (uint32_t)kBBoxSlots; k += blockDim.x)
+        for (uint32_t k = threadIdx.x; k < (uint32_t)kBBoxSlots; k += kThreads)
             reinterpret_cast<uint4 *>(slots)[k] = make_uint4(0u, 0u, 0u, 0u);
 }
 
@@ -353,19 +392,26 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
     const uint32_t *__restrict__ keys, int64_t n, TreeBufs tb, LeafPlace lpl,
     uint32_t *__restrict__ clear, int64_t clear_n, uint32_t *__restrict__ slots) {
     __shared__ uint32_t s_keys[kBuildWindow];
-    clear_words(clear, clear_n, slots);
+    clear_words<kBuildThreads>(clear, clear_n, slots);
     const int n32 = (int)n;
     const DeltaU32 delta{load_window(keys, n32, s_keys), n32};
     for (int r = 0; r < kBuildPerThread; ++r) {
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
         const bool valid = i < n32 - 1;
+        // body i's leaf record: read before the searches, written after them
+        // (the searches wait on LDS only, so the reads' latency is hidden)
+        float4 leaf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t at = 0u;
         if (valid) {
-            place_leaf(tb, lpl, i);
-            if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
+            const float2 p = lpl.pos[i];
+            leaf = make_float4(p.x, p.y, lpl.mass[i], 0.0f);
+            at = lpl.rank[i];
         }
         // every lane takes part in its wavefront's searches of wide nodes
         const Split sp = karras_node(delta, i, n32, valid);
         if (!valid) return;
+        tb.leaf[at] = leaf;
+        if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
         if (lpl.split) {  // the splitters of the next step's bucket sort
             const int sh = lpl.split_shift, mask = (1 << sh) - 1;
             if (i > 0 && (i & mask) == 0) lpl.split[(i >> sh) - 1] = keys[i];
@@ -399,19 +445,26 @@ __global__ __launch_bounds__(kBuildThreads) void build_u64_kernel(const uint64_t
                                                         int64_t clear_n,
                                                         uint32_t *__restrict__ slots) {
     __shared__ uint64_t s_keys[kBuildWindow];
-    clear_words(clear, clear_n, slots);
+    clear_words<kBuildThreads>(clear, clear_n, slots);
     const int n32 = (int)n;
     const DeltaU64 delta{load_window(keys, n32, s_keys), sim_off, n32};
     for (int r = 0; r < kBuildPerThread; ++r) {
         const int i = (int)blockIdx.x * kBuildNodes + r * kBuildThreads + (int)threadIdx.x;
         const bool valid = i < n32 - 1;
+        // body i's leaf record: read before the searches, written after them
+        // (the searches wait on LDS only, so the reads' latency is hidden)
+        float4 leaf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t at = 0u;
         if (valid) {
-            place_leaf(tb, lpl, i);
-            if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
+            const float2 p = lpl.pos[i];
+            leaf = make_float4(p.x, p.y, lpl.mass[i], 0.0f);
+            at = lpl.rank[i];
         }
         // every lane takes part in its wavefront's searches of wide nodes
         const Split sp = karras_node(delta, i, n32, valid);
         if (!valid) return;
+        tb.leaf[at] = leaf;
+        if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);
         const uint32_t ib = in_block(sp);
