@@ -87,7 +87,8 @@ struct nbody_sim {
     // any change of the positions outside a step, or a tree-only step).
     uint32_t *bbox_slots = nullptr;
     bool bbox_fresh = false;
-    int max_chunks = 1;
+    int max_chunks = 1;    // bounding-box partials per sim (allocated)
+    int sim_chunks = 1;    // bounding-box chunks of the largest sim (launched)
     int64_t *sim_off = nullptr;
     uint32_t *chunk_order = nullptr;  // ensembles: the queue walk's chunks, largest sims first
     uint32_t resident_waves = 0;      // wavefronts the GPU holds at once
@@ -388,7 +389,8 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     if (slots)
         prime_bbox(s);
     else if (s->ensemble || s->cfg.morton_mode == NBODY_MORTON_BBOX30)
-        launch_bbox(s->pos, s->sim_off, s->num_sims, s->max_chunks, s->bbox_partial, s->bbox,
+        launch_bbox(s->pos, s->sim_off, s->num_sims, s->ensemble ? s->sim_chunks : s->max_chunks,
+                    s->bbox_partial, s->bbox,
                     nullptr, st);
     mark(1);
     // phase 1: Morton encode (lbvh.wgsl:42-69) and the sort's digit counts
@@ -591,7 +593,14 @@ nbody_sim *create_common(const nbody_config *cfg, int32_t device, int *err, bool
     s->num_sims = num_sims;
     s->sim_off_host.assign(num_sims + 1, 0);
     if (ensemble) {
-        for (int i = 0; i < num_sims; ++i) s->sim_off_host[i + 1] = s->sim_off_host[i] + n_per_sim[i];
+        int64_t largest = 1;
+        for (int i = 0; i < num_sims; ++i) {
+            s->sim_off_host[i + 1] = s->sim_off_host[i] + n_per_sim[i];
+            largest = std::max<int64_t>(largest, n_per_sim[i]);
+        }
+        // the per-sim boxes' grid: the largest sim's chunks, not the whole
+        // ensemble's (every other workgroup would find no bodies)
+        s->sim_chunks = (int)((largest + kBBoxChunk - 1) / kBBoxChunk);
         capacity = s->sim_off_host[num_sims];
         int sim_bits = 0;
         while ((1 << sim_bits) < num_sims) ++sim_bits;
