@@ -303,6 +303,54 @@ def test_coincident_bodies():
     assert np.array_equal(g["accel"], r["accel"])
 
 
+def test_deep_tree():
+    """A tree 46 levels deep (plain trees of up to 2^26 bodies stay below
+    57): 30 bodies whose bbox30 keys are single bits, each peeling one level
+    off the path to the origin's key cell, a body at the box's far corner,
+    and 2^16 bodies inside the origin's key cell at distinct positions (equal
+    keys: a sub-tree of 16 levels on the sorted position),
+    so that the walks open nodes all the way down.  Exact mode: tree,
+    accelerations and state bit for bit.  Fast mode (the pair walk and its
+    fixed stack): no overflow, the benchmarked walk's sums equal the
+    stats-mode walk's, and the accelerations within 1e-5 of the oracle's."""
+    m = 1 << 16
+    rng = np.random.default_rng(11)
+    cluster = rng.random((m, 2)) * 1.4e-5  # every coordinate rounds to key cell 0
+    # equal keys order by body index: index the cell's bodies along a finer
+    # Morton curve, so that its sub-tree is spatially coherent
+    q = (cluster / 1.4e-5 * 65535).astype(np.uint64)
+    code = np.zeros(m, np.uint64)
+    for b in range(16):
+        code |= ((q[:, 0] >> np.uint64(b)) & np.uint64(1)) << np.uint64(2 * b + 1)
+        code |= ((q[:, 1] >> np.uint64(b)) & np.uint64(1)) << np.uint64(2 * b)
+    cluster = cluster[np.argsort(code, kind="stable")]
+    single = ([(2.0 ** b / 32767, 0.0) for b in range(15)] +
+              [(0.0, 2.0 ** b / 32767) for b in range(15)])
+    pos = np.vstack([cluster, single, [(1.0, 1.0)]]).astype(np.float32)
+    n = len(pos)
+    mass = np.full(n, 1.0 / n, np.float32)
+    vel = np.zeros_like(pos)
+    cfg, P = params(theta=0.5, eps=1e-9)
+    g = gpu_step(cfg, mass, pos, vel)
+    r = oracle_step(P, mass, pos, vel)
+    _compare_tree(g, r)
+    assert len(np.unique(g["keys"])) == 32  # the cell's key, 30 single bits, the corner
+    depth, frontier = 0, np.array([0])
+    while frontier.size:
+        ch = g["children"][frontier].ravel()
+        frontier = ch[ch < n - 1]
+        depth += 1
+    assert depth >= 46, depth
+    assert np.array_equal(g["accel"], r["accel"])
+    assert np.array_equal(g["pos"], r["pos"]) and np.array_equal(g["vel"], r["vel"])
+    f = gpu_step(cfg, mass, pos, vel, math="fast")
+    fs = gpu_step(cfg, mass, pos, vel, math="fast", stats=True)
+    assert fs["stats"]["overflowed"] == 0
+    assert np.array_equal(f["accel"], fs["accel"])
+    e = rel_err(f["accel"], r["accel"])
+    assert np.median(e) < 1e-6 and np.mean(e > 1e-5) < 1e-3
+
+
 def test_stats_match_oracle():
     mass, pos, vel = nbody_sim.generate("uniform-disk", 8192, seed=2)
     cfg, P = params(theta=0.5)
