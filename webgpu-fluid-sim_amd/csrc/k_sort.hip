@@ -327,14 +327,16 @@ int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t 
 // ---------------------------------------------------------------- bucket sort
 // Bodies move little between steps, so the previous step's sorted keys split
 // the next step's keys into nearly equal buckets: splitter j (j = 1 .. B-1)
-// is the key at sorted position j * S.  A body's bucket is the number of
-// splitters <= its key (the encode counts them, so equal keys share a
-// bucket, as does every key of a bucket's range).  One stable one-sweep pass
-// over the bucket ids places every bucket, then bucket_sort_kernel sorts
-// each bucket, one workgroup per bucket, with stable 8-bit LSD passes over
-// the key bits that vary inside the bucket's key range.  A sort of at most
-// kSortLdsMax keys is one bucket, sorted by one workgroup straight from the
-// encode.  The result is the radix sort's: the stable order is unique.
+// is the key at sorted position j * S (S = 1024, 2048 or 4096: bucket_shift).
+// A body's bucket is the number of splitters <= its key (the encode finds
+// it, so equal keys share a bucket, as does every key of a bucket's range).
+// The encode groups each tile's bodies by bucket (encode_group_kernel);
+// bucket_sort_kernel gathers each bucket tile after tile, one workgroup per
+// bucket, and sorts it with stable 8-bit LSD passes over the key bits that
+// vary inside the bucket's key range, or by counting within bins.  A sort of
+// at most kSortLdsMax keys is one bucket, sorted by one workgroup straight
+// from the encode.  The result is the radix sort's: the stable order is
+// unique.
 constexpr int kBucketThreads = 1024;
 constexpr int kBucketWaves = kBucketThreads / kWave;
 constexpr int kBucketRB = 8;
