@@ -241,6 +241,14 @@ __device__ __forceinline__ void list_node(const TreeBufs &tb, uint32_t i, const 
 }
 
 __device__ __forceinline__ int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+// a key from a per-lane source lane
+__device__ __forceinline__ uint32_t shfl_key(uint32_t v, int src) {
+    return (uint32_t)__shfl((int)v, src);
+}
+__device__ __forceinline__ uint64_t shfl_key(uint64_t v, int src) {
+    return (uint64_t)(uint32_t)__shfl((int)(uint32_t)v, src) |
+           ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(v >> 32), src) << 32);
+}
 __device__ __forceinline__ uint32_t bcast(uint32_t v, int lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
@@ -268,6 +276,33 @@ __device__ __forceinline__ int wave_search(const Delta &delta, K ki, int i, int 
         const int next_lo = lo + c * step;
         hi = min(lo + (c + 1) * step, hi);
         lo = next_lo;
+    }
+    return lo;
+}
+
+// Two such searches at once, one per half-wavefront of 32 lanes (31
+// probes per round each): the two wide nodes a wavefront most often holds
+// (the children at one split, of neighbouring indices) take the rounds of
+// one search instead of two.  Arguments per lane: those of the search its
+// half serves (uniform within each half); every lane active.  Returns each
+// half's result (uniform within the half).
+template <typename Delta, typename K>
+__device__ __forceinline__ int wave_search2(const Delta &delta, K ki, int i, int d, int bound,
+                                            int lo, int hi) {
+    const int lane = threadIdx.x & 63, h = lane & 31;
+    const typename Delta::Node nd{delta, ki, i};
+    while (__ballot(hi - lo > 1)) {
+        const bool open = hi - lo > 1;
+        const int step = (hi - lo + 31) / 32;
+        const int x = lo + h * step;
+        const bool p = open && h > 0 && x < hi && nd(d > 0 ? i + x : i - x) > bound;
+        const uint64_t b = __ballot(p);
+        const int c = __builtin_popcount((uint32_t)(lane < 32 ? b : b >> 32));
+        if (open) {
+            const int next_lo = lo + c * step;
+            hi = min(lo + (c + 1) * step, hi);
+            lo = next_lo;
+        }
     }
     return lo;
 }
@@ -304,7 +339,20 @@ __device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n, b
             for (int t = l_max >> 1; t >= 1; t >>= 1)
                 if (l + t <= xw && dl.local(d > 0 ? i + l + t : i - l - t) > delta_min) l += t;
     }
-    for (uint64_t m = __ballot(far); m; m &= m - 1) {
+    uint64_t fm = __ballot(far);
+    if (__builtin_popcountll(fm) == 2) {
+        // both at once, the first one's in lanes 0-31, the second one's in 32-63
+        const int l0 = __builtin_ctzll(fm), l1 = __builtin_ctzll(fm & (fm - 1));
+        const int ld = lane < 32 ? l0 : l1;
+        const int di = __shfl(d, ld), ii = __shfl(i, ld);
+        const int r = wave_search2(delta, shfl_key(ki, ld), ii, di, __shfl(delta_min, ld),
+                                   __shfl(xw, ld), (di > 0 ? n - 1 - ii : ii) + 1);
+        const int r0 = bcast(r, 0), r1 = bcast(r, 32);
+        if (lane == l0) l = r0;
+        if (lane == l1) l = r1;
+        fm = 0;
+    }
+    for (uint64_t m = fm; m; m &= m - 1) {
         const int ld = __builtin_ctzll(m);
         const int xa = bcast(d, ld) > 0 ? n - 1 - bcast(i, ld) : bcast(i, ld);
         const int r = wave_search(delta, bcast(ki, ld), bcast(i, ld), bcast(d, ld),
@@ -334,7 +382,18 @@ __device__ __forceinline__ Split karras_node(const Delta &delta, int i, int n, b
             }
         }
     }
-    for (uint64_t m = __ballot(far2); m; m &= m - 1) {
+    uint64_t fm2 = __ballot(far2);
+    if (__builtin_popcountll(fm2) == 2) {
+        const int l0 = __builtin_ctzll(fm2), l1 = __builtin_ctzll(fm2 & (fm2 - 1));
+        const int ld = lane < 32 ? l0 : l1;
+        const int r = wave_search2(delta, shfl_key(ki, ld), __shfl(i, ld), __shfl(d, ld),
+                                   __shfl(delta_node, ld), __shfl(xw, ld), __shfl(l, ld));
+        const int r0 = bcast(r, 0), r1 = bcast(r, 32);
+        if (lane == l0) s = r0;
+        if (lane == l1) s = r1;
+        fm2 = 0;
+    }
+    for (uint64_t m = fm2; m; m &= m - 1) {
         const int ld = __builtin_ctzll(m);
         const int r = wave_search(delta, bcast(ki, ld), bcast(i, ld), bcast(d, ld),
                                   bcast(delta_node, ld), bcast(xw, ld), bcast(l, ld));
