@@ -961,18 +961,23 @@ __device__ __forceinline__ void setup_entry(const TreeBufs &tb, ListLds &ls,
         ls.in[t][2 * side] = tb.pcm[kPairStride * (size_t)p + side];
         ls.in[t][2 * side + 1] = tb.paabb[2 * (size_t)p + side];
     }
+    // the parent's range and slot and the children's ranges in one round
+    // trip (an index outside the internal nodes reads node 0, unused)
     const uint32_t par = pw & kParentMask;
+    const bool par_in = par < nint;
+    const uint2 rpar = tb.range[par_in ? par : 0u];
+    const uint32_t spar = tb.slot[par_in ? par : 0u];
+    const uint2 rch[2] = {tb.range[ch.x < nint ? ch.x : 0u], tb.range[ch.y < nint ? ch.y : 0u]};
     uint32_t q = kNone;
     // a top node's parent is a top node; a phase-2 node's parent is listed
     // with it unless it spans super-blocks
-    if (par < nint && (shift == kSuperShift || !spans(tb.range[par], kSuperShift)))
-        q = tb.slot[par];
+    if (par_in && (shift == kSuperShift || !spans(rpar, kSuperShift))) q = spar;
     uint32_t pend = 0u, src = 0u;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const uint32_t c = side ? ch.y : ch.x;
         if (c >= nint) continue;
-        const uint2 rc = tb.range[c];
+        const uint2 rc = rch[side];
         if (spans(rc, shift))
             ++pend;
         else if (shift == kSuperShift && spans(rc, kBlockShift))
@@ -1001,17 +1006,23 @@ __device__ __forceinline__ void setup_part_entry(const TreeBufs &tb, ListLds &ls
     const uint32_t pw = tb.parent[p];
     const uint2 ch = tb.child[p];
     const uint32_t par = pw & kParentMask;
+    // the parent's range and slot and the children's ranges in one round
+    // trip (an index outside the internal nodes reads node 0, unused)
+    const bool par_in = par < nint;
+    const uint2 rpar = tb.range[par_in ? par : 0u];
+    const uint32_t spar = tb.slot[par_in ? par : 0u];
+    const uint2 rch[2] = {tb.range[ch.x < nint ? ch.x : 0u], tb.range[ch.y < nint ? ch.y : 0u]};
     uint32_t q = kNone;
     // the parent, a top node, is in the part when it lies in the same
     // hyper-block as the node, or when both span hyper-blocks
-    if (par < nint && (cross || !spans(tb.range[par], kHyperShift))) q = tb.slot[par];
+    if (par_in && (cross || !spans(rpar, kHyperShift))) q = spar;
     uint32_t pend = 0u;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const uint32_t c = side ? ch.y : ch.x;
         bool entry = false, mid = false;
         if (c < nint) {
-            const uint2 rc = tb.range[c];
+            const uint2 rc = rch[side];
             if (spans(rc, kSuperShift))
                 entry = !cross || spans(rc, kHyperShift);
             else
