@@ -612,19 +612,29 @@ __global__ __launch_bounds__(kFillBlock) void fill_local_kernel(const uint32_t *
     const uint32_t t = base + j;
     s_arrive[j] = 0u;
     if (j == 0) s_nstart = 0u;
+    // every load before the first LDS write of a loaded value: one memory
+    // round trip
+    // (the leaf's first: the compiler moves the start test below up to the
+    // node's range, and its wait then comes after every load is issued)
+    uint32_t pw_leaf = kNoNode;
+    float4 leaf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (t < (uint64_t)n) {
+        pw_leaf = n > 1 ? tb.parent[nint + t] : kNoNode;
+        // the build placed the leaf records; a single body has no build
+        leaf = n == 1 ? gather_leaf(tb, idx, mass, pos, 0) : tb.leaf[t];
+    }
     uint2 ch = make_uint2(0u, 0u), rg = make_uint2(0u, 0u);
+    uint32_t par_int = 0u;
     if (t < nint) {
         ch = tb.child[t];
         rg = tb.range[t];
+        par_int = tb.parent[t];
+    }
+    if (t < nint) {
         s_child[j] = ch;
-        s_par_int[j] = tb.parent[t];
+        s_par_int[j] = par_int;
     }
-    uint32_t pw_leaf = kNoNode;
-    if (t < (uint64_t)n) {
-        pw_leaf = n > 1 ? tb.parent[nint + t] : kNoNode;
-        // the sort gathered the leaf records; a single body has no sort
-        s_cm[kFillBlock + j] = n == 1 ? gather_leaf(tb, idx, mass, pos, 0) : tb.leaf[t];
-    }
+    if (t < (uint64_t)n) s_cm[kFillBlock + j] = leaf;
     __syncthreads();
 
     // lbvh.wgsl:191-232, restricted to the nodes whose leaf range lies in
