@@ -283,7 +283,12 @@ __global__ __launch_bounds__(kEncodeThreads) void encode_group_kernel(
         const int64_t i = base + wid * (64 * kItems) + it * 64 + lane;
         p[it] = i < n ? pos[i] : make_float2(0.0f, 0.0f);
     }
-    for (int j = tid; j < nb - 1; j += kEncodeThreads) s_split[j] = split[j];
+    // the splitters (one per thread: nb - 1 < kMaxBuckets <= kEncodeThreads)
+    // are read into registers and written to LDS after the box's loads are
+    // issued: one memory round trip for positions, splitters and box
+    static_assert(kMaxBuckets <= kEncodeThreads, "one splitter per thread");
+    const bool has_split = tid < nb - 1;
+    const uint32_t sv = has_split ? split[tid] : 0u;
     for (int j = tid; j < kWaves * D; j += kEncodeThreads) s_wcnt[j] = 0u;
     float4 box = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if constexpr (kMode == 1) {
@@ -294,6 +299,7 @@ __global__ __launch_bounds__(kEncodeThreads) void encode_group_kernel(
             box = bbox[0];
         }
     }
+    if (has_split) s_split[tid] = sv;
     __syncthreads();
     const int ns = nb - 1;
     int top = 1;
