@@ -124,11 +124,9 @@ __device__ __forceinline__ void fold_bbox(uint32_t *slots, BoxAcc b, uint32_t wa
 // (ax, ay) (barnes_hut.wgsl:110, 120), written by body index; returns the
 // drifted position.
 template <bool kExact>
-__device__ __forceinline__ float2 kick_drift(const WalkArgs &a, uint32_t t, float px, float py,
-                                             float ax, float ay) {
-    const uint32_t bi = a.idx[t];
+__device__ __forceinline__ float2 kick_drift_v(const WalkArgs &a, uint32_t bi, float2 v, float px,
+                                               float py, float ax, float ay) {
     const float dt = a.p.dt;
-    float2 v = a.vel[bi];
     float2 x;
     if constexpr (kExact) {
         v.x = v.x + ax * dt;  // barnes_hut.wgsl:110
@@ -145,6 +143,12 @@ __device__ __forceinline__ float2 kick_drift(const WalkArgs &a, uint32_t t, floa
     a.pos[bi] = x;
     if (a.accel) a.accel[bi] = make_float2(ax, ay);
     return x;
+}
+template <bool kExact>
+__device__ __forceinline__ float2 kick_drift(const WalkArgs &a, uint32_t t, float px, float py,
+                                             float ax, float ay) {
+    const uint32_t bi = a.idx[t];
+    return kick_drift_v<kExact>(a, bi, a.vel[bi], px, py, ax, ay);
 }
 
 // One interaction of body (px, py, m1) with node record c.  Returns true
@@ -553,8 +557,12 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
     const uint32_t leaf0 = n - 1u;
     const uint32_t my_leaf = leaf0 + t;
     float px = 0.0f, py = 0.0f, m1 = 0.0f;
+    uint32_t bi = 0u;  // the body, and its velocity for the epilogue, read
+    float2 v0 = make_float2(0.0f, 0.0f);  // now: their latency hides behind the walk
     if (valid) {
-        const float4 me = a.leaf[t];  // written by the fill: {pos, mass, 0}
+        const float4 me = a.leaf[t];  // written by the build: {pos, mass, 0}
+        bi = a.idx[t];
+        v0 = a.vel[bi];
         px = me.x;
         py = me.y;
         m1 = me.z;
@@ -776,8 +784,8 @@ __device__ __forceinline__ void pair_walk_chunk(const WalkArgs &a, uint32_t chun
 
     BoxAcc box;
     if (valid)
-        box.add(kick_drift<false>(a, t, px, py, accel_out<false>(ax, p.grav),
-                                  accel_out<false>(ay, p.grav)));
+        box.add(kick_drift_v<false>(a, bi, v0, px, py, accel_out<false>(ax, p.grav),
+                                    accel_out<false>(ay, p.grav)));
     fold_bbox(a.bbox_slots, box, t >> 6);
     if constexpr (kStats) {
         // the counters are wave-uniform: one atomic per wave and counter
