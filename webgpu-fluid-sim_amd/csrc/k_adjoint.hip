@@ -170,12 +170,14 @@ __global__ __launch_bounds__(256) void adjoint_pair_kernel(const AdjointArgs a) 
     const uint32_t my_leaf = leaf0 + t;
     float px = 0.0f, py = 0.0f, gx = 0.0f, gy = 0.0f, hx_sum = 0.0f, hy_sum = 0.0f;
     uint32_t bi = 0;
+    float2 lx0 = make_float2(0.0f, 0.0f);  // the epilogue's input, read now
     if (valid) {
         const float4 me = a.leaf[t];
         px = me.x;
         py = me.y;
         bi = a.idx[t];
         const float2 g = a.la[bi];
+        lx0 = a.lx[bi];
         gx = g.x;
         gy = g.y;
     }
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(256) void adjoint_pair_kernel(const AdjointArgs a) 
         }
     }
     if (valid) {
-        float2 l = a.lx[bi];
+        float2 l = lx0;
         l.x -= hx_sum;
         l.y -= hy_sum;
         a.lx[bi] = l;
