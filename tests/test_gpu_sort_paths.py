@@ -45,10 +45,11 @@ def same(a, b):
             assert np.array_equal(u, v), (c, name)
 
 
-# bucket spacings: 1024 positions up to 2^18 keys (encode tiles of 1024
-# bodies below 2^18, 4096 from there), 2048 up to 2^19, 4096 above
+# bucket spacings (992, 1984 or 3968 positions, the narrowest giving at most
+# 256 buckets, then 3968 up to 1024 buckets) on either side of each change,
+# with the encode's 1024-body tiles below 2^18 keys and 4096 from there
 @pytest.mark.parametrize("n", [2, 3, 1000, 4096, 4097, 16384, 4096 * 5 + 1, 50000, 131079,
-                               262144, 262145, 524288, 524289])
+                               253952, 253953, 262143, 262144, 507904, 507905, 524288])
 def test_sort_paths_identical(n):
     bodies = nbody_sim.generate("exp-disk", n, seed=7)
     cfg, _ = params(theta=0.7, dt=5e-5)

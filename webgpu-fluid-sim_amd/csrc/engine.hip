@@ -439,7 +439,7 @@ int enqueue_step(nbody_sim *s, bool tree_only = false) {
     const int64_t clear = sort_clear_words(n, s->key_bits, s->radix_bits, s->ensemble);
     // the build of a bucket-sortable size writes the next sort's splitters
     const bool splits = s->split && n > kSortLdsMax && n <= bucket_sort_max();
-    const LeafPlace lpl{s->pos, s->mass, s->rank, splits ? s->split : nullptr, bucket_shift(n)};
+    const LeafPlace lpl{s->pos, s->mass, s->rank, splits ? s->split : nullptr, bucket_spacing(n)};
     if (s->ensemble)
         launch_build_u64((const uint64_t *)s->keys[s->result_buf], n, s->sim_off, s->tree, lpl,
                          s->sim_root, s->sort.hist, clear, nullptr, st);

@@ -452,7 +452,7 @@ void launch_encode_group_rb(const float2 *pos, int64_t n, int morton_mode, float
 // Tiles of 1024 bodies below 2^18 (a small sort's few tiles would leave most
 // CUs idle: 65 536 bodies, encode 12.3 -> 9.1 us; 131 072, 12.8 -> 10.5; no
 // gain at 2^18), 4096 above.  Below 2^18 keys there are at most 256 buckets
-// (bucket_shift), which the 1024-body tiles' 8-bit bucket ranking needs.
+// (bucket_spacing), which the 1024-body tiles' 8-bit bucket ranking needs.
 int encode_group_tile(int64_t n) { return n < (int64_t(1) << 18) ? 1024 : 4096; }
 int encode_group_tiles(int64_t n) {
     const int t = encode_group_tile(n);

@@ -327,7 +327,7 @@ int sort_impl(K *keys_a, uint32_t *vals_a, K *keys_b, uint32_t *vals_b, int64_t 
 // ---------------------------------------------------------------- bucket sort
 // Bodies move little between steps, so the previous step's sorted keys split
 // the next step's keys into nearly equal buckets: splitter j (j = 1 .. B-1)
-// is the key at sorted position j * S (S = 1024, 2048 or 4096: bucket_shift).
+// is the key at sorted position j * S (S about 1024, 2048 or 4096: bucket_spacing).
 // A body's bucket is the number of splitters <= its key (the encode finds
 // it, so equal keys share a bucket, as does every key of a bucket's range).
 // The encode groups each tile's bodies by bucket (encode_group_kernel);

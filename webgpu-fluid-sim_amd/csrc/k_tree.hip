@@ -472,10 +472,10 @@ __global__ __launch_bounds__(kBuildThreads) void build_u32_kernel(
         tb.leaf[at] = leaf;
         if (i == n32 - 2) place_leaf(tb, lpl, n32 - 1);
         if (lpl.split) {  // the splitters of the next step's bucket sort
-            const int sh = lpl.split_shift, mask = (1 << sh) - 1;
-            if (i > 0 && (i & mask) == 0) lpl.split[(i >> sh) - 1] = keys[i];
-            if (i == n32 - 2 && ((n32 - 1) & mask) == 0)
-                lpl.split[((n32 - 1) >> sh) - 1] = keys[n32 - 1];
+            const int every = lpl.split_every;
+            if (i > 0 && i % every == 0) lpl.split[i / every - 1] = keys[i];
+            if (i == n32 - 2 && (n32 - 1) % every == 0)
+                lpl.split[(n32 - 1) / every - 1] = keys[n32 - 1];
         }
         list_node(tb, (uint32_t)i, sp, n32);
         tb.child[i] = make_uint2(sp.lc, sp.rc);

@@ -172,17 +172,26 @@ constexpr int64_t kSortLdsMax = 4096;
 constexpr int kBucketShiftMax = 12;  // widest splitter spacing: 4096 sorted positions
 constexpr int kMaxBuckets = 1024;
 constexpr int kBucketCap = 6144;     // largest bucket sorted in LDS (larger: through memory)
-// log2 of the splitter spacing for n keys: the narrowest of 1024, 2048 and
-// 4096 positions that makes at most 256 buckets (small sorts want many
-// workgroups, large ones few passes per key)
-__host__ __device__ inline int bucket_shift(int64_t n) {
-    for (int sh = 10; sh < kBucketShiftMax; ++sh)
-        if (((n + (int64_t(1) << sh) - 1) >> sh) <= 256) return sh;
-    return kBucketShiftMax;
+// The splitter spacing for n keys: 2^k less 1/32 of it for k = 10 or 11,
+// the narrowest that makes at most 256 buckets (small sorts want many
+// workgroups), else 4096 less 1/32 while that makes at most kMaxBuckets,
+// else 4096.  A bucket holds about that many keys, give or take the few
+// bodies that crossed a splitter since the last step, and the 1/32 keeps it
+// within 2^k, the size the per-bucket sort handles in whole rows of 1024
+// (one more key would cost a whole row more).  Below 2^18 keys the spacing
+// is at most 1984, so at most 256 buckets (the encode's 1024-body tiles rank
+// 8 bits of bucket id).
+__host__ __device__ inline int bucket_spacing(int64_t n) {
+    for (int sh = 10; sh < kBucketShiftMax; ++sh) {
+        const int s = (1 << sh) - (1 << sh) / 32;
+        if ((n + s - 1) / s <= 256) return s;
+    }
+    const int full = 1 << kBucketShiftMax, s = full - full / 32;
+    return (n + s - 1) / s <= kMaxBuckets ? s : full;
 }
 __host__ __device__ inline int sort_buckets(int64_t n) {
-    const int sh = bucket_shift(n);
-    return (int)((n + (int64_t(1) << sh) - 1) >> sh);
+    const int s = bucket_spacing(n);
+    return (int)((n + s - 1) / s);
 }
 uint32_t sort_key_max(int key_bits);
 int launch_sort_small(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b,
@@ -256,8 +265,8 @@ struct LeafPlace {
     const float2 *pos;
     const float *mass;
     const uint32_t *rank;
-    uint32_t *split = nullptr;  // u32 builds: the next sort's splitters (keys at j << split_shift)
-    int split_shift = kBucketShiftMax;
+    uint32_t *split = nullptr;  // u32 builds: the next sort's splitters (keys at j * split_every)
+    int split_every = 1 << kBucketShiftMax;
 };
 void launch_build_u32(const uint32_t *keys, int64_t n, const TreeBufs &t, const LeafPlace &lp,
                       uint32_t *clear, int64_t clear_words, uint32_t *slots, hipStream_t s);
